@@ -1,0 +1,88 @@
+"""ctypes binding of libgpk's C ABI (``include/gpk.h``).
+
+The library is required: importing the compute path raises if ``libgpk.so``
+has not been built, and there is no host fallback.  Device memory and streams
+come from PyTorch (``tensor.data_ptr()``, ``torch.cuda.current_stream()``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).with_name("libgpk.so")
+
+GPK_OK = 0
+STATUS = {0: "OK", 1: "BAD_ARG", 2: "NOT_PD", 3: "NO_DEVICE", 4: "HIP", 5: "WORKSPACE"}
+KERNEL_IDS = {"rbf": 0, "matern12": 1, "matern32": 2, "matern52": 3}
+PRECISIONS = {"fp64": 0, "mixed": 1}
+RESULT_LEN = 8
+RES_MLL, RES_LOGDET, RES_QUAD, RES_RESID, RES_INFO = 0, 1, 2, 3, 4
+
+_c = ctypes
+_P = _c.c_void_p
+_I = _c.c_int
+_L = _c.c_int64
+_D = _c.c_double
+_S = _c.c_size_t
+
+# name -> (restype, argtypes); mirrors include/gpk.h
+SIGNATURES = {
+    "gpk_version": (_c.c_char_p, []),
+    "gpk_last_error": (_c.c_char_p, []),
+    "gpk_cross_covariance": (_I, [_I, _P, _L, _P, _L, _L, _P, _I, _P, _L, _P]),
+    "gpk_gram": (_I, [_I, _P, _L, _L, _P, _I, _D, _I, _P, _L, _P]),
+    "gpk_mll_workspace_bytes": (_S, [_L, _L, _I]),
+    "gpk_mll_forward": (_I, [_I, _I, _P, _P, _L, _L, _P, _I, _D, _I, _P, _S, _P, _P]),
+    "gpk_mll_backward": (_I, [_I, _I, _P, _L, _L, _P, _I, _D, _P, _S, _P, _P]),
+    "gpk_set_mixed_arith": (_I, [_I, _I, _I]),
+    "gpk_mfma_selftest": (_I, [_P, _P, _P]),
+}
+
+
+class GPKError(RuntimeError):
+    """A libgpk call returned a non-OK status."""
+
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed with {STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libgpk.so (once) and declare its signatures."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python gpjax_amd/csrc/build.py` "
+                "(or __graft_entry__.build()); gpjax_amd has no host fallback")
+        handle = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def call(name: str, *args) -> int:
+    """Call a status-returning entry point; raise GPKError on failure."""
+    rc = getattr(lib(), name)(*args)
+    if rc != GPK_OK:
+        raise GPKError(name, rc, lib().gpk_last_error().decode())
+    return rc
+
+
+def ptr(t) -> int:
+    """Device address of a contiguous torch tensor."""
+    if not t.is_contiguous():
+        raise ValueError("libgpk needs contiguous tensors")
+    return t.data_ptr()
+
+
+def stream_ptr(device) -> int:
+    import torch
+    return torch.cuda.current_stream(device).cuda_stream

@@ -1,0 +1,195 @@
+// Fused LAUUM + adjoint trace pass for d MLL / d theta.
+//
+// The reference obtains the gradient from jax.grad through the LU slogdet and
+// solve (gpjax/objectives.py:96-107, gpjax/linalg/operations.py:111,164; used as
+// in tests/test_objectives.py:83-85 and gpjax/fit.py:160).  That derivative is
+//   d MLL / d theta = 1/2 tr(W dSigma/dtheta),   W = alpha alpha^T - Sigma^-1.
+// Here every NB-tile (I, J), I >= J, of Sigma^-1 = X^T X (X = L^-1, lower) is
+// accumulated on the matrix cores over k >= I*NB, and in the epilogue of the
+// same workgroup W_ij is combined with the kernel tile recomputed from the
+// inputs:
+//   S_var  += wt * W_ij * k(rho_ij)
+//   S_ls_d += wt * W_ij * F(rho_ij) * u_d,    u_d = (x_id - x_jd)^2 / l_d^2
+//   S_tr   += W_ii
+// with wt = 2 off the diagonal tile (symmetry) and 1 on it.  Sigma^-1 and
+// dK/dtheta are never written to memory.  Each tile writes its nls + 2 sums to
+// `partial`; a deterministic reduction follows.
+#include "gemm_core.h"
+
+namespace gpk {
+namespace {
+
+template <int AR, typename TS, int DMAX>
+__global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __restrict__ X,
+                                                      int64_t ld, int64_t np,
+                                                      const double* __restrict__ x, int64_t n,
+                                                      int d, const double* __restrict__ theta,
+                                                      int nls, const double* __restrict__ alpha,
+                                                      double* __restrict__ partial) {
+  typedef typename AccT<AR>::scalar AS;
+  constexpr size_t ML = smem_bytes<AR>();
+  constexpr size_t XS = (size_t)2 * TILE * (DMAX + 1) * sizeof(double);
+  constexpr size_t AB = (size_t)2 * TILE * sizeof(double);
+  constexpr size_t WB = (size_t)4 * 16 * 64 * sizeof(AS);
+  constexpr size_t EP = XS + AB + WB;
+  __shared__ __align__(16) char smem[ML > EP ? ML : EP];
+  __shared__ double red_s[4][DMAX + 2];
+
+  int64_t mt, nt;
+  lower_tile(blockIdx.x, mt, nt);
+  const int64_t m0 = mt * TILE, n0 = nt * TILE;
+  typename AccT<AR>::vec acc[4][4];
+  if constexpr (AR == AR_F64) {
+    mainloop_f64<TS, TS, true, true>(X, ld, X, ld, m0, n0, m0, np, acc, smem);
+  } else {
+    mainloop_bf16<AR, true, true>(X, ld, X, ld, m0, n0, m0, np, acc, smem);
+  }
+  __syncthreads();
+
+  // Epilogue staging: scaled inputs and alpha of the tile's rows (i) and
+  // columns (j), and a per-lane strip through which each lane re-reads its own
+  // accumulators with a rolled loop.
+  double* xm = reinterpret_cast<double*>(smem);        // [TILE][DMAX+1]
+  double* xn = xm + TILE * (DMAX + 1);                 // [TILE][DMAX+1]
+  double* am = xn + TILE * (DMAX + 1);                 // [TILE]
+  double* an = am + TILE;                              // [TILE]
+  AS* wbuf = reinterpret_cast<AS*>(an + TILE);         // [4 waves][16][64 lanes]
+  const int t = threadIdx.x;
+  for (int e = t; e < TILE * DMAX; e += 256) {
+    const int r = e / DMAX, q = e % DMAX;
+    if (q < d) {
+      const double l = theta[nls == 1 ? 0 : q];
+      const int64_t gi = m0 + r, gj = n0 + r;
+      xm[r * (DMAX + 1) + q] = gi < n ? x[gi * d + q] / l : 0.0;
+      xn[r * (DMAX + 1) + q] = gj < n ? x[gj * d + q] / l : 0.0;
+    }
+  }
+  if (t < TILE) {
+    am[t] = alpha[m0 + t];
+    an[t] = alpha[n0 + t];
+  }
+  __syncthreads();
+
+  const int lane = t & 63, w = t >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const double wt = (mt == nt) ? 1.0 : 2.0;
+  const bool ard = nls > 1;
+  double s_var = 0.0, s_tr = 0.0;
+  double s_ls[DMAX];
+#pragma unroll
+  for (int q = 0; q < DMAX; ++q) s_ls[q] = 0.0;
+  AS* mine = wbuf + w * 16 * 64 + lane;
+
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    // Only this lane reads these 16 slots back, so no barrier is needed.
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) mine[(j * 4 + r) * 64] = acc[i][j][r];
+#pragma unroll 1
+    for (int e = 0; e < 16; ++e) {
+      const int j = e >> 2, r = e & 3;
+      const int li = wm * 64 + i * 16 + 4 * (lane >> 4) + r;
+      const int lj = wn * 64 + j * 16 + (lane & 15);
+      const int64_t gi = m0 + li, gj = n0 + lj;
+      if (gi >= n || gj >= n) continue;
+      const double wij = am[li] * an[lj] - (double)mine[e * 64];
+      const double* pi = xm + li * (DMAX + 1);
+      const double* pj = xn + lj * (DMAX + 1);
+      double d2 = 0.0;
+#pragma unroll
+      for (int q = 0; q < DMAX; ++q) {
+        if (q < d) {
+          const double df = pi[q] - pj[q];
+          d2 += df * df;
+        }
+      }
+      double kv, fv;
+      kval_fac(kind, d2, kv, fv);
+      const double g = wt * wij;
+      s_var += g * kv;
+      if (gi == gj) s_tr += wij;
+      const double gf = g * fv;
+      if (ard) {
+#pragma unroll
+        for (int q = 0; q < DMAX; ++q) {
+          if (q < d) {
+            const double df = pi[q] - pj[q];
+            s_ls[q] += gf * df * df;
+          }
+        }
+      } else {
+        s_ls[0] += gf * d2;
+      }
+    }
+  }
+
+  // Workgroup reduction: wave shuffles, then across the four waves.
+  const int nv = nls + 2;
+#pragma unroll
+  for (int q = 0; q < DMAX + 2; ++q) {
+    double v = q < DMAX ? s_ls[q] : (q == DMAX ? s_var : s_tr);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) red_s[w][q] = v;
+  }
+  __syncthreads();
+  if (t < nv) {
+    const int q = t < nls ? t : (t == nls ? DMAX : DMAX + 1);
+    const double v = red_s[0][q] + red_s[1][q] + red_s[2][q] + red_s[3][q];
+    partial[(int64_t)blockIdx.x * nv + t] = v;
+  }
+}
+
+template <int AR, typename TS>
+int launch_ar(int kind, const TS* X, int64_t ld, int64_t np, const double* x, int64_t n, int d,
+              const double* theta, int nls, const double* alpha, double* partial,
+              hipStream_t s) {
+  const int64_t nt = np / TILE;
+  dim3 grid((unsigned)(nt * (nt + 1) / 2));
+#define GPK_ADJ(DM)                                                                        \
+  hipLaunchKernelGGL((adjoint_kernel<AR, TS, DM>), grid, dim3(256), 0, s, kind, X, ld, np, x, \
+                     n, d, theta, nls, alpha, partial)
+  if (d <= 1) GPK_ADJ(1);
+  else if (d <= 2) GPK_ADJ(2);
+  else if (d <= 4) GPK_ADJ(4);
+  else if (d <= 8) GPK_ADJ(8);
+  else if (d <= 16) GPK_ADJ(16);
+  else GPK_ADJ(32);
+#undef GPK_ADJ
+  GPK_LAUNCH_CHECK();
+  return GPK_OK;
+}
+
+}  // namespace
+
+template <typename TS>
+int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, const double* x,
+                   int64_t n, int d, const double* theta, int nls, const double* alpha,
+                   double* partial, hipStream_t s) {
+  if (d < 1 || d > MAX_D) {
+    set_error("adjoint: input dimension %d outside [1, %d]", d, MAX_D);
+    return GPK_ERR_BAD_ARG;
+  }
+  if (np % TILE) {
+    set_error("adjoint: padded size %lld not a multiple of %d", (long long)np, TILE);
+    return GPK_ERR_BAD_ARG;
+  }
+  if (arith == AR_F64) return launch_ar<AR_F64, TS>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s);
+  if constexpr (std::is_same<TS, float>::value) {
+    if (arith == AR_BF16X3) return launch_ar<AR_BF16X3, float>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s);
+    if (arith == AR_BF16X6) return launch_ar<AR_BF16X6, float>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s);
+  }
+  set_error("adjoint: unsupported arithmetic %d", arith);
+  return GPK_ERR_BAD_ARG;
+}
+
+template int launch_adjoint<double>(int, int, const double*, int64_t, int64_t, const double*,
+                                    int64_t, int, const double*, int, const double*, double*,
+                                    hipStream_t);
+template int launch_adjoint<float>(int, int, const float*, int64_t, int64_t, const double*,
+                                   int64_t, int, const double*, int, const double*, double*,
+                                   hipStream_t);
+
+}  // namespace gpk

@@ -1,0 +1,446 @@
+// C ABI of libgpk and the host-side orchestration of the exact-GP path.
+//
+// Forward (gpk_mll_forward), replacing conjugate_mll (gpjax/objectives.py:93-107):
+//   1. Sigma = K + (jitter + obs_sd^2) I into L (lower NB-tiles), identity pad.
+//   2. Recursive Cholesky:  potrf(A11); L21 = A21 L11^-T (recursive TRSM whose
+//      leaves multiply by the float64 leaf inverses); A22 -= L21 L21^T (SYRK on
+//      the matrix cores); potrf(A22).  Leaves factor NB x NB blocks in float64
+//      and also emit their inverses and log-diagonal sums.
+//   3. Recursive triangular inverse X = L^-1:  X21 = -X22 (L21 X11).
+//   4. alpha = X^T X r, then float64 iterative refinement against Sigma
+//      recomputed from the inputs: alpha += X^T X (r - Sigma alpha).
+//   5. logdet = sum of leaf log-diagonals, quad = r . alpha, mll.
+// Backward (gpk_mll_backward): the fused LAUUM + adjoint tile pass and a
+// deterministic reduction to the gradient.
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <type_traits>
+
+#include "gpk_internal.h"
+
+namespace gpk {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+namespace {
+
+constexpr size_t ALIGN = 256;
+inline size_t al(size_t b) { return (b + ALIGN - 1) / ALIGN * ALIGN; }
+
+// Arithmetic of the three product stages in the mixed path.
+int g_ar_potrf = AR_BF16X6;
+int g_ar_trtri = AR_BF16X3;
+int g_ar_lauum = AR_BF16X6;
+
+struct Layout {
+  int64_t np = 0;
+  int64_t htop = 0;
+  size_t ts = 0;
+  size_t off_L = 0, off_X = 0, off_T = 0, off_leafinv = 0, off_leafld = 0, off_leafinfo = 0;
+  size_t off_r = 0, off_alpha = 0, off_u = 0, off_v = 0, off_res = 0, off_delta = 0;
+  size_t off_part = 0, off_gpart = 0, off_scal = 0, total = 0;
+};
+
+int64_t split_point(int64_t n) { return (n / NB / 2) * NB; }
+
+Layout make_layout(int64_t n, int64_t d, int prec) {
+  Layout L;
+  L.np = round_up(n, 256);
+  L.ts = prec == GPK_PREC_FP64 ? sizeof(double) : sizeof(float);
+  const int64_t np = L.np;
+  L.htop = np > NB ? split_point(np) : 0;
+  const int64_t nleaf = np / NB;
+  const int64_t ntile = nleaf * (nleaf + 1) / 2;
+  const int64_t nchunk = (np + 1023) / 1024;
+  size_t o = 0;
+  L.off_L = o; o += al((size_t)np * np * L.ts);
+  L.off_X = o; o += al((size_t)np * np * L.ts);
+  L.off_T = o; o += al((size_t)(np - L.htop) * (L.htop > 0 ? L.htop : 1) * L.ts);
+  L.off_leafinv = o; o += al((size_t)np * NB * sizeof(double));
+  L.off_leafld = o; o += al((size_t)nleaf * sizeof(double));
+  L.off_leafinfo = o; o += al((size_t)nleaf * sizeof(int));
+  L.off_r = o; o += al((size_t)np * sizeof(double));
+  L.off_alpha = o; o += al((size_t)np * sizeof(double));
+  L.off_u = o; o += al((size_t)np * sizeof(double));
+  L.off_v = o; o += al((size_t)np * sizeof(double));
+  L.off_res = o; o += al((size_t)np * sizeof(double));
+  L.off_delta = o; o += al((size_t)np * sizeof(double));
+  L.off_part = o; o += al((size_t)nchunk * np * sizeof(double));
+  L.off_gpart = o; o += al((size_t)ntile * (d + 2) * sizeof(double));
+  L.off_scal = o; o += al(16 * sizeof(double));
+  L.total = o;
+  return L;
+}
+
+template <typename T>
+T* at(void* ws, size_t off) { return reinterpret_cast<T*>(static_cast<char*>(ws) + off); }
+
+#define GPK_TRY(...)           \
+  do {                         \
+    int rc_ = (__VA_ARGS__);   \
+    if (rc_ != GPK_OK) return rc_; \
+  } while (0)
+
+// Recursive factorisation driver over one matrix buffer.
+template <typename TS>
+struct Factor {
+  TS* L;
+  TS* X;
+  int64_t ld;
+  TS* T;
+  double* leafinv;
+  double* leafld;
+  int* leafinfo;
+  int ar_potrf, ar_trtri;
+  hipStream_t s;
+
+  int leaf(int64_t o) {
+    return launch_leaf<TS>(L, X, ld, o, leafinv + (o / NB) * NB * NB, leafld + o / NB,
+                           leafinfo + o / NB, s);
+  }
+  // Solve Y L^T = B in place, B = L[row0 : row0+m, o : o+n], triangle L[o:o+n, o:o+n].
+  int trsm(int64_t row0, int64_t m, int64_t o, int64_t n) {
+    if (n == NB) {
+      const double* inv = leafinv + (o / NB) * NB * NB;
+      return launch_gemm<TS, double, TS>(AR_F64, false, false, L + row0 * ld + o, ld, inv, NB,
+                                         L + row0 * ld + o, ld, m, NB, NB, 1.0, false, false,
+                                         KR_FULL, s);
+    }
+    const int64_t h = split_point(n);
+    GPK_TRY(trsm(row0, m, o, h));
+    GPK_TRY(launch_gemm<TS, TS, TS>(ar_potrf, false, false, L + row0 * ld + o, ld,
+                                    L + (o + h) * ld + o, ld, L + row0 * ld + o + h, ld, m,
+                                    n - h, h, -1.0, true, false, KR_FULL, s));
+    return trsm(row0, m, o + h, n - h);
+  }
+  int potrf(int64_t o, int64_t n) {
+    if (n == NB) return leaf(o);
+    const int64_t h = split_point(n);
+    GPK_TRY(potrf(o, h));
+    GPK_TRY(trsm(o + h, n - h, o, h));
+    GPK_TRY(launch_gemm<TS, TS, TS>(ar_potrf, false, false, L + (o + h) * ld + o, ld,
+                                    L + (o + h) * ld + o, ld, L + (o + h) * ld + o + h, ld,
+                                    n - h, n - h, h, -1.0, true, true, KR_FULL, s));
+    return potrf(o + h, n - h);
+  }
+  // X = L^-1 below the leaves, whose diagonal blocks the leaves already wrote.
+  int trtri(int64_t o, int64_t n) {
+    if (n == NB) return GPK_OK;
+    const int64_t h = split_point(n);
+    GPK_TRY(trtri(o, h));
+    GPK_TRY(trtri(o + h, n - h));
+    // T = L21 X11   ((n-h) x h; X11 lower, so k >= column tile)
+    GPK_TRY(launch_gemm<TS, TS, TS>(ar_trtri, false, true, L + (o + h) * ld + o, ld,
+                                    X + o * ld + o, ld, T, h, n - h, h, h, 1.0, false, false,
+                                    KR_GE_N, s));
+    // X21 = -X22 T  (X22 lower, so k <= row tile)
+    return launch_gemm<TS, TS, TS>(ar_trtri, false, true, X + (o + h) * ld + o + h, ld, T, h,
+                                   X + (o + h) * ld + o, ld, n - h, h, n - h, -1.0, false,
+                                   false, KR_LE_M, s);
+  }
+};
+
+// ---- small kernels ----
+__global__ void prep_r_kernel(const double* __restrict__ y, const double* __restrict__ theta,
+                              int nls, int64_t n, int64_t np, double* __restrict__ r) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) r[i] = i < n ? y[i] - theta[nls + 2] : 0.0;
+}
+
+__global__ void sub_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                           int64_t np, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) out[i] = a[i] - b[i];
+}
+
+__global__ void add_kernel(double* __restrict__ a, const double* __restrict__ b, int64_t np) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) a[i] += b[i];
+}
+
+__device__ double block_sum(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  const int nw = blockDim.x >> 6;
+  for (int q = 0; q < nw; ++q) s += red[q];
+  return s;
+}
+
+__global__ __launch_bounds__(256) void finalize_fwd_kernel(
+    const double* __restrict__ leafld, const int* __restrict__ leafinfo, int64_t nleaf,
+    const double* __restrict__ r, const double* __restrict__ alpha,
+    const double* __restrict__ res, int64_t n, int64_t np, double* __restrict__ result) {
+  __shared__ double red[4];
+  double q = 0.0, rr = 0.0, ee = 0.0;
+  for (int64_t i = threadIdx.x; i < np; i += 256) {
+    if (i < n) q += r[i] * alpha[i];
+    rr += r[i] * r[i];
+    ee += res[i] * res[i];
+  }
+  q = block_sum(q, red);
+  rr = block_sum(rr, red);
+  ee = block_sum(ee, red);
+  if (threadIdx.x == 0) {
+    double logdet = 0.0;
+    int info = 0;
+    for (int64_t b = 0; b < nleaf; ++b) {
+      logdet += leafld[b];
+      if (info == 0 && leafinfo[b] != 0) info = leafinfo[b];
+    }
+    const double log2pi = 1.8378770664093453;
+    double mll = -0.5 * ((double)n * log2pi + logdet + q);
+    if (info != 0) mll = __builtin_nan("");
+    result[GPK_RES_MLL] = mll;
+    result[GPK_RES_LOGDET] = logdet;
+    result[GPK_RES_QUAD] = q;
+    result[GPK_RES_RESID] = rr > 0.0 ? sqrt(ee / rr) : sqrt(ee);
+    result[GPK_RES_INFO] = (double)info;
+    for (int k = 5; k < GPK_RESULT_LEN; ++k) result[k] = 0.0;
+  }
+}
+
+// One workgroup per gradient slot: sum the tile partials in a fixed order.
+__global__ __launch_bounds__(256) void finalize_bwd_kernel(
+    const double* __restrict__ gpart, int64_t ntile, int nls, const double* __restrict__ theta,
+    const double* __restrict__ alpha, int64_t n, double gbar, double* __restrict__ grad) {
+  __shared__ double red[4];
+  const int q = blockIdx.x;
+  const int nv = nls + 2;
+  double s = 0.0;
+  if (q < nv) {
+    for (int64_t t = threadIdx.x; t < ntile; t += 256) s += gpart[t * nv + q];
+  } else {
+    for (int64_t i = threadIdx.x; i < n; i += 256) s += alpha[i];
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) {
+    const double var = theta[nls], sd = theta[nls + 1];
+    double g;
+    if (q < nls) g = 0.5 * var * s / theta[q];
+    else if (q == nls) g = 0.5 * s;
+    else if (q == nls + 1) g = sd * s;
+    else g = s;
+    grad[q] = gbar * g;
+  }
+}
+
+inline unsigned nblk(int64_t np) { return (unsigned)((np + 255) / 256); }
+
+int check_common(int kind, int prec, const void* x, int64_t n, int64_t d, const void* theta,
+                 int nls) {
+  if (kind < GPK_KERNEL_RBF || kind > GPK_KERNEL_MATERN52) {
+    set_error("unknown kernel kind %d", kind);
+    return GPK_ERR_BAD_ARG;
+  }
+  if (prec != GPK_PREC_FP64 && prec != GPK_PREC_MIXED) {
+    set_error("unknown precision %d", prec);
+    return GPK_ERR_BAD_ARG;
+  }
+  if (!x || !theta) {
+    set_error("null input pointer");
+    return GPK_ERR_BAD_ARG;
+  }
+  if (n < 1 || d < 1 || d > MAX_D) {
+    set_error("need n >= 1 and 1 <= d <= %d (got n=%lld, d=%lld)", MAX_D, (long long)n,
+              (long long)d);
+    return GPK_ERR_BAD_ARG;
+  }
+  if (nls != 1 && nls != d) {
+    set_error("nls must be 1 or d (got %d, d=%lld)", nls, (long long)d);
+    return GPK_ERR_BAD_ARG;
+  }
+  return GPK_OK;
+}
+
+template <typename TS>
+int mll_forward_t(int kind, int prec, const double* x, const double* y, int64_t n, int d,
+                  const double* theta, int nls, double jitter, int iters, void* ws,
+                  const Layout& Lo, double* result, hipStream_t s) {
+  const int64_t np = Lo.np;
+  TS* L = at<TS>(ws, Lo.off_L);
+  TS* X = at<TS>(ws, Lo.off_X);
+  double* r = at<double>(ws, Lo.off_r);
+  double* alpha = at<double>(ws, Lo.off_alpha);
+  double* u = at<double>(ws, Lo.off_u);
+  double* v = at<double>(ws, Lo.off_v);
+  double* res = at<double>(ws, Lo.off_res);
+  double* delta = at<double>(ws, Lo.off_delta);
+  double* part = at<double>(ws, Lo.off_part);
+  const bool f64 = std::is_same<TS, double>::value;
+  Factor<TS> F{L,
+               X,
+               np,
+               at<TS>(ws, Lo.off_T),
+               at<double>(ws, Lo.off_leafinv),
+               at<double>(ws, Lo.off_leafld),
+               at<int>(ws, Lo.off_leafinfo),
+               f64 ? (int)AR_F64 : g_ar_potrf,
+               f64 ? (int)AR_F64 : g_ar_trtri,
+               s};
+
+  hipLaunchKernelGGL(prep_r_kernel, dim3(nblk(np)), dim3(256), 0, s, y, theta, nls, n, np, r);
+  GPK_LAUNCH_CHECK();
+  GPK_TRY(launch_kernel_matrix<TS>(kind, x, n, x, n, d, theta, nls, jitter, 2, L, np, np, np,
+                                   true, s));
+  GPK_TRY(F.potrf(0, np));
+  GPK_TRY(F.trtri(0, np));
+  // alpha0 = X^T X r
+  GPK_TRY(launch_trmv<TS>(X, np, np, r, u, false, part, s));
+  GPK_TRY(launch_trmv<TS>(X, np, np, u, alpha, true, part, s));
+  for (int it = 0; it < iters; ++it) {
+    GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, alpha, v, part, s));
+    hipLaunchKernelGGL(sub_kernel, dim3(nblk(np)), dim3(256), 0, s, r, v, np, res);
+    GPK_LAUNCH_CHECK();
+    GPK_TRY(launch_trmv<TS>(X, np, np, res, u, false, part, s));
+    GPK_TRY(launch_trmv<TS>(X, np, np, u, delta, true, part, s));
+    hipLaunchKernelGGL(add_kernel, dim3(nblk(np)), dim3(256), 0, s, alpha, delta, np);
+    GPK_LAUNCH_CHECK();
+  }
+  GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, alpha, v, part, s));
+  hipLaunchKernelGGL(sub_kernel, dim3(nblk(np)), dim3(256), 0, s, r, v, np, res);
+  GPK_LAUNCH_CHECK();
+  hipLaunchKernelGGL(finalize_fwd_kernel, dim3(1), dim3(256), 0, s, at<double>(ws, Lo.off_leafld),
+                     at<int>(ws, Lo.off_leafinfo), np / NB, r, alpha, res, n, np, result);
+  GPK_LAUNCH_CHECK();
+  return GPK_OK;
+}
+
+template <typename TS>
+int mll_backward_t(int kind, const double* x, int64_t n, int d, const double* theta, int nls,
+                   double gbar, void* ws, const Layout& Lo, double* grad, hipStream_t s) {
+  const int64_t np = Lo.np;
+  const bool f64 = std::is_same<TS, double>::value;
+  double* gpart = at<double>(ws, Lo.off_gpart);
+  GPK_TRY(launch_adjoint<TS>(kind, f64 ? (int)AR_F64 : g_ar_lauum, at<TS>(ws, Lo.off_X), np, np,
+                             x, n, d, theta, nls, at<double>(ws, Lo.off_alpha), gpart, s));
+  const int64_t nleaf = np / NB;
+  hipLaunchKernelGGL(finalize_bwd_kernel, dim3((unsigned)(nls + 3)), dim3(256), 0, s, gpart,
+                     nleaf * (nleaf + 1) / 2, nls, theta, at<double>(ws, Lo.off_alpha), n, gbar,
+                     grad);
+  GPK_LAUNCH_CHECK();
+  return GPK_OK;
+}
+
+}  // namespace
+}  // namespace gpk
+
+using namespace gpk;
+
+extern "C" {
+
+const char* gpk_version(void) { return "gpk 0.1.0 (gfx950)"; }
+
+const char* gpk_last_error(void) { return g_err; }
+
+int gpk_cross_covariance(int kind, const double* x, int64_t n, const double* z, int64_t m,
+                         int64_t d, const double* theta, int nls, double* out, int64_t ldo,
+                         void* stream) {
+  int rc = check_common(kind, GPK_PREC_FP64, x, n, d, theta, nls);
+  if (rc) return rc;
+  if (!z || !out || m < 1 || ldo < m) {
+    set_error("cross_covariance: bad z/out/m/ldo");
+    return GPK_ERR_BAD_ARG;
+  }
+  return launch_kernel_matrix<double>(kind, x, n, z, m, (int)d, theta, nls, 0.0, 0, out, ldo, n,
+                                      m, false, (hipStream_t)stream);
+}
+
+int gpk_gram(int kind, const double* x, int64_t n, int64_t d, const double* theta, int nls,
+             double jitter, int add_noise, double* out, int64_t ldo, void* stream) {
+  int rc = check_common(kind, GPK_PREC_FP64, x, n, d, theta, nls);
+  if (rc) return rc;
+  if (!out || ldo < n) {
+    set_error("gram: bad out/ldo");
+    return GPK_ERR_BAD_ARG;
+  }
+  return launch_kernel_matrix<double>(kind, x, n, x, n, (int)d, theta, nls, jitter,
+                                      add_noise ? 2 : 1, out, ldo, n, n, false,
+                                      (hipStream_t)stream);
+}
+
+size_t gpk_mll_workspace_bytes(int64_t n, int64_t d, int prec) {
+  if (n < 1 || d < 1) return 0;
+  return make_layout(n, d, prec).total;
+}
+
+int gpk_set_mixed_arith(int potrf_products, int trtri_products, int lauum_products) {
+  auto ok = [](int v) { return v == AR_BF16X3 || v == AR_BF16X6; };
+  if (!ok(potrf_products) || !ok(trtri_products) || !ok(lauum_products)) {
+    set_error("mixed arithmetic: products must be 3 or 6");
+    return GPK_ERR_BAD_ARG;
+  }
+  g_ar_potrf = potrf_products;
+  g_ar_trtri = trtri_products;
+  g_ar_lauum = lauum_products;
+  return GPK_OK;
+}
+
+int gpk_mll_forward(int kind, int prec, const double* x, const double* y, int64_t n, int64_t d,
+                    const double* theta, int nls, double jitter, int refine_iters, void* ws,
+                    size_t ws_bytes, double* result, void* stream) {
+  int rc = check_common(kind, prec, x, n, d, theta, nls);
+  if (rc) return rc;
+  if (!y || !ws || !result) {
+    set_error("mll_forward: null y/ws/result");
+    return GPK_ERR_BAD_ARG;
+  }
+  const Layout Lo = make_layout(n, d, prec);
+  if (ws_bytes < Lo.total) {
+    set_error("mll_forward: workspace %zu bytes < required %zu", ws_bytes, Lo.total);
+    return GPK_ERR_WORKSPACE;
+  }
+  if (reinterpret_cast<uintptr_t>(ws) % ALIGN) {
+    set_error("mll_forward: workspace must be %zu-byte aligned", ALIGN);
+    return GPK_ERR_BAD_ARG;
+  }
+  const int iters = refine_iters >= 0 ? refine_iters : (prec == GPK_PREC_FP64 ? 1 : 4);
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == GPK_PREC_FP64)
+    return mll_forward_t<double>(kind, prec, x, y, n, (int)d, theta, nls, jitter, iters, ws, Lo,
+                                 result, s);
+  return mll_forward_t<float>(kind, prec, x, y, n, (int)d, theta, nls, jitter, iters, ws, Lo,
+                              result, s);
+}
+
+int gpk_mll_backward(int kind, int prec, const double* x, int64_t n, int64_t d,
+                     const double* theta, int nls, double gbar_host, void* ws, size_t ws_bytes,
+                     double* grad, void* stream) {
+  int rc = check_common(kind, prec, x, n, d, theta, nls);
+  if (rc) return rc;
+  if (!ws || !grad) {
+    set_error("mll_backward: null ws/grad");
+    return GPK_ERR_BAD_ARG;
+  }
+  const Layout Lo = make_layout(n, d, prec);
+  if (ws_bytes < Lo.total) {
+    set_error("mll_backward: workspace %zu bytes < required %zu", ws_bytes, Lo.total);
+    return GPK_ERR_WORKSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == GPK_PREC_FP64)
+    return mll_backward_t<double>(kind, x, n, (int)d, theta, nls, gbar_host, ws, Lo, grad, s);
+  return mll_backward_t<float>(kind, x, n, (int)d, theta, nls, gbar_host, ws, Lo, grad, s);
+}
+
+int gpk_mfma_selftest(double* out_f64, float* out_bf16, void* stream) {
+  if (!out_f64 || !out_bf16) {
+    set_error("selftest: null output");
+    return GPK_ERR_BAD_ARG;
+  }
+  return launch_mfma_selftest(out_f64, out_bf16, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,283 @@
+// Tile main loop shared by the GEMM kernels and the fused adjoint kernel.
+//
+// One 256-thread workgroup computes a 128 x 128 tile of
+//   acc(m, n) = sum_{k in [kb, ke)} A(m, k) * B(n, k)
+// with four waves in a 2 x 2 arrangement, 64 x 64 per wave, on the CDNA4
+// matrix cores:
+//   AR_F64    : v_mfma_f64_16x16x4f64, operands staged in LDS as float64,
+//               k-slabs of 16, double-buffered LDS (one barrier per slab).
+//   AR_BF16X3 : fp32 operands split on load into bf16 hi + lo planes;
+//               products hi*hi + hi*lo + lo*hi (Henry, Tang & Heinecke,
+//               ARITH 2019) on v_mfma_f32_16x16x32_bf16, fp32 accumulate.
+//   AR_BF16X6 : three planes hi + mid + lo, six products
+//               (hh, hm, mh, mm, hl, lh): about fp32-product accuracy.
+// Operand layouts: A(m, k) = A[m*lda + k] (row-contiguous) or A[k*lda + m]
+// (column-contiguous); likewise B(n, k).  All extents are multiples of 128.
+//
+// Matrix-core lane layouts used (gfx950, wave64):
+//   16x16x4 f64 : lane l supplies A[l%16][l/16], B[l/16][l%16];
+//                 acc element r of lane l is D[4*(l/16) + r][l%16].
+//   16x16x32 bf16: lane l supplies A[l%16][8*(l/16) .. +7],
+//                 B[8*(l/16) .. +7][l%16]; acc layout as above.
+// gpk_mfma_selftest checks both layouts on the device.
+#pragma once
+
+#include "gpk_internal.h"
+
+namespace gpk {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int TILE = 128;
+
+template <int AR>
+struct AccT {
+  typedef f32x4 vec;
+  typedef float scalar;
+};
+template <>
+struct AccT<AR_F64> {
+  typedef f64x4 vec;
+  typedef double scalar;
+};
+
+// ---- bf16 split helpers (round to nearest even, quiet NaN preserved) ----
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  unsigned int u = __float_as_uint(f);
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (unsigned short)((u >> 16) | 0x40u);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) {
+  return __uint_as_float(((unsigned int)h) << 16);
+}
+template <int NP>
+__device__ __forceinline__ void split_bf16(float a, unsigned short (&p)[NP]) {
+  float r = a;
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    p[q] = f2bf(r);
+    r = r - bf2f(p[q]);
+  }
+}
+
+// ---- LDS footprint of one main loop ----
+constexpr int F64_BK = 16;
+constexpr int F64_LDP = TILE + 1;  // padded row length (doubles)
+constexpr size_t F64_SMEM = 2 /*A,B*/ * 2 /*buffers*/ * F64_BK * F64_LDP * sizeof(double);
+constexpr int BF_BK = 32;
+constexpr int BF_LDK = BF_BK + 8;  // padded k length (bf16)
+template <int AR>
+constexpr int bf_planes() { return AR == AR_BF16X6 ? 3 : 2; }
+template <int AR>
+constexpr size_t smem_bytes() {
+  return AR == AR_F64 ? F64_SMEM
+                      : (size_t)2 * bf_planes<AR>() * TILE * BF_LDK * sizeof(unsigned short);
+}
+
+template <typename T>
+__device__ __forceinline__ double to_f64(T v) { return (double)v; }
+
+// Load 8 consecutive elements as float64.
+template <typename T>
+__device__ __forceinline__ void load8(const T* __restrict__ p, double (&v)[8]) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] = (double)p[q];
+}
+
+// ---------------------------------------------------------------------------
+// float64 main loop
+// ---------------------------------------------------------------------------
+template <typename TA, typename TB, bool A_KM, bool B_KN>
+__device__ __forceinline__ void mainloop_f64(const TA* __restrict__ A, int64_t lda,
+                                             const TB* __restrict__ B, int64_t ldb, int64_t m0,
+                                             int64_t n0, int64_t kb, int64_t ke,
+                                             f64x4 (&acc)[4][4], char* smem) {
+  double* As = reinterpret_cast<double*>(smem);                     // [2][BK][LDP]
+  double* Bs = As + 2 * F64_BK * F64_LDP;                           // [2][BK][LDP]
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = w >> 1, wn = w & 1;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+  if (kb >= ke) return;
+
+  // Global -> register staging: 8 elements per thread per operand.
+  // Row-contiguous operand: row = t/2, k offset (t%2)*8.
+  // Column-contiguous operand: k = t/16, row offset (t%16)*8.
+  double ra[8], rb[8];
+  auto gload = [&](int64_t k0) {
+    if (!A_KM) load8(A + (m0 + (t >> 1)) * lda + k0 + (t & 1) * 8, ra);
+    else load8(A + (k0 + (t >> 4)) * lda + m0 + (t & 15) * 8, ra);
+    if (!B_KN) load8(B + (n0 + (t >> 1)) * ldb + k0 + (t & 1) * 8, rb);
+    else load8(B + (k0 + (t >> 4)) * ldb + n0 + (t & 15) * 8, rb);
+  };
+  auto sstore = [&](int buf) {
+    double* a = As + buf * F64_BK * F64_LDP;
+    double* b = Bs + buf * F64_BK * F64_LDP;
+    if (!A_KM) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) a[((t & 1) * 8 + q) * F64_LDP + (t >> 1)] = ra[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) a[(t >> 4) * F64_LDP + (t & 15) * 8 + q] = ra[q];
+    }
+    if (!B_KN) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) b[((t & 1) * 8 + q) * F64_LDP + (t >> 1)] = rb[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) b[(t >> 4) * F64_LDP + (t & 15) * 8 + q] = rb[q];
+    }
+  };
+
+  const int64_t nslab = (ke - kb) / F64_BK;
+  gload(kb);
+  sstore(0);
+  __syncthreads();
+  for (int64_t sl = 0; sl < nslab; ++sl) {
+    const int buf = (int)(sl & 1);
+    if (sl + 1 < nslab) gload(kb + (sl + 1) * F64_BK);
+    const double* a = As + buf * F64_BK * F64_LDP;
+    const double* b = Bs + buf * F64_BK * F64_LDP;
+#pragma unroll
+    for (int kk = 0; kk < F64_BK / 4; ++kk) {
+      const int kr = kk * 4 + (lane >> 4);
+      double av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) av[i] = a[kr * F64_LDP + wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[j] = b[kr * F64_LDP + wn * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+    }
+    if (sl + 1 < nslab) sstore(buf ^ 1);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// bf16-split main loop (fp32 operands)
+// ---------------------------------------------------------------------------
+template <int AR, bool A_KM, bool B_KN>
+__device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64_t lda,
+                                              const float* __restrict__ B, int64_t ldb,
+                                              int64_t m0, int64_t n0, int64_t kb, int64_t ke,
+                                              f32x4 (&acc)[4][4], char* smem) {
+  constexpr int NP = bf_planes<AR>();
+  unsigned short* As = reinterpret_cast<unsigned short*>(smem);   // [NP][TILE][LDK]
+  unsigned short* Bs = As + NP * TILE * BF_LDK;                    // [NP][TILE][LDK]
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = w >> 1, wn = w & 1;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  if (kb >= ke) return;
+
+  // 16 fp32 per thread per operand.
+  // Row-contiguous: row = t/2, k offset (t%2)*16.  Column-contiguous: k = t/8,
+  // row offset (t%8)*16.
+  float ra[16], rb[16];
+  auto gload = [&](int64_t k0) {
+    const float* pa = !A_KM ? A + (m0 + (t >> 1)) * lda + k0 + (t & 1) * 16
+                            : A + (k0 + (t >> 3)) * lda + m0 + (t & 7) * 16;
+    const float* pb = !B_KN ? B + (n0 + (t >> 1)) * ldb + k0 + (t & 1) * 16
+                            : B + (k0 + (t >> 3)) * ldb + n0 + (t & 7) * 16;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float4 va = *reinterpret_cast<const float4*>(pa + 4 * q);
+      const float4 vb = *reinterpret_cast<const float4*>(pb + 4 * q);
+      ra[4 * q] = va.x; ra[4 * q + 1] = va.y; ra[4 * q + 2] = va.z; ra[4 * q + 3] = va.w;
+      rb[4 * q] = vb.x; rb[4 * q + 1] = vb.y; rb[4 * q + 2] = vb.z; rb[4 * q + 3] = vb.w;
+    }
+  };
+  auto sstore_one = [&](unsigned short* base, const float (&r)[16], bool km) {
+    if (!km) {
+      // 16 consecutive k of one row: contiguous in every plane.
+      const int row = t >> 1, ko = (t & 1) * 16;
+      unsigned short p[16][NP];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) split_bf16<NP>(r[q], p[q]);
+#pragma unroll
+      for (int pl = 0; pl < NP; ++pl) {
+        unsigned short* d = base + (pl * TILE + row) * BF_LDK + ko;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) d[q] = p[q][pl];
+      }
+    } else {
+      const int k = t >> 3, ro = (t & 7) * 16;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        unsigned short p[NP];
+        split_bf16<NP>(r[q], p);
+#pragma unroll
+        for (int pl = 0; pl < NP; ++pl) base[(pl * TILE + ro + q) * BF_LDK + k] = p[pl];
+      }
+    }
+  };
+
+  const int64_t nslab = (ke - kb) / BF_BK;
+  gload(kb);
+  for (int64_t sl = 0; sl < nslab; ++sl) {
+    sstore_one(As, ra, A_KM);
+    sstore_one(Bs, rb, B_KN);
+    __syncthreads();
+    if (sl + 1 < nslab) gload(kb + (sl + 1) * BF_BK);
+    bf16x8 av[NP][4], bv[NP][4];
+#pragma unroll
+    for (int pl = 0; pl < NP; ++pl) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        av[pl][i] = *reinterpret_cast<const bf16x8*>(
+            As + (pl * TILE + wm * 64 + i * 16 + (lane & 15)) * BF_LDK + 8 * (lane >> 4));
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        bv[pl][j] = *reinterpret_cast<const bf16x8*>(
+            Bs + (pl * TILE + wn * 64 + j * 16 + (lane & 15)) * BF_LDK + 8 * (lane >> 4));
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        f32x4 c = acc[i][j];
+        if (AR == AR_BF16X6) {
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[2][i], bv[0][j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0][i], bv[2][j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[1][i], bv[1][j], c, 0, 0, 0);
+        }
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[1][i], bv[0][j], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0][i], bv[1][j], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0][i], bv[0][j], c, 0, 0, 0);
+        acc[i][j] = c;
+      }
+    __syncthreads();
+  }
+}
+
+// Tile decode for lower-triangular tile sets: t -> (mt, nt), mt >= nt, with
+// tiles of small mt first.
+__device__ __forceinline__ void lower_tile(int64_t t, int64_t& mt, int64_t& nt) {
+  int64_t a = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while (a * (a + 1) / 2 > t) --a;
+  while ((a + 1) * (a + 2) / 2 <= t) ++a;
+  mt = a;
+  nt = t - a * (a + 1) / 2;
+}
+
+__device__ __forceinline__ void k_range(int krange, int64_t mt, int64_t nt, int64_t K,
+                                        int64_t& kb, int64_t& ke) {
+  kb = 0;
+  ke = K;
+  if (krange == KR_GE_N) kb = nt * TILE;
+  else if (krange == KR_LE_M) ke = (mt + 1) * TILE < K ? (mt + 1) * TILE : K;
+  else if (krange == KR_GE_M) kb = mt * TILE;
+}
+
+}  // namespace gpk

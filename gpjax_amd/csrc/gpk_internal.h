@@ -1,0 +1,151 @@
+// Internal declarations shared by the libgpk HIP sources (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/gpk.h"
+
+namespace gpk {
+
+// Block size of the recursive factorisation: leaves are NB x NB diagonal
+// blocks factored in LDS, and every GEMM tile is NB x NB.  Internal matrices
+// are padded to a multiple of NB with an identity tail.
+constexpr int NB = 128;
+// Largest input dimension the fused kernels keep in LDS / registers.
+constexpr int MAX_D = 32;
+
+void set_error(const char* fmt, ...);
+
+#define GPK_HIP_TRY(call)                                                            \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      ::gpk::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                       __LINE__);                                                    \
+      return GPK_ERR_HIP;                                                            \
+    }                                                                                \
+  } while (0)
+
+#define GPK_LAUNCH_CHECK()                                                           \
+  do {                                                                               \
+    hipError_t e_ = hipGetLastError();                                               \
+    if (e_ != hipSuccess) {                                                          \
+      ::gpk::set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(e_),     \
+                       __FILE__, __LINE__);                                          \
+      return GPK_ERR_HIP;                                                            \
+    }                                                                                \
+  } while (0)
+
+inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+// ---------------------------------------------------------------------------
+// Stationary kernel formulas (unit variance), from rho^2 of the
+// lengthscale-scaled inputs.  kval() is k(rho); kfac() is F(rho) with
+// dK/dl_d = var * F(rho) * u_d / l_d, u_d = (x_d - z_d)^2 / l_d^2.
+//   RBF        gpjax/kernels/stationary/rbf.py:43          k = exp(-rho^2/2)
+//   Matern-1/2 gpjax/kernels/stationary/matern12.py:47     k = exp(-rho)
+//   Matern-3/2 gpjax/kernels/stationary/matern32.py:48-53  k = (1+s3 rho) exp(-s3 rho)
+//   Matern-5/2 gpjax/kernels/stationary/matern52.py:47-52  k = (1+s5 rho+5/3 rho^2) exp(-s5 rho)
+// Matern distances are sqrt(max(rho^2, 1e-36)) (stationary/utils.py:67).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double kval(int kind, double d2) {
+  if (kind == GPK_KERNEL_RBF) return exp(-0.5 * d2);
+  const double t = sqrt(fmax(d2, 1e-36));
+  if (kind == GPK_KERNEL_MATERN12) return exp(-t);
+  if (kind == GPK_KERNEL_MATERN32) {
+    const double s3 = 1.7320508075688772;
+    return (1.0 + s3 * t) * exp(-s3 * t);
+  }
+  const double s5 = 2.23606797749979;
+  return (1.0 + s5 * t + (5.0 / 3.0) * t * t) * exp(-s5 * t);
+}
+
+// Both k(rho) and F(rho) with one exponential.
+__device__ __forceinline__ void kval_fac(int kind, double d2, double& k, double& f) {
+  if (kind == GPK_KERNEL_RBF) {
+    k = exp(-0.5 * d2);
+    f = k;
+    return;
+  }
+  const double t = sqrt(fmax(d2, 1e-36));
+  if (kind == GPK_KERNEL_MATERN12) {
+    const double e = exp(-t);
+    k = e;
+    f = d2 > 1e-36 ? e / t : 0.0;
+    return;
+  }
+  if (kind == GPK_KERNEL_MATERN32) {
+    const double s3 = 1.7320508075688772;
+    const double e = exp(-s3 * t);
+    k = (1.0 + s3 * t) * e;
+    f = 3.0 * e;
+    return;
+  }
+  const double s5 = 2.23606797749979;
+  const double e = exp(-s5 * t);
+  k = (1.0 + s5 * t + (5.0 / 3.0) * t * t) * e;
+  f = (5.0 / 3.0) * (1.0 + s5 * t) * e;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (each defined next to its kernels).
+// ---------------------------------------------------------------------------
+
+// Gram / cross-covariance tiles.  Rows i < rows_total, cols j < cols_total are
+// written; entries with i >= n or j >= m are the identity pad (1 on i == j,
+// else 0).  diag_mode: 0 none, 1 add jitter, 2 add jitter + obs_sd^2 (i == j < n).
+// lower_only: only 128-tiles on or below the diagonal (rows_total == cols_total).
+template <typename TS>
+int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, int64_t m,
+                         int d, const double* theta, int nls, double jitter, int diag_mode,
+                         TS* out, int64_t ldo, int64_t rows_total, int64_t cols_total,
+                         bool lower_only, hipStream_t s);
+
+// Leaf: factor the NB x NB diagonal block at (o, o) of A in float64 in LDS,
+// write L (lower, zero upper) to A, L^-1 to X (lower, zero upper) and to
+// leafinv (float64), 2*sum(log diag) to *logdet, first bad pivot to *info.
+template <typename TS>
+int launch_leaf(TS* A, TS* X, int64_t ld, int64_t o, double* leafinv, double* logdet, int* info,
+                hipStream_t s);
+
+// k-range modes of the tile GEMM.
+enum KRange { KR_FULL = 0, KR_GE_N = 1, KR_LE_M = 2, KR_GE_M = 3 };
+// Product arithmetic of the tile GEMM.
+enum Arith { AR_F64 = 0, AR_BF16X3 = 3, AR_BF16X6 = 6 };
+
+// C[M x N] = alpha * sum_k A(m, k) B(n, k) (+ C if beta_one), M, N, K multiples
+// of NB.  A(m, k) is A[m*lda + k] (a_km = false) or A[k*lda + m] (a_km = true);
+// B(n, k) is B[n*ldb + k] (b_kn = false) or B[k*ldb + n] (b_kn = true).
+// lower_tiles: only tiles with mt >= nt.  krange restricts k per tile.
+template <typename TA, typename TB, typename TC>
+int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const TB* B,
+                int64_t ldb, TC* C, int64_t ldc, int64_t M, int64_t N, int64_t K, double alpha,
+                bool beta_one, bool lower_tiles, int krange, hipStream_t s);
+
+// Fused LAUUM + adjoint: for every lower NB-tile of Sinv = X^T X (X = L^-1,
+// lower, Np x Np), reduce 1/2 tr(W dSigma/dtheta) contributions with
+// W = alpha alpha^T - Sinv against kernel tiles recomputed from x.
+// partial: [num_tiles][nls + 2] float64.
+template <typename TS>
+int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, const double* x,
+                   int64_t n, int d, const double* theta, int nls, const double* alpha,
+                   double* partial, hipStream_t s);
+
+// y = X r (trans = false) or y = X^T r (trans = true), X lower triangular
+// np x np (only k <= i is read); float64 accumulation through `partial`.
+template <typename TS>
+int launch_trmv(const TS* X, int64_t ld, int64_t np, const double* r, double* y, bool trans,
+                double* partial, hipStream_t s);
+
+// v = Sigma a, Sigma recomputed from x in float64 (identity pad for i >= n).
+int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
+                        const double* theta, int nls, double jitter, const double* a, double* v,
+                        double* partial, hipStream_t s);
+
+int launch_mfma_selftest(double* out64, float* outbf, hipStream_t s);
+
+}  // namespace gpk

@@ -1,0 +1,154 @@
+// Leaf of the recursive Cholesky: factor one NB x NB diagonal block in float64
+// and form its inverse in the same sweep.
+//
+// Replaces the diagonal work of jnp.linalg.cholesky / slogdet
+// (gpjax/linalg/operations.py:54-55,163-165).  Algorithm: right-looking
+// Cholesky by columns; alongside it the numerators of X = L^-1 are eliminated
+// row by row (forward substitution of L X = I, right-looking form).  Storage is
+// one NB x NB float64 array held in registers across the workgroup: the lower
+// triangle holds the trailing matrix / L, the strict upper triangle holds
+// X^T (entry (c, i), c < i, is the numerator of X[i][c]).  At step k column k
+// of that array is exactly [X-row-k numerators ; pivot ; L-column-k
+// numerators], so one broadcast of column k per step drives both updates.
+//
+// Thread t of 256 owns the 8 x 8 block rows 8*(t/16).., cols 8*(t%16)..;
+// column k+1 is published to a double-buffered LDS vector by its owners
+// right after their update, so each step costs one barrier.
+//
+// Outputs: L (lower, zero strict upper) into A, X = L^-1 (lower, zero strict
+// upper) into X, X in float64 into leafinv, 2*sum(log L_kk) into *logdet and
+// 1 + global index of the first non-positive pivot (or 0) into *info.  A
+// non-positive pivot turns the rest of the block into NaN, which propagates
+// to the objective.
+#include "gpk_internal.h"
+
+namespace gpk {
+namespace {
+
+template <typename TS>
+__global__ __launch_bounds__(256) void leaf_kernel(TS* A, TS* X, int64_t ld, int64_t o,
+                                                   double* __restrict__ leafinv,
+                                                   double* __restrict__ logdet,
+                                                   int* __restrict__ info) {
+  __shared__ double col_s[2][NB];
+  __shared__ double dinv_s[NB];
+  __shared__ double tr_s[NB][NB + 1];
+
+  const int t = threadIdx.x;
+  const int rb = (t >> 4) * 8;
+  const int cb = (t & 15) * 8;
+  double s[8][8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const TS* row = A + (o + rb + a) * ld + o + cb;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int i = rb + a, j = cb + b;
+      s[a][b] = (j <= i) ? (double)row[b] : 0.0;
+    }
+  }
+  if (cb == 0) {
+#pragma unroll
+    for (int a = 0; a < 8; ++a) col_s[0][rb + a] = s[a][0];
+  }
+  __syncthreads();
+
+  double ld_acc = 0.0;
+  int bad = -1;
+  for (int k = 0; k < NB; ++k) {
+    const double* col = col_s[k & 1];
+    const double piv = col[k];
+    const double dk = sqrt(piv);
+    const double inv = 1.0 / dk;
+    if (t == 0) {
+      if (bad < 0 && !(piv > 0.0)) bad = k;
+      ld_acc += log(dk);
+      dinv_s[k] = inv;
+    }
+    // Skip blocks with no element to touch: every update at step k has column > k
+    // or is column k itself.
+    if (cb + 7 >= k) {
+      double sc_c[8];
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const int j = cb + b;
+        sc_c[b] = (j == k) ? inv : col[j] * inv;
+      }
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        const int i = rb + a;
+        const double sci = (i == k) ? inv : col[i] * inv;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          const int j = cb + b;
+          if (j <= i) {
+            // lower: trailing matrix, becomes L
+            if (j == k) {
+              s[a][b] = (i == k) ? dk : sci;
+            } else if (j > k) {
+              s[a][b] -= sci * sc_c[b];
+            }
+          } else {
+            // strict upper: (i, j) holds the numerator of X[j][i]
+            if (j == k) {
+              s[a][b] = sci;
+            } else if (j > k && i <= k) {
+              s[a][b] -= sc_c[b] * sci;
+            }
+          }
+        }
+      }
+    }
+    if (k + 1 < NB && cb <= k + 1 && k + 1 < cb + 8) {
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        if (cb + b == k + 1) {
+#pragma unroll
+          for (int a = 0; a < 8; ++a) col_s[(k + 1) & 1][rb + a] = s[a][b];
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // L out (lower incl. diagonal), and X^T staged in LDS for a coalesced write.
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    TS* row = A + (o + rb + a) * ld + o + cb;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int i = rb + a, j = cb + b;
+      row[b] = (TS)((j <= i) ? s[a][b] : 0.0);
+      if (j > i) tr_s[j][i] = s[a][b];
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < NB * NB; e += 256) {
+    const int i = e / NB, j = e % NB;
+    const double v = (j < i) ? tr_s[i][j] : (j == i ? dinv_s[i] : 0.0);
+    X[(o + i) * ld + o + j] = (TS)v;
+    leafinv[e] = v;
+  }
+  if (t == 0) {
+    *logdet = 2.0 * ld_acc;
+    *info = bad >= 0 ? (int)(o + bad + 1) : 0;
+  }
+}
+
+}  // namespace
+
+template <typename TS>
+int launch_leaf(TS* A, TS* X, int64_t ld, int64_t o, double* leafinv, double* logdet, int* info,
+                hipStream_t s) {
+  hipLaunchKernelGGL(leaf_kernel<TS>, dim3(1), dim3(256), 0, s, A, X, ld, o, leafinv, logdet,
+                     info);
+  GPK_LAUNCH_CHECK();
+  return GPK_OK;
+}
+
+template int launch_leaf<double>(double*, double*, int64_t, int64_t, double*, double*, int*,
+                                 hipStream_t);
+template int launch_leaf<float>(float*, float*, int64_t, int64_t, double*, double*, int*,
+                                hipStream_t);
+
+}  // namespace gpk

@@ -1,0 +1,232 @@
+// Vector kernels of the exact-GP path: triangular matvecs with X = L^-1,
+// the Sigma * a matvec with Sigma recomputed from the inputs (float64, for
+// iterative refinement of alpha = Sigma^-1 r), and the matrix-core lane-layout
+// self-test.  Every reduction writes per-chunk partials and sums them in a
+// fixed order, so results are bitwise reproducible run to run.
+#include "gemm_core.h"
+
+namespace gpk {
+namespace {
+
+constexpr int TRMV_CW = 2048;  // columns per chunk, non-transposed
+constexpr int TRMV_RC = 1024;  // rows per chunk, transposed
+
+// y_i partial over columns [c*CW, (c+1)*CW), k <= i.  One wave per row, 16 rows
+// per workgroup.
+template <typename TS>
+__global__ __launch_bounds__(256) void trmv_n_kernel(const TS* __restrict__ X, int64_t ld,
+                                                     int64_t np, const double* __restrict__ r,
+                                                     double* __restrict__ partial) {
+  const int64_t c = blockIdx.y;
+  const int64_t row0 = (int64_t)blockIdx.x * 16;
+  const int64_t k0 = c * TRMV_CW;
+  if (k0 > row0 + 15) return;
+  __shared__ double rs[TRMV_CW];
+  for (int e = threadIdx.x; e < TRMV_CW; e += 256) rs[e] = (k0 + e < np) ? r[k0 + e] : 0.0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int q = 0; q < 4; ++q) {
+    const int64_t i = row0 + w * 4 + q;
+    const int64_t kend = (i + 1 < k0 + TRMV_CW) ? i + 1 : k0 + TRMV_CW;
+    double acc = 0.0;
+    const TS* xr = X + i * ld;
+    for (int64_t k = k0 + lane; k < kend; k += 64) acc += (double)xr[k] * rs[k - k0];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) partial[c * np + i] = acc;
+  }
+}
+
+__global__ void trmv_n_reduce(const double* __restrict__ partial, int64_t np,
+                              double* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int64_t nc = i / TRMV_CW;
+  double s = 0.0;
+  for (int64_t c = 0; c <= nc; ++c) s += partial[c * np + i];
+  y[i] = s;
+}
+
+// y_k partial over rows [c*RC, (c+1)*RC), i >= k.  One thread per column.
+template <typename TS>
+__global__ __launch_bounds__(256) void trmv_t_kernel(const TS* __restrict__ X, int64_t ld,
+                                                     int64_t np, const double* __restrict__ r,
+                                                     double* __restrict__ partial) {
+  const int64_t c = blockIdx.y;
+  const int64_t col0 = (int64_t)blockIdx.x * 256;
+  const int64_t i0 = c * TRMV_RC;
+  const int64_t i1 = (i0 + TRMV_RC < np) ? i0 + TRMV_RC : np;
+  if (i1 <= col0) return;
+  __shared__ double rs[TRMV_RC];
+  for (int e = threadIdx.x; e < TRMV_RC; e += 256) rs[e] = (i0 + e < np) ? r[i0 + e] : 0.0;
+  __syncthreads();
+  const int64_t k = col0 + threadIdx.x;
+  double acc = 0.0;
+  const int64_t ib = (k > i0) ? k : i0;
+  for (int64_t i = ib; i < i1; ++i) acc += (double)X[i * ld + k] * rs[i - i0];
+  partial[c * np + k] = acc;
+}
+
+__global__ void trmv_t_reduce(const double* __restrict__ partial, int64_t np, int64_t nchunks,
+                              double* __restrict__ y) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  double s = 0.0;
+  for (int64_t c = k / TRMV_RC; c < nchunks; ++c) s += partial[c * np + k];
+  y[k] = s;
+}
+
+// v_i partial = sum_{j in chunk} Sigma_ij a_j, Sigma recomputed in float64.
+constexpr int SMV_CW = 4096;  // columns per chunk
+constexpr int SMV_JT = 256;   // columns staged per LDS pass
+
+template <int DMAX>
+__global__ __launch_bounds__(256) void sigma_mv_kernel(int kind, const double* __restrict__ x,
+                                                       int64_t n, int64_t np, int d,
+                                                       const double* __restrict__ theta,
+                                                       int nls, double jitter,
+                                                       const double* __restrict__ a,
+                                                       double* __restrict__ partial) {
+  __shared__ double xj_s[SMV_JT][DMAX + 1];
+  __shared__ double aj_s[SMV_JT];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t c = blockIdx.y;
+  const int64_t j0c = c * SMV_CW;
+  const double var = theta[nls];
+  const double sd = theta[nls + 1];
+  double xi[DMAX];
+#pragma unroll
+  for (int q = 0; q < DMAX; ++q)
+    xi[q] = (q < d && i < n) ? x[i * d + q] / theta[nls == 1 ? 0 : q] : 0.0;
+  double acc = 0.0;
+  for (int64_t j0 = j0c; j0 < j0c + SMV_CW && j0 < np; j0 += SMV_JT) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < SMV_JT * DMAX; e += 256) {
+      const int r = e / DMAX, q = e % DMAX;
+      const int64_t gj = j0 + r;
+      xj_s[r][q] = (q < d && gj < n) ? x[gj * d + q] / theta[nls == 1 ? 0 : q] : 0.0;
+    }
+    aj_s[threadIdx.x] = (j0 + threadIdx.x < np) ? a[j0 + threadIdx.x] : 0.0;
+    __syncthreads();
+    if (i < n) {
+      const int64_t jend = (j0 + SMV_JT < n) ? SMV_JT : n - j0;
+      for (int jj = 0; jj < jend; ++jj) {
+        double d2 = 0.0;
+#pragma unroll
+        for (int q = 0; q < DMAX; ++q) {
+          const double df = xi[q] - xj_s[jj][q];
+          d2 += df * df;
+        }
+        acc += var * kval(kind, d2) * aj_s[jj];
+      }
+    }
+  }
+  if (i < np) {
+    // diagonal additions (jitter + obs_sd^2) and the identity pad
+    if (i >= j0c && i < j0c + SMV_CW) {
+      if (i < n) acc += (jitter + sd * sd) * a[i];
+      else acc += a[i];
+    }
+    partial[c * np + i] = acc;
+  }
+}
+
+__global__ void sum_chunks(const double* __restrict__ partial, int64_t np, int64_t nchunks,
+                           double* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  double s = 0.0;
+  for (int64_t c = 0; c < nchunks; ++c) s += partial[c * np + i];
+  y[i] = s;
+}
+
+// ---- matrix-core lane layout self-test ----
+// A[i][k] = i + 100 k, B[k][j] = 1000 + j - 7 k: D = A B^T-as-B computed per lane.
+__global__ void mfma_selftest_kernel(double* out64, float* outbf) {
+  const int l = threadIdx.x;
+  {
+    const double a = (double)(l % 16) + 100.0 * (l / 16);
+    const double b = 1000.0 + (l % 16) - 7.0 * (l / 16);
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    for (int r = 0; r < 4; ++r) out64[l * 4 + r] = acc[r];
+  }
+  {
+    bf16x8 av, bv;
+    for (int q = 0; q < 8; ++q) {
+      const int k = 8 * (l / 16) + q;
+      av[q] = (__bf16)(float)((l % 16) + 2 * (k % 8) - (k / 8));
+      bv[q] = (__bf16)(float)(3 * (l % 16) - k + 1);
+    }
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+    for (int r = 0; r < 4; ++r) outbf[l * 4 + r] = acc[r];
+  }
+}
+
+}  // namespace
+
+template <typename TS>
+int launch_trmv(const TS* X, int64_t ld, int64_t np, const double* r, double* y, bool trans,
+                double* partial, hipStream_t s) {
+  if (np % 256) {
+    set_error("trmv: size %lld not a multiple of 256", (long long)np);
+    return GPK_ERR_BAD_ARG;
+  }
+  if (!trans) {
+    const int64_t nc = (np + TRMV_CW - 1) / TRMV_CW;
+    hipLaunchKernelGGL(trmv_n_kernel<TS>, dim3((unsigned)(np / 16), (unsigned)nc), dim3(256), 0,
+                       s, X, ld, np, r, partial);
+    GPK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(trmv_n_reduce, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s,
+                       partial, np, y);
+  } else {
+    const int64_t nc = (np + TRMV_RC - 1) / TRMV_RC;
+    hipLaunchKernelGGL(trmv_t_kernel<TS>, dim3((unsigned)(np / 256), (unsigned)nc), dim3(256), 0,
+                       s, X, ld, np, r, partial);
+    GPK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(trmv_t_reduce, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s,
+                       partial, np, nc, y);
+  }
+  GPK_LAUNCH_CHECK();
+  return GPK_OK;
+}
+
+template int launch_trmv<double>(const double*, int64_t, int64_t, const double*, double*, bool,
+                                 double*, hipStream_t);
+template int launch_trmv<float>(const float*, int64_t, int64_t, const double*, double*, bool,
+                                double*, hipStream_t);
+
+int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
+                        const double* theta, int nls, double jitter, const double* a, double* v,
+                        double* partial, hipStream_t s) {
+  if (d < 1 || d > MAX_D) {
+    set_error("sigma matvec: input dimension %d outside [1, %d]", d, MAX_D);
+    return GPK_ERR_BAD_ARG;
+  }
+  const int64_t nc = (np + SMV_CW - 1) / SMV_CW;
+  dim3 grid((unsigned)((np + 255) / 256), (unsigned)nc);
+#define GPK_SMV(DM)                                                                          \
+  hipLaunchKernelGGL(sigma_mv_kernel<DM>, grid, dim3(256), 0, s, kind, x, n, np, d, theta, nls, \
+                     jitter, a, partial)
+  if (d <= 1) GPK_SMV(1);
+  else if (d <= 2) GPK_SMV(2);
+  else if (d <= 4) GPK_SMV(4);
+  else if (d <= 8) GPK_SMV(8);
+  else if (d <= 16) GPK_SMV(16);
+  else GPK_SMV(32);
+#undef GPK_SMV
+  GPK_LAUNCH_CHECK();
+  hipLaunchKernelGGL(sum_chunks, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, partial, np,
+                     nc, v);
+  GPK_LAUNCH_CHECK();
+  return GPK_OK;
+}
+
+int launch_mfma_selftest(double* out64, float* outbf, hipStream_t s) {
+  hipLaunchKernelGGL(mfma_selftest_kernel, dim3(1), dim3(64), 0, s, out64, outbf);
+  GPK_LAUNCH_CHECK();
+  return GPK_OK;
+}
+
+}  // namespace gpk

@@ -1,0 +1,106 @@
+"""Objectives on the libgpk path.
+
+``conjugate_mll(posterior, data)`` has the signature and meaning of
+``gpjax/objectives.py:36-107``: the log marginal likelihood
+-1/2 [n log 2pi + log|Sigma| + r^T Sigma^-1 r], Sigma = K + jitter I + obs_sd^2 I
+(``objectives.py:96-103``), r = y - m(x).  It returns a float64 torch scalar whose
+backward is the hand-written adjoint (``gpk_mll_backward``) -- the role
+``jax.custom_vjp`` plays for a JAX caller -- giving gradients w.r.t. the
+lengthscale(s), variance, obs_stddev and a trainable constant mean in
+constrained coordinates.
+
+Divergence from the reference, by design: the reference evaluates log|Sigma|
+with an LU ``slogdet`` and discards the sign (``linalg/operations.py:164``), so
+an indefinite Sigma still yields a finite number there.  The Cholesky path here
+returns NaN and reports the failing pivot (``info``) instead.
+"""
+
+from __future__ import annotations
+
+import math
+
+import torch
+
+from . import _lib, config
+from .mean_functions import Constant
+
+
+def _precision_id(precision: str | None) -> int:
+    p = precision or config.precision
+    if p not in _lib.PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(_lib.PRECISIONS)}, got {p!r}")
+    return _lib.PRECISIONS[p]
+
+
+class _ConjugateMLL(torch.autograd.Function):
+    """Forward: gpk_mll_forward.  Backward: gpk_mll_backward (fused adjoint)."""
+
+    @staticmethod
+    def forward(ctx, ls, var, sd, c, X, y, kind_id, prec_id, jitter, refine, nls, info_out):
+        n, d = X.shape
+        dev = X.device
+        theta = torch.cat([ls.reshape(-1)[:nls], var.reshape(1), sd.reshape(1),
+                           c.reshape(1)]).to(dev, torch.float64).contiguous()
+        nbytes = _lib.lib().gpk_mll_workspace_bytes(n, d, prec_id)
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        res = torch.empty(_lib.RESULT_LEN, dtype=torch.float64, device=dev)
+        it = -1 if refine is None else int(refine)
+        _lib.call("gpk_mll_forward", kind_id, prec_id, _lib.ptr(X), _lib.ptr(y), n, d,
+                  _lib.ptr(theta), nls, float(jitter), it, _lib.ptr(ws), nbytes, _lib.ptr(res),
+                  _lib.stream_ptr(dev))
+        ctx.ws, ctx.nbytes, ctx.theta = ws, nbytes, theta
+        ctx.X, ctx.kind_id, ctx.prec_id, ctx.nls = X, kind_id, prec_id, nls
+        ctx.shapes = (ls.shape, var.shape, sd.shape, c.shape)
+        if info_out is not None:
+            info_out["result"] = res
+        return res[_lib.RES_MLL].clone()
+
+    @staticmethod
+    def backward(ctx, gbar):
+        X = ctx.X
+        n, d = X.shape
+        nls = ctx.nls
+        grad = torch.empty(nls + 3, dtype=torch.float64, device=X.device)
+        _lib.call("gpk_mll_backward", ctx.kind_id, ctx.prec_id, _lib.ptr(X), n, d,
+                  _lib.ptr(ctx.theta), nls, 1.0, _lib.ptr(ctx.ws), ctx.nbytes, _lib.ptr(grad),
+                  _lib.stream_ptr(X.device))
+        g = grad * gbar
+        ls_s, var_s, sd_s, c_s = ctx.shapes
+        ls_g = g[:nls].reshape(ls_s) if math.prod(ls_s) == nls else g[:nls].sum().expand(ls_s)
+        ctx.ws = None
+        return (ls_g, g[nls].reshape(var_s), g[nls + 1].reshape(sd_s), g[nls + 2].reshape(c_s),
+                None, None, None, None, None, None, None, None)
+
+
+def _mean_constant(mean_function, device) -> torch.Tensor:
+    if not isinstance(mean_function, Constant):
+        raise TypeError("the GPU path supports Constant / Zero mean functions")
+    return mean_function.value_tensor(device)
+
+
+def conjugate_mll_with_info(posterior, data, precision: str | None = None,
+                            refine_iters: int | None = None):
+    """(mll, result) where result is the device buffer of ``gpk_mll_forward``
+    (slots: mll, logdet, quad, relative refinement residual, info)."""
+    kernel = posterior.prior.kernel
+    X = kernel.slice_input(data.X)
+    if not X.is_cuda:
+        raise ValueError("conjugate_mll: data must live on a GPU device")
+    y = data.y.reshape(-1).to(torch.float64).contiguous()
+    n, d = X.shape
+    if y.shape[0] != n:
+        raise ValueError("X and y disagree in length")
+    nls = kernel.nls(d)
+    c = _mean_constant(posterior.prior.mean_function, X.device)
+    info: dict = {}
+    mll = _ConjugateMLL.apply(kernel.lengthscale.value, kernel.variance.value,
+                              posterior.likelihood.obs_stddev.value, c, X, y, kernel.kind_id,
+                              _precision_id(precision), float(posterior.prior.jitter),
+                              config.refine_iters if refine_iters is None else refine_iters,
+                              nls, info)
+    return mll, info["result"]
+
+
+def conjugate_mll(posterior, data, precision: str | None = None):
+    """Log marginal likelihood (``gpjax/objectives.py:36-107``), differentiable."""
+    return conjugate_mll_with_info(posterior, data, precision)[0]

@@ -1,0 +1,130 @@
+/*
+ * gpk.h -- C ABI of libgpk, the MI355X (gfx950) exact-GP / collapsed-sparse-GP
+ * hot path.
+ *
+ * Conventions
+ *   - Every pointer argument is a DEVICE pointer unless the name ends in _host.
+ *     Buffers are caller-owned; nothing is freed or retained by the library.
+ *   - Every call is stream-ordered on `stream` (a hipStream_t passed as void*;
+ *     NULL = the legacy default stream) and returns without a host sync.
+ *   - Matrices are row-major float64 with an explicit leading dimension.
+ *   - Return value: GPK_OK or an error code; gpk_last_error() gives a
+ *     thread-local message for the last failing call.
+ *   - Numerical failure (a non-positive pivot in the Cholesky) is detected on
+ *     the device and reported through the `result` buffer (info word and a NaN
+ *     objective), never by a host sync.
+ *
+ * Hyperparameter vector `theta` (device, float64, length nls + 3):
+ *   theta[0 .. nls-1]  lengthscale(s) l_d   (nls == 1: isotropic, nls == d: ARD)
+ *   theta[nls]         kernel variance sigma_f^2
+ *   theta[nls + 1]     likelihood obs_stddev sigma_n (a standard deviation)
+ *   theta[nls + 2]     constant mean c (0 for the Zero mean)
+ * Gradients use the same layout, in constrained coordinates.
+ */
+#ifndef GPK_H
+#define GPK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define GPK_OK 0
+#define GPK_ERR_BAD_ARG 1
+#define GPK_ERR_NOT_PD 2
+#define GPK_ERR_NO_DEVICE 3
+#define GPK_ERR_HIP 4
+#define GPK_ERR_WORKSPACE 5
+
+/* stationary kernel families (gpjax/kernels/stationary/{rbf,matern12,matern32,matern52}.py) */
+#define GPK_KERNEL_RBF 0
+#define GPK_KERNEL_MATERN12 1
+#define GPK_KERNEL_MATERN32 2
+#define GPK_KERNEL_MATERN52 3
+
+/* arithmetic of the factorisation path */
+#define GPK_PREC_FP64 0  /* float64 storage, FP64 MFMA products                    */
+#define GPK_PREC_MIXED 1 /* float32 storage, BF16 split products, FP64 panels      */
+
+/* result[] slots written by gpk_mll_forward */
+#define GPK_RES_MLL 0      /* marginal log-likelihood (NaN if not PD)             */
+#define GPK_RES_LOGDET 1   /* log|Sigma|                                          */
+#define GPK_RES_QUAD 2     /* r^T Sigma^-1 r                                      */
+#define GPK_RES_RESID 3    /* ||r - Sigma alpha|| / ||r|| after refinement        */
+#define GPK_RES_INFO 4     /* 0, or 1 + index of the first non-positive pivot     */
+#define GPK_RESULT_LEN 8
+
+const char* gpk_version(void);
+const char* gpk_last_error(void);
+
+/*
+ * Cross-covariance K[i, j] = var * k(x_i / l, z_j / l), i < n, j < m.
+ * Replaces DenseKernelComputation._cross_covariance
+ * (gpjax/kernels/computations/dense.py:32-36) behind AbstractKernel.cross_covariance
+ * (gpjax/kernels/base.py:101-113).  Direct pairwise differences of the
+ * lengthscale-scaled inputs, as gpjax/kernels/stationary/utils.py:42-67.
+ */
+int gpk_cross_covariance(int kind, const double* x, int64_t n, const double* z, int64_t m,
+                         int64_t d, const double* theta, int nls, double* out, int64_t ldo,
+                         void* stream);
+
+/*
+ * Gram K[i, j] = var * k(x_i, x_j) (+ jitter on the diagonal, + obs_stddev^2 when
+ * add_noise != 0), full n x n.  Replaces AbstractKernelComputation.gram
+ * (gpjax/kernels/computations/base.py:56-72) and the Sigma assembly of
+ * gpjax/objectives.py:100-103 / gpjax/linalg/utils.py:39-65.
+ */
+int gpk_gram(int kind, const double* x, int64_t n, int64_t d, const double* theta, int nls,
+             double jitter, int add_noise, double* out, int64_t ldo, void* stream);
+
+/* Bytes of workspace that gpk_mll_forward / gpk_mll_backward need for (n, d, prec). */
+size_t gpk_mll_workspace_bytes(int64_t n, int64_t d, int prec);
+
+/*
+ * Conjugate marginal log-likelihood, gpjax/objectives.py:36-107
+ * (GaussianDistribution.log_prob, gpjax/distributions.py:115-134, whose
+ * logdet/solve are gpjax/linalg/operations.py:109-111,163-165):
+ *   mll = -1/2 [ n log 2pi + log|Sigma| + r^T Sigma^-1 r ],
+ *   Sigma = K + (jitter + obs_stddev^2) I,  r = y - c.
+ * Computes L = chol(Sigma) and L^-1 in the workspace, alpha = Sigma^-1 r with
+ * `refine_iters` steps of float64 iterative refinement, and writes
+ * result[GPK_RESULT_LEN] (slots GPK_RES_*).  The workspace then holds the state
+ * that gpk_mll_backward consumes.
+ */
+int gpk_mll_forward(int kind, int prec, const double* x, const double* y, int64_t n, int64_t d,
+                    const double* theta, int nls, double jitter, int refine_iters, void* ws,
+                    size_t ws_bytes, double* result, void* stream);
+
+/*
+ * Gradient of the MLL (what jax.grad of conjugate_mll returns,
+ * tests/test_objectives.py:83-85, gpjax/fit.py:160), scaled by the host scalar
+ * `gbar_host` (the upstream adjoint): grad = gbar * 1/2 tr((alpha alpha^T -
+ * Sigma^-1) dSigma/dtheta), layout of theta.  Must follow gpk_mll_forward on the
+ * same workspace and stream.  Sigma^-1 is never stored: each tile of
+ * L^-T L^-1 is formed on the matrix cores and reduced against the kernel
+ * derivative tile, recomputed from x, in the same pass.
+ */
+int gpk_mll_backward(int kind, int prec, const double* x, int64_t n, int64_t d,
+                     const double* theta, int nls, double gbar_host, void* ws, size_t ws_bytes,
+                     double* grad, void* stream);
+
+/*
+ * Products per element used by the GPK_PREC_MIXED stages (3 = bf16 hi/lo
+ * split, 6 = hi/mid/lo split): Cholesky updates, triangular inverse, and the
+ * fused Sigma^-1 adjoint.  Process-wide; defaults 6, 3, 6.  No counterpart in
+ * the reference (which has a single float64 path); it exists to trade speed
+ * against the accuracy figures the benchmark reports.
+ */
+int gpk_set_mixed_arith(int potrf_products, int trtri_products, int lauum_products);
+
+/* Self-test of the matrix-core lane layouts (writes 64*4 doubles + 64*4 floats). */
+int gpk_mfma_selftest(double* out_f64, float* out_bf16, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPK_H */

@@ -1,0 +1,129 @@
+"""GPU parity of the exact-GP path against the float64 oracle (C ABI through gpjax_amd).
+
+Tolerances: the plain FP64 path must match the oracle within 1e-10 relative on
+the MLL and 1e-8 relative (norm-wise) on the gradient; the mixed path within
+1e-6 on the MLL and 1e-4 on the gradient.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import gpjax_amd as gpx
+from gpjax_amd import _lib
+from oracle import gp_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+KCLS = {"rbf": gpx.kernels.RBF, "matern12": gpx.kernels.Matern12,
+        "matern32": gpx.kernels.Matern32, "matern52": gpx.kernels.Matern52}
+
+
+def _problem(n, d, kind, ard, seed, obs_sd=0.3, var=1.3):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-2, 2, (n, d))
+    y = np.sin(X[:, 0]) + 0.1 * rng.standard_normal(n)
+    ls = np.exp(rng.uniform(np.log(0.5), np.log(2.0), d)) if ard else 0.9
+    return X, y, ls, var, obs_sd
+
+
+def _model(cuda, X, y, kind, ls, var, obs_sd, jitter=1e-6, mean=None):
+    k = KCLS[kind](lengthscale=ls, variance=var)
+    mf = gpx.mean_functions.Zero() if mean is None else \
+        gpx.mean_functions.Constant(gpx.parameters.Real(mean))
+    prior = gpx.gps.Prior(mean_function=mf, kernel=k, jitter=jitter)
+    post = prior * gpx.likelihoods.Gaussian(num_datapoints=X.shape[0], obs_stddev=obs_sd)
+    D = gpx.Dataset(X=torch.tensor(X, device=cuda), y=torch.tensor(y.reshape(-1, 1), device=cuda))
+    return post, D, k
+
+
+def test_mfma_lane_layouts(cuda):
+    o64 = torch.zeros(256, dtype=torch.float64, device=cuda)
+    obf = torch.zeros(256, dtype=torch.float32, device=cuda)
+    _lib.call("gpk_mfma_selftest", o64.data_ptr(), obf.data_ptr(),
+              torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    i = np.arange(16)
+    A = i[:, None] + 100.0 * np.arange(4)[None, :]          # A[i][k]
+    B = 1000.0 + i[None, :] - 7.0 * np.arange(4)[:, None]   # B[k][j]
+    D = A @ B
+    l = np.arange(64)
+    got = o64.cpu().numpy().reshape(64, 4)
+    exp = np.stack([D[4 * (l // 16) + r, l % 16] for r in range(4)], 1)
+    np.testing.assert_array_equal(got, exp)
+    k = np.arange(32)
+    Ab = i[:, None] + 2 * (k[None, :] % 8) - k[None, :] // 8
+    Bb = 3 * i[None, :] - k[:, None] + 1
+    Db = (Ab @ Bb).astype(np.float64)
+    gotb = obf.cpu().numpy().reshape(64, 4)
+    expb = np.stack([Db[4 * (l // 16) + r, l % 16] for r in range(4)], 1)
+    np.testing.assert_array_equal(gotb, expb)
+
+
+@pytest.mark.parametrize("kind", list(KCLS))
+@pytest.mark.parametrize("ard", [False, True])
+def test_gram_and_cross_covariance(cuda, kind, ard):
+    rng = np.random.default_rng(7)
+    X = rng.normal(size=(150, 3))
+    Z = rng.normal(size=(77, 3))
+    ls = np.array([0.7, 1.3, 2.1]) if ard else 1.1
+    k = KCLS[kind](lengthscale=ls, variance=1.7)
+    K = k.gram(torch.tensor(X, device=cuda)).cpu().numpy()
+    np.testing.assert_allclose(K, O.gram(kind, X, ls, 1.7), rtol=1e-13, atol=1e-15)
+    C = k.cross_covariance(torch.tensor(X, device=cuda), torch.tensor(Z, device=cuda)).cpu().numpy()
+    np.testing.assert_allclose(C, O.cross_covariance(kind, X, Z, ls, 1.7), rtol=1e-13, atol=1e-15)
+
+
+CASES = [(1, 1, "rbf", False), (2, 2, "rbf", True), (10, 3, "matern32", True),
+         (64, 1, "matern12", False), (257, 3, "matern52", True), (300, 8, "rbf", True),
+         (700, 2, "matern32", False)]
+
+
+@pytest.mark.parametrize("n,d,kind,ard", CASES)
+def test_mll_fp64_parity(cuda, n, d, kind, ard):
+    X, y, ls, var, sd = _problem(n, d, kind, ard, seed=n + d)
+    post, D, k = _model(cuda, X, y, kind, ls, var, sd)
+    mll, res = gpx.objectives.conjugate_mll_with_info(post, D, precision="fp64")
+    ref = O.conjugate_mll_parts(kind, X, y, ls, var, sd)
+    r = res.cpu().numpy()
+    assert r[_lib.RES_INFO] == 0
+    assert abs(float(mll) - ref["mll"]) <= 1e-10 * abs(ref["mll"]) + 1e-12
+    assert abs(r[_lib.RES_LOGDET] - ref["logdet"]) <= 1e-10 * abs(ref["logdet"]) + 1e-10
+    assert r[_lib.RES_RESID] < 1e-12
+
+
+@pytest.mark.parametrize("n,d,kind,ard", CASES)
+def test_mll_grad_fp64_parity(cuda, n, d, kind, ard):
+    X, y, ls, var, sd = _problem(n, d, kind, ard, seed=2 * n + d)
+    post, D, k = _model(cuda, X, y, kind, ls, var, sd, mean=0.2)
+    mll = gpx.objectives.conjugate_mll(post, D, precision="fp64")
+    mll.backward()
+    ref = O.conjugate_mll_grad(kind, X, y, ls, var, sd, mean=0.2)
+    g = np.concatenate([np.atleast_1d(k.lengthscale.value.grad.cpu().numpy()),
+                        [k.variance.value.grad.item(),
+                         post.likelihood.obs_stddev.value.grad.item(),
+                         post.prior.mean_function.constant.value.grad.item()]])
+    gr = np.concatenate([np.atleast_1d(ref["lengthscale"]),
+                         [ref["variance"], ref["obs_stddev"], ref["mean"]]])
+    assert abs(float(mll) - ref["mll"]) <= 1e-10 * abs(ref["mll"]) + 1e-12
+    assert np.linalg.norm(g - gr) <= 1e-8 * np.linalg.norm(gr) + 1e-12, (g, gr)
+
+
+@pytest.mark.parametrize("n,d,kind,ard", [(700, 2, "matern32", False), (1500, 8, "rbf", True),
+                                          (2000, 1, "rbf", False)])
+def test_mll_grad_mixed_parity(cuda, n, d, kind, ard):
+    X, y, ls, var, sd = _problem(n, d, kind, ard, seed=3 * n + d, obs_sd=0.1)
+    post, D, k = _model(cuda, X, y, kind, ls, var, sd)
+    mll, res = gpx.objectives.conjugate_mll_with_info(post, D, precision="mixed")
+    mll.backward()
+    ref = O.conjugate_mll_grad(kind, X, y, ls, var, sd)
+    g = np.concatenate([np.atleast_1d(k.lengthscale.value.grad.cpu().numpy()),
+                        [k.variance.value.grad.item(),
+                         post.likelihood.obs_stddev.value.grad.item()]])
+    gr = np.concatenate([np.atleast_1d(ref["lengthscale"]), [ref["variance"], ref["obs_stddev"]]])
+    rel_mll = abs(float(mll) - ref["mll"]) / abs(ref["mll"])
+    rel_g = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+    print(f"mixed n={n} rel_mll={rel_mll:.2e} rel_grad={rel_g:.2e} resid={res[3].item():.2e}")
+    assert rel_mll <= 1e-6
+    assert rel_g <= 1e-4
+    assert res[_lib.RES_RESID].item() <= 1e-10
