@@ -1,0 +1,202 @@
+#!/usr/bin/env python
+"""MLL + gradient evaluations/s of the exact GP at N=65,536, D=16 on one MI355X.
+
+Workload: ``gpjax_amd.workloads.exact_large`` -- RBF ARD kernel, 65,536 points in
+16 dimensions, random-Fourier-feature targets (synthetic, seeded).  One step =
+one ``objectives.conjugate_mll`` forward plus its backward (the gradient w.r.t.
+the 16 lengthscales, variance and obs_stddev), inputs resident in HBM.
+
+The headline value is the mixed-precision path (float32 storage, BF16-split
+products on the matrix cores, float64 diagonal blocks / refinement /
+log-determinant).  The plain float64 GPU path is timed beside it; its MLL and
+gradient are the accuracy reference at this size (the float64 path itself is
+checked against the float64 CPU oracle at smaller N in tests/).  The CPU
+baseline runs the float64 oracle on the host cores on a bounded sample
+(same recipe, smaller N) and is extrapolated to N=65,536 by N^3.
+
+Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("MLL+grad evals/s at N=65,536: mixed-precision vs FP64 GPU, "
+          "at |ΔMLL|/|MLL|≤1e-6")
+BF16_PEAK_TFLOPS = 2500.0
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def lauum_flops(n: int, tile: int = 128) -> float:
+    """Algorithmic flops of the fused Sigma^-1 = X^T X pass (lower tiles, k >= row tile)."""
+    nt = (n + 255) // 256 * 256 // tile
+    return 2.0 * tile ** 3 * nt * (nt + 1) * (nt + 2) / 6.0
+
+
+def run_steps(post, D, precision, steps, params, report=False):
+    """`steps` MLL + gradient evaluations; with report=True each step syncs and
+    prints its wall time (used for the untimed warm-up steps only)."""
+    import torch
+
+    import gpjax_amd as gpx
+    out = None
+    for i in range(steps):
+        t0 = time.perf_counter()
+        for p in params:
+            p.value.grad = None
+        mll, res = gpx.objectives.conjugate_mll_with_info(post, D, precision=precision)
+        mll.backward()
+        out = (mll, res)
+        if report:
+            torch.cuda.synchronize()
+            print(f"# {precision} warm-up step {i}: {time.perf_counter() - t0:.2f}s "
+                  f"mll={float(mll.detach()):.6f}", file=sys.stderr, flush=True)
+    return out
+
+
+def grads(params):
+    import torch
+    return torch.cat([p.value.grad.reshape(-1) for p in params]).double().cpu()
+
+
+def cpu_baseline(n: int) -> dict:
+    threads = os.cpu_count() or 1
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OPENBLAS_NUM_THREADS=str(threads),
+               MKL_NUM_THREADS=str(threads))
+    cmd = [sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"), "--n", str(n),
+           "--threads", str(threads)]
+    proc = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
+    if proc.returncode != 0:
+        return {"value": None, "error": proc.stderr[-500:]}
+    return json.loads(proc.stdout.strip().splitlines()[-1])
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=65536)
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--fp64-warmup", type=int, default=1)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no "
+              "multi-GPU path", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import gpjax_amd as gpx
+    from gpjax_amd import _lib, workloads
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    w = workloads.exact_large(n=args.n)
+    print(f"# workload generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    X = torch.tensor(w["X"], device=dev)
+    y = torch.tensor(w["y"][:, None], device=dev)
+    D = gpx.Dataset(X=X, y=y)
+    kern = gpx.kernels.RBF(lengthscale=w["lengthscale"], variance=w["variance"])
+    lik = gpx.likelihoods.Gaussian(num_datapoints=args.n, obs_stddev=w["obs_stddev"])
+    post = gpx.gps.Prior(mean_function=gpx.mean_functions.Zero(), kernel=kern,
+                         jitter=w["jitter"]) * lik
+    params = [kern.lengthscale, kern.variance, lik.obs_stddev]
+
+    # ---- mixed precision: the headline ----
+    run_steps(post, D, "mixed", args.warmup, params, report=True)
+    torch.cuda.synchronize()
+    _lib.profile_collect()
+    _lib.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mll_m, res_m = run_steps(post, D, "mixed", args.steps, params)
+    torch.cuda.synchronize()
+    t_mixed = time.perf_counter() - t0
+    _lib.profile_enable(False)
+    phases_m = _lib.profile_collect()
+    g_m = grads(params)
+    mll_m = float(mll_m.detach())
+    resid_m = float(res_m[_lib.RES_RESID])
+    print(f"# mixed: {t_mixed / args.steps:.3f}s/eval mll={mll_m:.6f} resid={resid_m:.2e} "
+          f"phases={phases_m}", file=sys.stderr, flush=True)
+
+    # ---- plain float64 GPU path ----
+    fp64 = None
+    if not args.no_fp64:
+        run_steps(post, D, "fp64", args.fp64_warmup, params, report=True)
+        torch.cuda.synchronize()
+        _lib.profile_collect()
+        _lib.profile_enable(True)
+        t0 = time.perf_counter()
+        mll_d, res_d = run_steps(post, D, "fp64", args.fp64_steps, params)
+        torch.cuda.synchronize()
+        t_fp64 = time.perf_counter() - t0
+        _lib.profile_enable(False)
+        phases_d = _lib.profile_collect()
+        g_d = grads(params)
+        mll_d = float(mll_d.detach())
+        fp64 = dict(t=t_fp64 / args.fp64_steps, mll=mll_d, g=g_d, phases=phases_d,
+                    resid=float(res_d[_lib.RES_RESID]))
+        print(f"# fp64: {fp64['t']:.3f}s/eval mll={mll_d:.6f} phases={phases_d}",
+              file=sys.stderr, flush=True)
+
+    # ---- roofline of the dominant kernel (fused Sigma^-1 + adjoint, one launch per eval) ----
+    adj_ms, adj_cnt = phases_m["adjoint"]
+    adj_s = adj_ms / max(adj_cnt, 1) / 1e3
+    flops = 6.0 * lauum_flops(args.n)  # six bf16 products per fp32-class product
+    achieved = flops / adj_s / 1e12
+    roofline = {"kernel": "adjoint (fused L^-T L^-1 tile GEMM + gradient epilogue, bf16x6)",
+                "bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 4),
+                "traffic": None, "avg_launch_ms": round(adj_s * 1e3, 3)}
+
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
+
+    value = args.steps / t_mixed
+    line = {
+        "metric": METRIC, "value": round(value, 4), "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_mixed / args.steps * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x6-split/f32 with f64 panels", "data": "synthetic",
+        "config": {"workload": "exact GP MLL+grad, N=65536 D=16 RBF-ARD, RFF targets "
+                               "(gpjax_amd.workloads.exact_large)",
+                   "n": args.n, "d": 16, "params": 18, "precision": "mixed"},
+        "phases_ms": {k: round(v[0] / max(v[1], 1), 2) for k, v in phases_m.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if fp64 is not None:
+        rel_mll = abs(mll_m - fp64["mll"]) / abs(fp64["mll"])
+        rel_g = float(torch.linalg.norm(g_m - fp64["g"]) / torch.linalg.norm(fp64["g"]))
+        line["fp64_gpu"] = {"evals_per_s": round(1.0 / fp64["t"], 4),
+                            "ms_per_step": round(fp64["t"] * 1e3, 1),
+                            "phases_ms": {k: round(v[0] / max(v[1], 1), 2)
+                                          for k, v in fp64["phases"].items()},
+                            "tflops_fp64_equiv": round(
+                                (args.n ** 3) / fp64["t"] / 1e12, 2)}
+        line["speedup_vs_fp64"] = round(fp64["t"] / (t_mixed / args.steps), 3)
+        line["accuracy"] = {"rel_mll_vs_fp64": rel_mll, "rel_grad_vs_fp64": rel_g,
+                            "mll": mll_m, "mll_fp64": fp64["mll"],
+                            "refine_resid_mixed": resid_m,
+                            "gate_mll_1e-6": rel_mll <= 1e-6, "gate_grad_1e-4": rel_g <= 1e-4}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -37,7 +37,26 @@ SIGNATURES = {
     "gpk_mll_backward": (_I, [_I, _I, _P, _L, _L, _P, _I, _D, _P, _S, _P, _P]),
     "gpk_set_mixed_arith": (_I, [_I, _I, _I]),
     "gpk_mfma_selftest": (_I, [_P, _P, _P]),
+    "gpk_predict_workspace_bytes": (_S, [_L, _L, _L, _I, _I]),
+    "gpk_predict": (_I, [_I, _I, _P, _P, _L, _L, _P, _L, _P, _I, _D, _D, _I, _I, _P, _S, _P, _P,
+                         _P]),
+    "gpk_profile_enable": (_I, [_I]),
+    "gpk_profile_collect": (_I, [_P, _P]),
 }
+
+PHASES = ("gram", "potrf", "trtri", "solve", "adjoint")
+
+
+def profile_enable(on: bool) -> None:
+    call("gpk_profile_enable", 1 if on else 0)
+
+
+def profile_collect() -> dict:
+    """Per-phase (total ms, count) since the last collect; synchronises."""
+    ms = (ctypes.c_double * len(PHASES))()
+    cnt = (ctypes.c_int * len(PHASES))()
+    call("gpk_profile_collect", ctypes.cast(ms, _P), ctypes.cast(cnt, _P))
+    return {p: (ms[i], cnt[i]) for i, p in enumerate(PHASES)}
 
 
 class GPKError(RuntimeError):

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __rest
 #pragma unroll 1
     for (int e = 0; e < 16; ++e) {
       const int j = e >> 2, r = e & 3;
-      const int li = wm * 64 + i * 16 + 4 * (lane >> 4) + r;
+      const int li = wm * 64 + i * 16 + acc_row<AR>(lane, r);
       const int lj = wn * 64 + j * 16 + (lane & 15);
       const int64_t gi = m0 + li, gj = n0 + lj;
       if (gi >= n || gj >= n) continue;

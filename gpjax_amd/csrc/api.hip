@@ -15,7 +15,9 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
+#include <vector>
 
 #include "gpk_internal.h"
 
@@ -34,6 +36,37 @@ namespace {
 
 constexpr size_t ALIGN = 256;
 inline size_t al(size_t b) { return (b + ALIGN - 1) / ALIGN * ALIGN; }
+
+// ---- phase profiling: HIP events recorded on the launch stream ----
+struct PhaseRec {
+  int phase;
+  hipEvent_t a, b;
+};
+std::mutex g_prof_mu;
+std::vector<PhaseRec> g_prof;
+bool g_prof_on = false;
+
+struct PhaseScope {
+  int phase;
+  hipStream_t s;
+  hipEvent_t a = nullptr;
+  PhaseScope(int ph, hipStream_t st) : phase(ph), s(st) {
+    if (!g_prof_on) return;
+    if (hipEventCreate(&a) != hipSuccess) { a = nullptr; return; }
+    if (hipEventRecord(a, s) != hipSuccess) { (void)hipEventDestroy(a); a = nullptr; }
+  }
+  ~PhaseScope() {
+    if (!a) return;
+    hipEvent_t b = nullptr;
+    if (hipEventCreate(&b) != hipSuccess || hipEventRecord(b, s) != hipSuccess) {
+      (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+      return;
+    }
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    g_prof.push_back({phase, a, b});
+  }
+};
 
 // Arithmetic of the three product stages in the mixed path.
 int g_ar_potrf = AR_BF16X6;
@@ -293,10 +326,20 @@ int mll_forward_t(int kind, int prec, const double* x, const double* y, int64_t 
 
   hipLaunchKernelGGL(prep_r_kernel, dim3(nblk(np)), dim3(256), 0, s, y, theta, nls, n, np, r);
   GPK_LAUNCH_CHECK();
-  GPK_TRY(launch_kernel_matrix<TS>(kind, x, n, x, n, d, theta, nls, jitter, 2, L, np, np, np,
-                                   true, s));
-  GPK_TRY(F.potrf(0, np));
-  GPK_TRY(F.trtri(0, np));
+  {
+    PhaseScope ps(GPK_PHASE_GRAM, s);
+    GPK_TRY(launch_kernel_matrix<TS>(kind, x, n, x, n, d, theta, nls, jitter, 2, L, np, np, np,
+                                     true, s));
+  }
+  {
+    PhaseScope ps(GPK_PHASE_POTRF, s);
+    GPK_TRY(F.potrf(0, np));
+  }
+  {
+    PhaseScope ps(GPK_PHASE_TRTRI, s);
+    GPK_TRY(F.trtri(0, np));
+  }
+  PhaseScope ps_solve(GPK_PHASE_SOLVE, s);
   // alpha0 = X^T X r
   GPK_TRY(launch_trmv<TS>(X, np, np, r, u, false, part, s));
   GPK_TRY(launch_trmv<TS>(X, np, np, u, alpha, true, part, s));
@@ -324,6 +367,7 @@ int mll_backward_t(int kind, const double* x, int64_t n, int d, const double* th
   const int64_t np = Lo.np;
   const bool f64 = std::is_same<TS, double>::value;
   double* gpart = at<double>(ws, Lo.off_gpart);
+  PhaseScope ps(GPK_PHASE_ADJOINT, s);
   GPK_TRY(launch_adjoint<TS>(kind, f64 ? (int)AR_F64 : g_ar_lauum, at<TS>(ws, Lo.off_X), np, np,
                              x, n, d, theta, nls, at<double>(ws, Lo.off_alpha), gpart, s));
   const int64_t nleaf = np / NB;
@@ -331,6 +375,83 @@ int mll_backward_t(int kind, const double* x, int64_t n, int d, const double* th
                      nleaf * (nleaf + 1) / 2, nls, theta, at<double>(ws, Lo.off_alpha), n, gbar,
                      grad);
   GPK_LAUNCH_CHECK();
+  return GPK_OK;
+}
+
+// ---- predict: ConjugatePosterior.predict (gpjax/gps.py:494-526) ----
+struct PredLayout {
+  int64_t tp = 0;
+  size_t off_kxt = 0, off_a = 0, off_ktt = 0, off_part = 0, off_vec = 0, off_res = 0, total = 0;
+};
+
+PredLayout make_pred_layout(const Layout& Lo, int64_t t, int full_cov) {
+  PredLayout P;
+  P.tp = round_up(t, NB);
+  const int64_t np = Lo.np, tp = P.tp;
+  size_t o = Lo.total;
+  P.off_kxt = o; o += al((size_t)np * tp * sizeof(double));
+  P.off_a = o; o += al((size_t)np * tp * sizeof(double));
+  P.off_ktt = o; o += al((size_t)(full_cov ? tp * tp : 1) * sizeof(double));
+  P.off_part = o; o += al((size_t)((np + 1023) / 1024) * tp * sizeof(double));
+  P.off_vec = o; o += al((size_t)tp * sizeof(double));
+  P.off_res = o; o += al((size_t)GPK_RESULT_LEN * sizeof(double));
+  P.total = o;
+  return P;
+}
+
+__global__ void pred_mean_kernel(const double* __restrict__ v, const double* __restrict__ theta,
+                                 int nls, int64_t t, double* __restrict__ mean) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < t) mean[j] = theta[nls + 2] + v[j];
+}
+
+__global__ void pred_var_kernel(const double* __restrict__ colsq,
+                                const double* __restrict__ theta, int nls, double jitter_prior,
+                                int64_t t, double* __restrict__ var_out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < t) var_out[j] = (theta[nls] + jitter_prior) - colsq[j];
+}
+
+template <typename TS>
+int predict_t(int kind, int prec, const double* x, const double* y, int64_t n, int d,
+              const double* xt, int64_t t, const double* theta, int nls, double jitter_post,
+              double jitter_prior, int full_cov, int iters, void* ws, const Layout& Lo,
+              const PredLayout& P, double* mean, double* cov, hipStream_t s) {
+  const int64_t np = Lo.np, tp = P.tp;
+  double* result = at<double>(ws, P.off_res);
+  // L, X = L^-1 and alpha = Sigma^-1 (y - m), Sigma with the posterior's jitter (gps.py:506-511)
+  GPK_TRY(mll_forward_t<TS>(kind, prec, x, y, n, d, theta, nls, jitter_post, iters, ws, Lo,
+                            result, s));
+  double* kxt = at<double>(ws, P.off_kxt);
+  double* A = at<double>(ws, P.off_a);
+  double* part = at<double>(ws, P.off_part);
+  double* vec = at<double>(ws, P.off_vec);
+  // Kxt (gps.py:515), zero outside n x t
+  GPK_TRY(launch_kernel_matrix<double>(kind, x, n, xt, t, d, theta, nls, 0.0, KM_ZERO_PAD, kxt,
+                                       tp, np, tp, false, s));
+  // mean = m(t) + Kxt^T alpha  (= m(t) + (L^-1 Kxt)^T L^-1 (y - m), gps.py:517-520)
+  GPK_TRY(launch_colreduce(kxt, tp, np, tp, at<double>(ws, Lo.off_alpha), vec, part, s));
+  hipLaunchKernelGGL(pred_mean_kernel, dim3(nblk(t)), dim3(256), 0, s, vec, theta, nls, t, mean);
+  GPK_LAUNCH_CHECK();
+  // A = L^-1 Kxt  (X lower: k <= row tile)
+  GPK_TRY(launch_gemm<TS, double, double>(AR_F64, false, true, at<TS>(ws, Lo.off_X), np, kxt, tp,
+                                          A, tp, np, tp, np, 1.0, false, false, KR_LE_M, s));
+  if (full_cov) {
+    // cov = Ktt + jitter_prior I - A^T A  (gps.py:514,522-523)
+    double* ktt = at<double>(ws, P.off_ktt);
+    GPK_TRY(launch_kernel_matrix<double>(kind, xt, t, xt, t, d, theta, nls, jitter_prior,
+                                         1 | KM_ZERO_PAD, ktt, tp, tp, tp, false, s));
+    GPK_TRY(launch_gemm<double, double, double>(AR_F64, true, true, A, tp, A, tp, ktt, tp, tp, tp,
+                                                np, -1.0, true, false, KR_FULL, s));
+    GPK_HIP_TRY(hipMemcpy2DAsync(cov, (size_t)t * sizeof(double), ktt, (size_t)tp * sizeof(double),
+                                 (size_t)t * sizeof(double), (size_t)t, hipMemcpyDeviceToDevice, s));
+  } else {
+    // diagonal only: var_j = k(0) + jitter_prior - sum_i A_ij^2
+    GPK_TRY(launch_colreduce(A, tp, np, tp, nullptr, vec, part, s));
+    hipLaunchKernelGGL(pred_var_kernel, dim3(nblk(t)), dim3(256), 0, s, vec, theta, nls,
+                       jitter_prior, t, cov);
+    GPK_LAUNCH_CHECK();
+  }
   return GPK_OK;
 }
 
@@ -433,6 +554,74 @@ int gpk_mll_backward(int kind, int prec, const double* x, int64_t n, int64_t d,
   if (prec == GPK_PREC_FP64)
     return mll_backward_t<double>(kind, x, n, (int)d, theta, nls, gbar_host, ws, Lo, grad, s);
   return mll_backward_t<float>(kind, x, n, (int)d, theta, nls, gbar_host, ws, Lo, grad, s);
+}
+
+size_t gpk_predict_workspace_bytes(int64_t n, int64_t d, int64_t t, int prec, int full_cov) {
+  if (n < 1 || d < 1 || t < 1) return 0;
+  return make_pred_layout(make_layout(n, d, prec), t, full_cov).total;
+}
+
+int gpk_predict(int kind, int prec, const double* x, const double* y, int64_t n, int64_t d,
+                const double* xt, int64_t t, const double* theta, int nls, double jitter_post,
+                double jitter_prior, int full_cov, int refine_iters, void* ws, size_t ws_bytes,
+                double* mean, double* cov, void* stream) {
+  int rc = check_common(kind, prec, x, n, d, theta, nls);
+  if (rc) return rc;
+  if (!y || !xt || t < 1 || !ws || !mean || !cov) {
+    set_error("predict: null y/xt/ws/mean/cov or t < 1");
+    return GPK_ERR_BAD_ARG;
+  }
+  const Layout Lo = make_layout(n, d, prec);
+  const PredLayout P = make_pred_layout(Lo, t, full_cov);
+  if (ws_bytes < P.total) {
+    set_error("predict: workspace %zu bytes < required %zu", ws_bytes, P.total);
+    return GPK_ERR_WORKSPACE;
+  }
+  if (reinterpret_cast<uintptr_t>(ws) % ALIGN) {
+    set_error("predict: workspace must be %zu-byte aligned", ALIGN);
+    return GPK_ERR_BAD_ARG;
+  }
+  const int iters = refine_iters >= 0 ? refine_iters : (prec == GPK_PREC_FP64 ? 1 : 4);
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == GPK_PREC_FP64)
+    return predict_t<double>(kind, prec, x, y, n, (int)d, xt, t, theta, nls, jitter_post,
+                             jitter_prior, full_cov, iters, ws, Lo, P, mean, cov, s);
+  return predict_t<float>(kind, prec, x, y, n, (int)d, xt, t, theta, nls, jitter_post,
+                          jitter_prior, full_cov, iters, ws, Lo, P, mean, cov, s);
+}
+
+int gpk_profile_enable(int on) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  g_prof_on = on != 0;
+  return GPK_OK;
+}
+
+int gpk_profile_collect(double* ms_host, int* counts_host) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  for (int p = 0; p < GPK_PHASE_COUNT; ++p) {
+    if (ms_host) ms_host[p] = 0.0;
+    if (counts_host) counts_host[p] = 0;
+  }
+  int rc = GPK_OK;
+  for (auto& rec : g_prof) {
+    float ms = 0.f;
+    if (rc == GPK_OK) {
+      hipError_t e = hipEventSynchronize(rec.b);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, rec.a, rec.b);
+      if (e != hipSuccess) {
+        set_error("profile: %s", hipGetErrorString(e));
+        rc = GPK_ERR_HIP;
+      }
+    }
+    if (rc == GPK_OK && rec.phase >= 0 && rec.phase < GPK_PHASE_COUNT) {
+      if (ms_host) ms_host[rec.phase] += ms;
+      if (counts_host) counts_host[rec.phase] += 1;
+    }
+    (void)hipEventDestroy(rec.a);
+    (void)hipEventDestroy(rec.b);
+  }
+  g_prof.clear();
+  return rc;
 }
 
 int gpk_mfma_selftest(double* out_f64, float* out_bf16, void* stream) {

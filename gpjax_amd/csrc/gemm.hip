@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + r;
+        const int64_t row = m0 + wm * 64 + i * 16 + acc_row<AR>(lane, r);
         const int64_t colx = n0 + wn * 64 + j * 16 + (lane & 15);
         TC* c = C + row * p.ldc + colx;
         double v = p.alpha * (double)acc[i][j][r];
@@ -125,5 +125,9 @@ template int launch_gemm<float, double, float>(int, bool, bool, const float*, in
                                                const double*, int64_t, float*, int64_t,
                                                int64_t, int64_t, int64_t, double, bool, bool,
                                                int, hipStream_t);
+template int launch_gemm<float, double, double>(int, bool, bool, const float*, int64_t,
+                                                const double*, int64_t, double*, int64_t,
+                                                int64_t, int64_t, int64_t, double, bool, bool,
+                                                int, hipStream_t);
 
 }  // namespace gpk

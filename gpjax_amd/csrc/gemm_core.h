@@ -16,9 +16,10 @@
 //
 // Matrix-core lane layouts used (gfx950, wave64):
 //   16x16x4 f64 : lane l supplies A[l%16][l/16], B[l/16][l%16];
-//                 acc element r of lane l is D[4*(l/16) + r][l%16].
+//                 acc element r of lane l is D[l/16 + 4*r][l%16].
 //   16x16x32 bf16: lane l supplies A[l%16][8*(l/16) .. +7],
-//                 B[8*(l/16) .. +7][l%16]; acc layout as above.
+//                 B[8*(l/16) .. +7][l%16]; acc element r of lane l is
+//                 D[4*(l/16) + r][l%16].
 // gpk_mfma_selftest checks both layouts on the device.
 #pragma once
 
@@ -259,6 +260,12 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
       }
     __syncthreads();
   }
+}
+
+// Row (within a 16 x 16 block) of accumulator element r held by `lane`.
+template <int AR>
+__device__ __forceinline__ int acc_row(int lane, int r) {
+  return AR == AR_F64 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
 }
 
 // Tile decode for lower-triangular tile sets: t -> (mt, nt), mt >= nt, with

@@ -97,7 +97,10 @@ __device__ __forceinline__ void kval_fac(int kind, double d2, double& k, double&
 
 // Gram / cross-covariance tiles.  Rows i < rows_total, cols j < cols_total are
 // written; entries with i >= n or j >= m are the identity pad (1 on i == j,
-// else 0).  diag_mode: 0 none, 1 add jitter, 2 add jitter + obs_sd^2 (i == j < n).
+// else 0).  diag_mode & KM_DIAG_MASK: 0 none, 1 add jitter, 2 add jitter +
+// obs_sd^2 (i == j < n); diag_mode | KM_ZERO_PAD makes the pad all zeros.
+constexpr int KM_DIAG_MASK = 3;
+constexpr int KM_ZERO_PAD = 4;
 // lower_only: only 128-tiles on or below the diagonal (rows_total == cols_total).
 template <typename TS>
 int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, int64_t m,
@@ -145,6 +148,11 @@ int launch_trmv(const TS* X, int64_t ld, int64_t np, const double* r, double* y,
 int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
                         const double* theta, int nls, double jitter, const double* a, double* v,
                         double* partial, hipStream_t s);
+
+// y_j = sum_i M[i][j] v_i (v non-null) or sum_i M[i][j]^2 (v null), dense
+// rows x cols float64; partial holds ceil(rows/1024) x cols doubles.
+int launch_colreduce(const double* M, int64_t ld, int64_t rows, int64_t cols, const double* v,
+                     double* y, double* partial, hipStream_t s);
 
 int launch_mfma_selftest(double* out64, float* outbf, hipStream_t s);
 

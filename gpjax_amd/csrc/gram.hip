@@ -96,12 +96,13 @@ __global__ __launch_bounds__(256) void kmat_kernel(int kind, const double* __res
     double v;
     if (i < n && j < m) {
       v = var * kval(kind, d2[r]);
-      if (diag_mode > 0 && i == j) {
+      const int dm = diag_mode & KM_DIAG_MASK;
+      if (dm > 0 && i == j) {
         v = v + jitter;
-        if (diag_mode > 1) v = v + sd * sd;
+        if (dm > 1) v = v + sd * sd;
       }
     } else {
-      v = (i == j) ? 1.0 : 0.0;
+      v = (i == j && !(diag_mode & KM_ZERO_PAD)) ? 1.0 : 0.0;
     }
     out[i * ldo + j] = (TS)v;
   }

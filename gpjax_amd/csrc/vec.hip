@@ -140,6 +140,34 @@ __global__ void sum_chunks(const double* __restrict__ partial, int64_t np, int64
   y[i] = s;
 }
 
+// y_j partial over rows [c*RC, (c+1)*RC) of a dense rows x cols float64 matrix:
+// sum_i M[i][j] v_i, or sum_i M[i][j]^2 when v is null.  One thread per column.
+__global__ __launch_bounds__(256) void colreduce_kernel(const double* __restrict__ M, int64_t ld,
+                                                        int64_t rows, int64_t cols,
+                                                        const double* __restrict__ v,
+                                                        double* __restrict__ partial) {
+  const int64_t c = blockIdx.y;
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= cols) return;
+  const int64_t i0 = c * TRMV_RC;
+  const int64_t i1 = (i0 + TRMV_RC < rows) ? i0 + TRMV_RC : rows;
+  double acc = 0.0;
+  for (int64_t i = i0; i < i1; ++i) {
+    const double m = M[i * ld + j];
+    acc += v ? m * v[i] : m * m;
+  }
+  partial[c * cols + j] = acc;
+}
+
+__global__ void colreduce_sum(const double* __restrict__ partial, int64_t cols, int64_t nchunks,
+                              double* __restrict__ y) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  double s = 0.0;
+  for (int64_t c = 0; c < nchunks; ++c) s += partial[c * cols + j];
+  y[j] = s;
+}
+
 // ---- matrix-core lane layout self-test ----
 // A[i][k] = i + 100 k, B[k][j] = 1000 + j - 7 k: D = A B^T-as-B computed per lane.
 __global__ void mfma_selftest_kernel(double* out64, float* outbf) {
@@ -219,6 +247,19 @@ int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
   GPK_LAUNCH_CHECK();
   hipLaunchKernelGGL(sum_chunks, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, partial, np,
                      nc, v);
+  GPK_LAUNCH_CHECK();
+  return GPK_OK;
+}
+
+int launch_colreduce(const double* M, int64_t ld, int64_t rows, int64_t cols, const double* v,
+                     double* y, double* partial, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return GPK_OK;
+  const int64_t nc = (rows + TRMV_RC - 1) / TRMV_RC;
+  hipLaunchKernelGGL(colreduce_kernel, dim3((unsigned)((cols + 255) / 256), (unsigned)nc),
+                     dim3(256), 0, s, M, ld, rows, cols, v, partial);
+  GPK_LAUNCH_CHECK();
+  hipLaunchKernelGGL(colreduce_sum, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s,
+                     partial, cols, nc, y);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }

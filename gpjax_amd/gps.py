@@ -45,6 +45,46 @@ class ConjugatePosterior:
     def __call__(self, test_inputs, train_data):
         return self.predict(test_inputs, train_data)
 
+    def predict(self, test_inputs: torch.Tensor, train_data, full_covariance: bool = True,
+                precision: str | None = None) -> GaussianDistribution:
+        """Latent predictive N(mean, cov) at ``test_inputs`` (``gps.py:443-526``).
+
+        Sigma uses this posterior's jitter (``gps.py:506``) and the predictive
+        covariance the prior's (``gps.py:523``).  With ``full_covariance=False``
+        only the diagonal is formed (a [T] variance vector is held as a diagonal
+        matrix), which avoids the T x T product for large T.  Values only: no
+        autograd graph is recorded through predict.
+        """
+        from . import _lib, config
+        from .objectives import _mean_constant, _precision_id
+
+        kernel = self.prior.kernel
+        X = kernel.slice_input(train_data.X)
+        T = kernel.slice_input(test_inputs)
+        y = train_data.y.reshape(-1).to(torch.float64).contiguous()
+        n, d = X.shape
+        t = T.shape[0]
+        if T.shape[1] != d:
+            raise ValueError("test inputs and training inputs differ in dimension")
+        nls = kernel.nls(d)
+        prec = _precision_id(precision)
+        with torch.no_grad():
+            c = _mean_constant(self.prior.mean_function, X.device)
+            theta = kernel.theta_tensor(d, self.likelihood.obs_stddev.value, c)
+            full = 1 if full_covariance else 0
+            nbytes = _lib.lib().gpk_predict_workspace_bytes(n, d, t, prec, full)
+            ws = torch.empty(nbytes, dtype=torch.uint8, device=X.device)
+            mean = torch.empty(t, dtype=torch.float64, device=X.device)
+            cov = torch.empty((t, t) if full else (t,), dtype=torch.float64, device=X.device)
+            iters = -1 if config.refine_iters is None else int(config.refine_iters)
+            _lib.call("gpk_predict", kernel.kind_id, prec, _lib.ptr(X), _lib.ptr(y), n, d,
+                      _lib.ptr(T), t, _lib.ptr(theta), nls, float(self.jitter),
+                      float(self.prior.jitter), full, iters, _lib.ptr(ws), nbytes,
+                      _lib.ptr(mean), _lib.ptr(cov), _lib.stream_ptr(X.device))
+        if not full:
+            cov = torch.diag(cov)
+        return GaussianDistribution(mean, cov)
+
 
 def construct_posterior(prior: Prior, likelihood):
     if isinstance(likelihood, Gaussian):

@@ -111,6 +111,23 @@ int gpk_mll_backward(int kind, int prec, const double* x, int64_t n, int64_t d,
                      const double* theta, int nls, double gbar_host, void* ws, size_t ws_bytes,
                      double* grad, void* stream);
 
+/* Bytes of workspace gpk_predict needs for (n, d, t test points, prec, full_cov). */
+size_t gpk_predict_workspace_bytes(int64_t n, int64_t d, int64_t t, int prec, int full_cov);
+
+/*
+ * Conjugate posterior predictive at t test inputs xt [t, d]
+ * (ConjugatePosterior.predict, gpjax/gps.py:443-526):
+ *   mean = c + Kxt^T Sigma^-1 (y - c),  Sigma = K + (jitter_post + obs_sd^2) I,
+ *   cov  = Ktt - (L^-1 Kxt)^T (L^-1 Kxt) + jitter_prior I.
+ * full_cov != 0 writes the dense t x t covariance (row-major, ld = t) to cov;
+ * full_cov == 0 writes only its diagonal (t entries).  The factorisation runs
+ * in `prec`; everything after it runs in float64.
+ */
+int gpk_predict(int kind, int prec, const double* x, const double* y, int64_t n, int64_t d,
+                const double* xt, int64_t t, const double* theta, int nls, double jitter_post,
+                double jitter_prior, int full_cov, int refine_iters, void* ws, size_t ws_bytes,
+                double* mean, double* cov, void* stream);
+
 /*
  * Products per element used by the GPK_PREC_MIXED stages (3 = bf16 hi/lo
  * split, 6 = hi/mid/lo split): Cholesky updates, triangular inverse, and the
@@ -119,6 +136,22 @@ int gpk_mll_backward(int kind, int prec, const double* x, int64_t n, int64_t d,
  * against the accuracy figures the benchmark reports.
  */
 int gpk_set_mixed_arith(int potrf_products, int trtri_products, int lauum_products);
+
+/*
+ * Phase profiling (no counterpart in the reference).  While enabled, each
+ * phase of gpk_mll_forward / gpk_mll_backward is bracketed by HIP events
+ * recorded on the call's stream.  gpk_profile_collect synchronises on those
+ * events, writes the summed milliseconds and call counts per phase into
+ * host arrays of GPK_PHASE_COUNT entries, and clears the record.
+ */
+#define GPK_PHASE_GRAM 0    /* Sigma assembly                                  */
+#define GPK_PHASE_POTRF 1   /* recursive Cholesky (leaves, TRSM, SYRK)          */
+#define GPK_PHASE_TRTRI 2   /* recursive triangular inverse                    */
+#define GPK_PHASE_SOLVE 3   /* alpha, float64 refinement, log-det, objective    */
+#define GPK_PHASE_ADJOINT 4 /* fused Sigma^-1 + gradient pass (backward)        */
+#define GPK_PHASE_COUNT 5
+int gpk_profile_enable(int on);
+int gpk_profile_collect(double* ms_host, int* counts_host);
 
 /* Self-test of the matrix-core lane layouts (writes 64*4 doubles + 64*4 floats). */
 int gpk_mfma_selftest(double* out_f64, float* out_bf16, void* stream);

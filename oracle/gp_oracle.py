@@ -250,6 +250,72 @@ def collapsed_elbo(kind, X, y, Z, ls, var, obs_sd, jitter=1e-6, mean=0.0) -> flo
     return (two_log_prob - two_trace) / 2.0                                       # :416
 
 
+def collapsed_elbo_grad(kind, X, y, Z, ls, var, obs_sd, jitter=1e-6, mean=0.0) -> dict:
+    """Collapsed ELBO and its gradient w.r.t. (l, var, obs_sd, c, Z), constrained coordinates.
+
+    Same objective as :func:`collapsed_elbo` (``objectives.py:342-416``), written
+    through the N-summed statistics P = Kzx Kxz, b = Kzx r, q = Kzx 1 and
+    C = Kzz + P / s (s = obs_sd^2), using the determinant and inversion lemmas
+    the reference's own comments state (``objectives.py:361-385``):
+      log|B| = log|C| - log|Kzz|,  ||L^-1 A r||^2 = b^T C^-1 b / s,
+      tr(A A^T) = tr(Kzz^-1 P) / s,  sum diag(Kxx) = n var (stationary).
+    Differentiating F gives 2 dF = 2 sum(W * dKzx) + tr(G_K dKzz) + g_s ds - n/s dvar
+    with c = C^-1 b,
+      G_P = -C^-1/s - c c^T/s^3 + Kzz^-1/s,  W = G_P Kzx + (c/s^2) r^T,
+      G_K = Kzz^-1 - C^-1 - c c^T/s^2 - Kzz^-1 P Kzz^-1/s,
+      g_s = -n/s + tr(C^-1 P)/s^2 + r^T r/s^2 + c^T P c/s^4 - 2 b^T c/s^3
+            + n var/s^2 - tr(Kzz^-1 P)/s^2,
+    and dKzx / dtheta, dKzx / dZ from the stationary kernel derivative
+    (dk/drho^2 = -var F / 2, F as in :func:`unit_kernel_dlog_factor`).
+    Returns ``{"elbo", "lengthscale", "variance", "obs_stddev", "mean", "Z"}``.
+    """
+    X = np.asarray(X, np.float64)
+    Z = np.asarray(Z, np.float64)
+    y = np.asarray(y, np.float64).reshape(-1)
+    n, D = X.shape
+    M = Z.shape[0]
+    s = float(obs_sd) ** 2
+    lsv = _as_ls(ls, D)
+    r = y - float(mean)
+    xs, zs = X / lsv, Z / lsv
+    d2 = squared_distance(zs, xs)
+    ku, F = unit_kernel_from_d2(kind, d2), unit_kernel_dlog_factor(kind, d2)
+    Kzx = float(var) * ku
+    P, b, q = Kzx @ Kzx.T, Kzx @ r, Kzx.sum(1)
+    d2z = squared_distance(zs, zs)
+    kzu, Fz = unit_kernel_from_d2(kind, d2z), unit_kernel_dlog_factor(kind, d2z)
+    Kzz = add_jitter(float(var) * kzu, jitter)
+    Lk = np.linalg.cholesky(Kzz)
+    C = Kzz + P / s
+    Lc = np.linalg.cholesky(C)
+    c = sla.cho_solve((Lc, True), b)
+    Kinv = sla.cho_solve((Lk, True), np.eye(M))
+    Cinv = sla.cho_solve((Lc, True), np.eye(M))
+    rr = float(r @ r)
+    trKP = float(np.sum(Kinv * P))
+    elbo = 0.5 * (-n * math.log(2 * math.pi * s) - 2 * np.sum(np.log(np.diag(Lc)))
+                  + 2 * np.sum(np.log(np.diag(Lk))) - rr / s + float(b @ c) / s ** 2
+                  - n * float(var) / s + trKP / s)
+    GP = -Cinv / s - np.outer(c, c) / s ** 3 + Kinv / s
+    W = GP @ Kzx + np.outer(c / s ** 2, r)
+    GK = Kinv - Cinv - np.outer(c, c) / s ** 2 - Kinv @ P @ Kinv / s
+    gs = (-n / s + float(np.sum(Cinv * P)) / s ** 2 + rr / s ** 2 + float(c @ P @ c) / s ** 4
+          - 2 * float(b @ c) / s ** 3 + n * float(var) / s ** 2 - trKP / s ** 2)
+    WF, GF = W * F, GK * Fz
+    g_ls = np.empty(D)
+    g_Z = np.empty((M, D))
+    for d in range(D):
+        dzx = zs[:, d][:, None] - xs[:, d][None, :]
+        dzz = zs[:, d][:, None] - zs[:, d][None, :]
+        g_ls[d] = float(var) * (float(np.sum(WF * dzx * dzx))
+                                + 0.5 * float(np.sum(GF * dzz * dzz))) / lsv[d]
+        g_Z[:, d] = -float(var) * (np.sum(WF * dzx, 1) + np.sum(GF * dzz, 1)) / lsv[d]
+    g_var = float(np.sum(W * ku)) + 0.5 * float(np.sum(GK * kzu)) - 0.5 * n / s
+    return {"elbo": elbo, "lengthscale": g_ls if np.ndim(ls) else float(np.sum(g_ls)),
+            "variance": g_var, "obs_stddev": gs * float(obs_sd),
+            "mean": float(np.sum(r)) / s - float(c @ q) / s ** 2, "Z": g_Z}
+
+
 def predict(kind, X, y, T, ls, var, obs_sd, jitter_post=1e-6, jitter_prior=1e-6, mean=0.0):
     """``ConjugatePosterior.predict`` (``gps.py:494-526``): latent mean [T] and covariance [T,T].
 

@@ -49,7 +49,7 @@ def test_mfma_lane_layouts(cuda):
     D = A @ B
     l = np.arange(64)
     got = o64.cpu().numpy().reshape(64, 4)
-    exp = np.stack([D[4 * (l // 16) + r, l % 16] for r in range(4)], 1)
+    exp = np.stack([D[l // 16 + 4 * r, l % 16] for r in range(4)], 1)
     np.testing.assert_array_equal(got, exp)
     k = np.arange(32)
     Ab = i[:, None] + 2 * (k[None, :] % 8) - k[None, :] // 8

@@ -1,0 +1,65 @@
+"""GPU parity of ConjugatePosterior.predict (gpk_predict) against the float64 oracle.
+
+Cases: the N=2,000, D=1 RBF workload with 500 test points on [-3.5, 3.5]
+(``workloads.exact_small``), and smaller Matern ARD problems.  FP64 path:
+mean within 1e-9 relative (norm-wise), covariance within 1e-9 of the largest
+entry; mixed path: mean within 1e-6, covariance within 1e-5.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import gpjax_amd as gpx
+from gpjax_amd import workloads
+from oracle import gp_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+KCLS = {"rbf": gpx.kernels.RBF, "matern12": gpx.kernels.Matern12,
+        "matern32": gpx.kernels.Matern32, "matern52": gpx.kernels.Matern52}
+
+
+def _post(kind, ls, var, sd, n, mean=0.0):
+    k = KCLS[kind](lengthscale=ls, variance=var)
+    mf = gpx.mean_functions.Constant(mean)
+    return gpx.gps.Prior(mean_function=mf, kernel=k) * \
+        gpx.likelihoods.Gaussian(num_datapoints=n, obs_stddev=sd)
+
+
+def _check(cuda, kind, X, y, T, ls, var, sd, mean, precision, tol_mu, tol_cov):
+    post = _post(kind, ls, var, sd, X.shape[0], mean)
+    D = gpx.Dataset(X=torch.tensor(X, device=cuda), y=torch.tensor(y[:, None], device=cuda))
+    Tt = torch.tensor(T, device=cuda)
+    dist = post.predict(Tt, D, precision=precision)
+    mu_ref, cov_ref = O.predict(kind, X, y, T, ls, var, sd, 1e-6, 1e-6, mean)
+    mu = dist.mean.cpu().numpy()
+    cov = dist.covariance().cpu().numpy()
+    assert np.linalg.norm(mu - mu_ref) <= tol_mu * np.linalg.norm(mu_ref)
+    assert np.max(np.abs(cov - cov_ref)) <= tol_cov * np.max(np.abs(cov_ref))
+    diag = post.predict(Tt, D, full_covariance=False, precision=precision).variance.cpu().numpy()
+    np.testing.assert_allclose(diag, np.diag(cov), rtol=0, atol=tol_cov * np.max(np.abs(cov_ref)))
+    noisy = post.likelihood.predict(dist)
+    np.testing.assert_allclose(torch.diagonal(noisy.covariance()).cpu().numpy(),
+                               np.diag(cov) + sd ** 2, rtol=1e-14)
+
+
+def test_predict_exact_small_workload(cuda):
+    w = workloads.exact_small()
+    _check(cuda, w["kind"], w["X"], w["y"], w["T"], w["lengthscale"], w["variance"],
+           w["obs_stddev"], 0.0, "fp64", 1e-9, 1e-9)
+
+
+@pytest.mark.parametrize("kind", ["matern12", "matern32", "matern52", "rbf"])
+def test_predict_ard(cuda, kind):
+    rng = np.random.default_rng(5)
+    X = rng.uniform(-2, 2, (333, 3))
+    y = np.sin(X[:, 0]) * np.cos(X[:, 1]) + 0.05 * rng.standard_normal(333)
+    T = rng.uniform(-2.5, 2.5, (77, 3))
+    _check(cuda, kind, X, y, T, np.array([0.8, 1.3, 2.0]), 1.4, 0.2, 0.3, "fp64", 1e-9, 1e-9)
+
+
+def test_predict_mixed(cuda):
+    w = workloads.exact_small()
+    _check(cuda, w["kind"], w["X"], w["y"], w["T"], w["lengthscale"], w["variance"],
+           w["obs_stddev"], 0.0, "mixed", 1e-6, 1e-5)

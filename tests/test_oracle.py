@@ -238,6 +238,68 @@ def test_collapsed_elbo_dense_form(kind):
 
 
 @pytest.mark.parametrize("kind", KINDS)
+def test_collapsed_elbo_grad_value_and_fd(kind):
+    rng = np.random.default_rng(21)
+    X = rng.uniform(0, 1, (120, 3))
+    y = np.sin(6 * X[:, 0]) + 0.1 * rng.standard_normal(120)
+    Z = X[rng.choice(120, 25, replace=False)] + 0.01
+    ls, var, sd, c = np.array([0.3, 0.5, 0.4]), 1.3, 0.2, 0.1
+    g = O.collapsed_elbo_grad(kind, X, y, Z, ls, var, sd, 1e-6, c)
+    f = lambda Z_=Z, l=ls, v=var, s=sd, m=c: O.collapsed_elbo(kind, X, y, Z_, l, v, s, 1e-6, m)
+    assert g["elbo"] == pytest.approx(f(), rel=1e-8)
+    h = 1e-6
+    fd_l = np.array([(f(l=ls + h * e) - f(l=ls - h * e)) / (2 * h) for e in np.eye(3)])
+    np.testing.assert_allclose(g["lengthscale"], fd_l, rtol=1e-5)
+    assert g["variance"] == pytest.approx((f(v=var + h) - f(v=var - h)) / (2 * h), rel=1e-5)
+    assert g["obs_stddev"] == pytest.approx((f(s=sd + h) - f(s=sd - h)) / (2 * h), rel=1e-5)
+    assert g["mean"] == pytest.approx((f(m=c + h) - f(m=c - h)) / (2 * h), rel=1e-5)
+    for (i, d) in [(0, 0), (7, 2), (24, 1)]:
+        E = np.zeros_like(Z)
+        E[i, d] = h
+        assert g["Z"][i, d] == pytest.approx((f(Z_=Z + E) - f(Z_=Z - E)) / (2 * h), rel=1e-4,
+                                             abs=1e-6)
+
+
+def test_collapsed_elbo_grad_vs_torch_autograd():
+    """Autograd of an independent torch transcription of objectives.py:342-416."""
+    rng = np.random.default_rng(22)
+    X = rng.uniform(0, 1, (80, 2))
+    y = np.cos(4 * X[:, 1]) + 0.1 * rng.standard_normal(80)
+    Z = X[rng.choice(80, 15, replace=False)] + 0.02
+    ls, var, sd, c, jit = np.array([0.35, 0.45]), 0.9, 0.25, -0.3, 1e-6
+    g = O.collapsed_elbo_grad("rbf", X, y, Z, ls, var, sd, jit, c)
+    T = lambda a: torch.tensor(a, dtype=torch.float64, requires_grad=True)
+    tl, tv, ts, tc, tZ = T(ls), T(var), T(sd), T(c), T(Z)
+    Xt, yt = torch.tensor(X), torch.tensor(y)
+
+    def k(A, B):
+        d = A[:, None, :] / tl - B[None, :, :] / tl
+        return tv * torch.exp(-0.5 * (d * d).sum(-1))
+
+    noise = ts ** 2
+    Kzz = k(tZ, tZ) + jit * torch.eye(15, dtype=torch.float64)
+    Kzx = k(tZ, Xt)
+    Lz = torch.linalg.cholesky(Kzz)
+    A = torch.linalg.solve_triangular(Lz, Kzx, upper=False) / torch.sqrt(noise)
+    AAT = A @ A.T
+    L = torch.linalg.cholesky(torch.eye(15, dtype=torch.float64) + AAT)
+    diff = yt - tc
+    LAd = torch.linalg.solve_triangular(L, (A @ diff)[:, None], upper=False)[:, 0]
+    quad = ((diff ** 2).sum() - (LAd ** 2).sum()) / noise
+    two_lp = -80 * torch.log(2 * math.pi * noise) - 2 * torch.log(torch.diagonal(L)).sum() - quad
+    elbo = (two_lp - (80 * tv / noise - torch.trace(AAT))) / 2
+    elbo.backward()
+    # The lemma-based form and the reference's triangular-solve form differ in
+    # rounding by ~1e-9 relative on this ill-conditioned RBF Kzz.
+    assert g["elbo"] == pytest.approx(elbo.item(), rel=1e-8)
+    np.testing.assert_allclose(g["lengthscale"], tl.grad.numpy(), rtol=1e-6)
+    assert g["variance"] == pytest.approx(tv.grad.item(), rel=1e-6)
+    assert g["obs_stddev"] == pytest.approx(ts.grad.item(), rel=1e-6)
+    assert g["mean"] == pytest.approx(tc.grad.item(), rel=1e-6)
+    np.testing.assert_allclose(g["Z"], tZ.grad.numpy(), rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.parametrize("kind", KINDS)
 def test_predict_dense_solve(kind):
     X, y = _data(50, 2, 11)
     T = np.random.default_rng(12).uniform(-2.5, 2.5, (16, 2))
