@@ -7,9 +7,10 @@ User code shaped like GPJax's examples works against these modules:
 All arithmetic runs in the HIP library ``libgpk.so`` (C ABI in include/gpk.h).
 """
 
-from . import config, gps, kernels, likelihoods, mean_functions, objectives, parameters
+from . import (config, gps, kernels, likelihoods, mean_functions, objectives, parameters,
+               variational_families)
 from .dataset import Dataset
 
 __version__ = "0.1.0"
 __all__ = ["Dataset", "config", "gps", "kernels", "likelihoods", "mean_functions",
-           "objectives", "parameters"]
+           "objectives", "parameters", "variational_families"]

@@ -19,7 +19,7 @@
 #include <type_traits>
 #include <vector>
 
-#include "gpk_internal.h"
+#include "factor.h"
 
 namespace gpk {
 
@@ -82,8 +82,6 @@ struct Layout {
   size_t off_part = 0, off_gpart = 0, off_scal = 0, total = 0;
 };
 
-int64_t split_point(int64_t n) { return (n / NB / 2) * NB; }
-
 Layout make_layout(int64_t n, int64_t d, int prec) {
   Layout L;
   L.np = round_up(n, 256);
@@ -112,74 +110,6 @@ Layout make_layout(int64_t n, int64_t d, int prec) {
   L.total = o;
   return L;
 }
-
-template <typename T>
-T* at(void* ws, size_t off) { return reinterpret_cast<T*>(static_cast<char*>(ws) + off); }
-
-#define GPK_TRY(...)           \
-  do {                         \
-    int rc_ = (__VA_ARGS__);   \
-    if (rc_ != GPK_OK) return rc_; \
-  } while (0)
-
-// Recursive factorisation driver over one matrix buffer.
-template <typename TS>
-struct Factor {
-  TS* L;
-  TS* X;
-  int64_t ld;
-  TS* T;
-  double* leafinv;
-  double* leafld;
-  int* leafinfo;
-  int ar_potrf, ar_trtri;
-  hipStream_t s;
-
-  int leaf(int64_t o) {
-    return launch_leaf<TS>(L, X, ld, o, leafinv + (o / NB) * NB * NB, leafld + o / NB,
-                           leafinfo + o / NB, s);
-  }
-  // Solve Y L^T = B in place, B = L[row0 : row0+m, o : o+n], triangle L[o:o+n, o:o+n].
-  int trsm(int64_t row0, int64_t m, int64_t o, int64_t n) {
-    if (n == NB) {
-      const double* inv = leafinv + (o / NB) * NB * NB;
-      return launch_gemm<TS, double, TS>(AR_F64, false, false, L + row0 * ld + o, ld, inv, NB,
-                                         L + row0 * ld + o, ld, m, NB, NB, 1.0, false, false,
-                                         KR_FULL, s);
-    }
-    const int64_t h = split_point(n);
-    GPK_TRY(trsm(row0, m, o, h));
-    GPK_TRY(launch_gemm<TS, TS, TS>(ar_potrf, false, false, L + row0 * ld + o, ld,
-                                    L + (o + h) * ld + o, ld, L + row0 * ld + o + h, ld, m,
-                                    n - h, h, -1.0, true, false, KR_FULL, s));
-    return trsm(row0, m, o + h, n - h);
-  }
-  int potrf(int64_t o, int64_t n) {
-    if (n == NB) return leaf(o);
-    const int64_t h = split_point(n);
-    GPK_TRY(potrf(o, h));
-    GPK_TRY(trsm(o + h, n - h, o, h));
-    GPK_TRY(launch_gemm<TS, TS, TS>(ar_potrf, false, false, L + (o + h) * ld + o, ld,
-                                    L + (o + h) * ld + o, ld, L + (o + h) * ld + o + h, ld,
-                                    n - h, n - h, h, -1.0, true, true, KR_FULL, s));
-    return potrf(o + h, n - h);
-  }
-  // X = L^-1 below the leaves, whose diagonal blocks the leaves already wrote.
-  int trtri(int64_t o, int64_t n) {
-    if (n == NB) return GPK_OK;
-    const int64_t h = split_point(n);
-    GPK_TRY(trtri(o, h));
-    GPK_TRY(trtri(o + h, n - h));
-    // T = L21 X11   ((n-h) x h; X11 lower, so k >= column tile)
-    GPK_TRY(launch_gemm<TS, TS, TS>(ar_trtri, false, true, L + (o + h) * ld + o, ld,
-                                    X + o * ld + o, ld, T, h, n - h, h, h, 1.0, false, false,
-                                    KR_GE_N, s));
-    // X21 = -X22 T  (X22 lower, so k <= row tile)
-    return launch_gemm<TS, TS, TS>(ar_trtri, false, true, X + (o + h) * ld + o + h, ld, T, h,
-                                   X + (o + h) * ld + o, ld, n - h, h, n - h, -1.0, false,
-                                   false, KR_LE_M, s);
-  }
-};
 
 // ---- small kernels ----
 __global__ void prep_r_kernel(const double* __restrict__ y, const double* __restrict__ theta,

@@ -1,0 +1,95 @@
+// Recursive Cholesky / triangular-solve / triangular-inverse driver on the tile
+// GEMM, shared by the exact-GP and collapsed-ELBO paths.
+//
+//   potrf(A):   potrf(A11); L21 = A21 L11^-T (trsm); A22 -= L21 L21^T; potrf(A22)
+//   trsm(B, L): Y L^T = B by recursion on the columns, leaves multiplying by the
+//               float64 inverse of an NB x NB diagonal block
+//   trtri(L):   X = L^-1 with X21 = -X22 (L21 X11), both halves recursively
+// These replace jnp.linalg.cholesky, jsp.linalg.solve_triangular and the
+// LU-based slogdet/solve behind gpjax/linalg/operations.py:54-55,105-111,163-165.
+// Splits fall on multiples of NB; leaves (NB x NB) are factored and inverted
+// in float64 in LDS (leaf.hip).
+#pragma once
+
+#include "gpk_internal.h"
+
+namespace gpk {
+
+#define GPK_TRY(...)               \
+  do {                             \
+    int rc_ = (__VA_ARGS__);       \
+    if (rc_ != GPK_OK) return rc_; \
+  } while (0)
+
+inline int64_t split_point(int64_t n) { return (n / NB / 2) * NB; }
+
+template <typename T>
+inline T* at(void* ws, size_t off) {
+  return reinterpret_cast<T*>(static_cast<char*>(ws) + off);
+}
+
+template <typename TS>
+struct Factor {
+  TS* L;             // matrix in, factor out (lower NB-tiles), leading dimension ld
+  TS* X;             // L^-1 out (lower)
+  int64_t ld;
+  TS* T;             // scratch of (n - split(n)) * split(n) elements
+  double* leafinv;   // n/NB blocks of NB x NB float64 leaf inverses
+  double* leafld;    // n/NB leaf log-determinants
+  int* leafinfo;     // n/NB leaf pivot status
+  int ar_potrf, ar_trtri;
+  hipStream_t s;
+
+  int leaf(int64_t o) {
+    return launch_leaf<TS>(L, X, ld, o, leafinv + (o / NB) * NB * NB, leafld + o / NB,
+                           leafinfo + o / NB, s);
+  }
+  // Solve Y L^T = B in place, B = L[row0 : row0+m, o : o+n], triangle L[o:o+n, o:o+n].
+  int trsm(int64_t row0, int64_t m, int64_t o, int64_t n) {
+    if (n == NB) {
+      const double* inv = leafinv + (o / NB) * NB * NB;
+      return launch_gemm<TS, double, TS>(AR_F64, false, false, L + row0 * ld + o, ld, inv, NB,
+                                         L + row0 * ld + o, ld, m, NB, NB, 1.0, false, false,
+                                         KR_FULL, s);
+    }
+    const int64_t h = split_point(n);
+    GPK_TRY(trsm(row0, m, o, h));
+    GPK_TRY(launch_gemm<TS, TS, TS>(ar_potrf, false, false, L + row0 * ld + o, ld,
+                                    L + (o + h) * ld + o, ld, L + row0 * ld + o + h, ld, m,
+                                    n - h, h, -1.0, true, false, KR_FULL, s));
+    return trsm(row0, m, o + h, n - h);
+  }
+  int potrf(int64_t o, int64_t n) {
+    if (n == NB) return leaf(o);
+    const int64_t h = split_point(n);
+    GPK_TRY(potrf(o, h));
+    GPK_TRY(trsm(o + h, n - h, o, h));
+    GPK_TRY(launch_gemm<TS, TS, TS>(ar_potrf, false, false, L + (o + h) * ld + o, ld,
+                                    L + (o + h) * ld + o, ld, L + (o + h) * ld + o + h, ld,
+                                    n - h, n - h, h, -1.0, true, true, KR_FULL, s));
+    return potrf(o + h, n - h);
+  }
+  // X = L^-1 below the leaves, whose diagonal blocks the leaves already wrote.
+  int trtri(int64_t o, int64_t n) {
+    if (n == NB) return GPK_OK;
+    const int64_t h = split_point(n);
+    GPK_TRY(trtri(o, h));
+    GPK_TRY(trtri(o + h, n - h));
+    // T = L21 X11   ((n-h) x h; X11 lower, so k >= column tile)
+    GPK_TRY(launch_gemm<TS, TS, TS>(ar_trtri, false, true, L + (o + h) * ld + o, ld,
+                                    X + o * ld + o, ld, T, h, n - h, h, h, 1.0, false, false,
+                                    KR_GE_N, s));
+    // X21 = -X22 T  (X22 lower, so k <= row tile)
+    return launch_gemm<TS, TS, TS>(ar_trtri, false, true, X + (o + h) * ld + o + h, ld, T, h,
+                                   X + (o + h) * ld + o, ld, n - h, h, n - h, -1.0, false,
+                                   false, KR_LE_M, s);
+  }
+};
+
+// Elements of the trtri scratch T for an n x n factorisation.
+inline int64_t factor_scratch_elems(int64_t n) {
+  const int64_t h = n > NB ? split_point(n) : 1;
+  return (n - h) * h;
+}
+
+}  // namespace gpk

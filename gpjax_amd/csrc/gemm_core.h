@@ -88,6 +88,26 @@ __device__ __forceinline__ void load8(const T* __restrict__ p, double (&v)[8]) {
   for (int q = 0; q < 8; ++q) v[q] = (double)p[q];
 }
 
+// One 16-deep k-slab of FP64 matrix-core products from LDS operands stored
+// k-major: a[k * F64_LDP + m], b[k * F64_LDP + n] (F64_LDP defined below).
+__device__ __forceinline__ void mfma_slab_f64(const double* a, const double* b, int ldp,
+                                              int lane, int wm, int wn, f64x4 (&acc)[4][4]) {
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    const int kr = kk * 4 + (lane >> 4);
+    double av[4], bv[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) av[i] = a[kr * ldp + wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bv[j] = b[kr * ldp + wn * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // float64 main loop
 // ---------------------------------------------------------------------------
@@ -142,22 +162,8 @@ __device__ __forceinline__ void mainloop_f64(const TA* __restrict__ A, int64_t l
   for (int64_t sl = 0; sl < nslab; ++sl) {
     const int buf = (int)(sl & 1);
     if (sl + 1 < nslab) gload(kb + (sl + 1) * F64_BK);
-    const double* a = As + buf * F64_BK * F64_LDP;
-    const double* b = Bs + buf * F64_BK * F64_LDP;
-#pragma unroll
-    for (int kk = 0; kk < F64_BK / 4; ++kk) {
-      const int kr = kk * 4 + (lane >> 4);
-      double av[4], bv[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) av[i] = a[kr * F64_LDP + wm * 64 + i * 16 + (lane & 15)];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) bv[j] = b[kr * F64_LDP + wn * 64 + j * 16 + (lane & 15)];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
-    }
+    mfma_slab_f64(As + buf * F64_BK * F64_LDP, Bs + buf * F64_BK * F64_LDP, F64_LDP, lane, wm,
+                  wn, acc);
     if (sl + 1 < nslab) sstore(buf ^ 1);
     __syncthreads();
   }

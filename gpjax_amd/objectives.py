@@ -104,3 +104,73 @@ def conjugate_mll_with_info(posterior, data, precision: str | None = None,
 def conjugate_mll(posterior, data, precision: str | None = None):
     """Log marginal likelihood (``gpjax/objectives.py:36-107``), differentiable."""
     return conjugate_mll_with_info(posterior, data, precision)[0]
+
+
+class _CollapsedELBO(torch.autograd.Function):
+    """Forward: gpk_elbo_forward.  Backward: gpk_elbo_backward (Kzx tiles recomputed)."""
+
+    @staticmethod
+    def forward(ctx, ls, var, sd, c, Z, X, y, kind_id, jitter, nls, info_out):
+        n, d = X.shape
+        m = Z.shape[0]
+        dev = X.device
+        theta = torch.cat([ls.reshape(-1)[:nls], var.reshape(1), sd.reshape(1),
+                           c.reshape(1)]).to(dev, torch.float64).contiguous()
+        Zc = Z.detach().to(dev, torch.float64).contiguous()
+        nbytes = _lib.lib().gpk_elbo_workspace_bytes(n, m, d, 0)
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        res = torch.empty(_lib.RESULT_LEN, dtype=torch.float64, device=dev)
+        _lib.call("gpk_elbo_forward", kind_id, 0, _lib.ptr(X), _lib.ptr(y), n, _lib.ptr(Zc), m, d,
+                  _lib.ptr(theta), nls, float(jitter), _lib.ptr(ws), nbytes, _lib.ptr(res),
+                  _lib.stream_ptr(dev))
+        ctx.ws, ctx.nbytes, ctx.theta, ctx.Z = ws, nbytes, theta, Zc
+        ctx.X, ctx.y, ctx.kind_id, ctx.nls = X, y, kind_id, nls
+        ctx.shapes = (ls.shape, var.shape, sd.shape, c.shape, Z.shape)
+        if info_out is not None:
+            info_out["result"] = res
+        return res[_lib.RES_MLL].clone()
+
+    @staticmethod
+    def backward(ctx, gbar):
+        X, y, Zc = ctx.X, ctx.y, ctx.Z
+        n, d = X.shape
+        m = Zc.shape[0]
+        nls = ctx.nls
+        grad = torch.empty(nls + 3, dtype=torch.float64, device=X.device)
+        gz = torch.empty((m, d), dtype=torch.float64, device=X.device)
+        _lib.call("gpk_elbo_backward", ctx.kind_id, 0, _lib.ptr(X), _lib.ptr(y), n, _lib.ptr(Zc),
+                  m, d, _lib.ptr(ctx.theta), nls, 1.0, _lib.ptr(ctx.ws), ctx.nbytes,
+                  _lib.ptr(grad), _lib.ptr(gz), _lib.stream_ptr(X.device))
+        g = grad * gbar
+        ls_s, var_s, sd_s, c_s, z_s = ctx.shapes
+        ls_g = g[:nls].reshape(ls_s) if math.prod(ls_s) == nls else g[:nls].sum().expand(ls_s)
+        ctx.ws = None
+        return (ls_g, g[nls].reshape(var_s), g[nls + 1].reshape(sd_s), g[nls + 2].reshape(c_s),
+                (gz * gbar).reshape(z_s), None, None, None, None, None, None)
+
+
+def collapsed_elbo_with_info(variational_family, data):
+    """(elbo, result) -- result is the device buffer of ``gpk_elbo_forward``."""
+    post = variational_family.posterior
+    kernel = post.prior.kernel
+    X = kernel.slice_input(data.X)
+    if not X.is_cuda:
+        raise ValueError("collapsed_elbo: data must live on a GPU device")
+    y = data.y.reshape(-1).to(torch.float64).contiguous()
+    n, d = X.shape
+    Z = variational_family.inducing_inputs.value
+    if Z.ndim != 2 or Z.shape[1] != d:
+        raise ValueError("inducing inputs must be [M, D] with the data's D")
+    nls = kernel.nls(d)
+    c = _mean_constant(post.prior.mean_function, X.device)
+    info: dict = {}
+    elbo = _CollapsedELBO.apply(kernel.lengthscale.value, kernel.variance.value,
+                                post.likelihood.obs_stddev.value, c, Z, X, y, kernel.kind_id,
+                                float(variational_family.jitter), nls, info)
+    return elbo, info["result"]
+
+
+def collapsed_elbo(variational_family, data):
+    """Collapsed ELBO (``gpjax/objectives.py:321-416``), differentiable in the kernel
+    hyperparameters, obs_stddev, a trainable constant mean and the inducing inputs."""
+    return collapsed_elbo_with_info(variational_family, data)[0]

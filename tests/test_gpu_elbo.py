@@ -1,0 +1,76 @@
+"""GPU parity of the collapsed ELBO (gpk_elbo_forward / _backward) against the oracle.
+
+The oracle's streamed-statistics form (``collapsed_elbo_grad``) is itself
+pinned against the reference-form ``collapsed_elbo`` and against central
+differences and torch autograd in tests/test_oracle.py.  Tolerances (float64
+path): ELBO within 1e-9 relative, gradients within 1e-7 relative norm-wise
+(theta) and 1e-7 of the largest entry (Z).  The reference's own invariant,
+Z = X gives ELBO == MLL within 1e-6 (tests/test_objectives.py:193-199), is
+checked on the device too.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import gpjax_amd as gpx
+from oracle import gp_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+KCLS = {"rbf": gpx.kernels.RBF, "matern12": gpx.kernels.Matern12,
+        "matern32": gpx.kernels.Matern32, "matern52": gpx.kernels.Matern52}
+
+
+def _setup(cuda, kind, n, m, d, seed, ard=True, mean=0.15):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(0, 1, (n, d))
+    y = np.sin(2 * np.pi * X).sum(1) + 0.2 * rng.standard_normal(n)
+    Z = X[rng.choice(n, m, replace=False)] + 0.003
+    ls = rng.uniform(0.2, 0.6, d) if ard else 0.4
+    var, sd = 1.2, 0.3
+    k = KCLS[kind](lengthscale=ls, variance=var)
+    mf = gpx.mean_functions.Constant(gpx.parameters.Real(mean))
+    post = gpx.gps.Prior(mean_function=mf, kernel=k) * \
+        gpx.likelihoods.Gaussian(num_datapoints=n, obs_stddev=sd)
+    q = gpx.variational_families.CollapsedVariationalGaussian(posterior=post, inducing_inputs=Z)
+    D = gpx.Dataset(X=torch.tensor(X, device=cuda), y=torch.tensor(y[:, None], device=cuda))
+    return X, y, Z, ls, var, sd, mean, k, post, q, D
+
+
+@pytest.mark.parametrize("kind,n,m,d,ard", [("rbf", 500, 40, 2, True),
+                                            ("matern32", 3000, 300, 8, True),
+                                            ("matern52", 1234, 129, 3, False),
+                                            ("matern12", 700, 64, 1, True)])
+def test_elbo_and_grad_parity(cuda, kind, n, m, d, ard):
+    X, y, Z, ls, var, sd, mean, k, post, q, D = _setup(cuda, kind, n, m, d, seed=n + m, ard=ard)
+    elbo, res = gpx.objectives.collapsed_elbo_with_info(q, D)
+    elbo.backward()
+    ref = O.collapsed_elbo_grad(kind, X, y, Z, ls, var, sd, 1e-6, mean)
+    ref_direct = O.collapsed_elbo(kind, X, y, Z, ls, var, sd, 1e-6, mean)
+    assert int(res[4].item()) == 0
+    assert abs(elbo.item() - ref["elbo"]) <= 1e-9 * abs(ref["elbo"])
+    assert abs(elbo.item() - ref_direct) <= 1e-8 * abs(ref_direct)
+    g = np.concatenate([np.atleast_1d(k.lengthscale.value.grad.cpu().numpy()),
+                        [k.variance.value.grad.item(), post.likelihood.obs_stddev.value.grad.item(),
+                         post.prior.mean_function.constant.value.grad.item()]])
+    gr = np.concatenate([np.atleast_1d(ref["lengthscale"]),
+                         [ref["variance"], ref["obs_stddev"], ref["mean"]]])
+    assert np.linalg.norm(g - gr) <= 1e-7 * np.linalg.norm(gr), (g, gr)
+    gz = q.inducing_inputs.value.grad.cpu().numpy()
+    assert np.max(np.abs(gz - ref["Z"])) <= 1e-7 * np.max(np.abs(ref["Z"]))
+
+
+def test_elbo_equals_mll_when_z_is_x(cuda):
+    rng = np.random.default_rng(3)
+    n, d = 200, 2
+    X = rng.uniform(-2, 2, (n, d))
+    y = np.sin(X[:, 0]) + 0.1 * rng.standard_normal(n)
+    k = gpx.kernels.RBF(lengthscale=1.0, variance=1.0)
+    post = gpx.gps.Prior(mean_function=gpx.mean_functions.Zero(), kernel=k) * \
+        gpx.likelihoods.Gaussian(num_datapoints=n, obs_stddev=1.0)
+    D = gpx.Dataset(X=torch.tensor(X, device=cuda), y=torch.tensor(y[:, None], device=cuda))
+    q = gpx.variational_families.CollapsedVariationalGaussian(posterior=post, inducing_inputs=X)
+    elbo = gpx.objectives.collapsed_elbo(q, D).item()
+    mll = gpx.objectives.conjugate_mll(post, D).item()
+    assert abs(elbo - mll) / abs(mll) < 1e-6
