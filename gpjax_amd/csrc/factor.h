@@ -39,10 +39,11 @@ struct Factor {
   int* leafinfo;     // n/NB leaf pivot status
   int ar_potrf, ar_trtri;
   hipStream_t s;
+  const double* diagres = nullptr;  // float64 residual of a float32-stored diagonal
 
   int leaf(int64_t o) {
     return launch_leaf<TS>(L, X, ld, o, leafinv + (o / NB) * NB * NB, leafld + o / NB,
-                           leafinfo + o / NB, s);
+                           leafinfo + o / NB, diagres, s);
   }
   // Solve Y L^T = B in place, B = L[row0 : row0+m, o : o+n], triangle L[o:o+n, o:o+n].
   int trsm(int64_t row0, int64_t m, int64_t o, int64_t n) {

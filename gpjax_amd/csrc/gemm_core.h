@@ -44,23 +44,29 @@ struct AccT<AR_F64> {
   typedef double scalar;
 };
 
-// ---- bf16 split helpers (round to nearest even, quiet NaN preserved) ----
-__device__ __forceinline__ unsigned short f2bf(float f) {
-  unsigned int u = __float_as_uint(f);
-  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (unsigned short)((u >> 16) | 0x40u);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(unsigned short h) {
-  return __uint_as_float(((unsigned int)h) << 16);
-}
+// ---- bf16 split: a = p0 + p1 (+ p2) + remainder, each plane rounded to
+// nearest even by the hardware conversion (v_cvt_pk_bf16_f32, two values per
+// instruction); the residual a - p is exact in fp32.  Sixteen values become
+// NP planes of sixteen bf16, packed as two uint4 per plane.
 template <int NP>
-__device__ __forceinline__ void split_bf16(float a, unsigned short (&p)[NP]) {
-  float r = a;
+__device__ __forceinline__ void split_pack16(const float (&v)[16], uint4 (&out)[NP][2]) {
 #pragma unroll
-  for (int q = 0; q < NP; ++q) {
-    p[q] = f2bf(r);
-    r = r - bf2f(p[q]);
+  for (int h = 0; h < 2; ++h) {
+    unsigned wq[NP][4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      float r0 = v[h * 8 + 2 * e], r1 = v[h * 8 + 2 * e + 1];
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        const __bf16 b0 = (__bf16)r0, b1 = (__bf16)r1;
+        wq[q][e] = (unsigned)__builtin_bit_cast(unsigned short, b0) |
+                   ((unsigned)__builtin_bit_cast(unsigned short, b1) << 16);
+        r0 -= (float)b0;
+        r1 -= (float)b1;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NP; ++q) out[q][h] = make_uint4(wq[q][0], wq[q][1], wq[q][2], wq[q][3]);
   }
 }
 
@@ -188,45 +194,42 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
     for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   if (kb >= ke) return;
 
-  // 16 fp32 per thread per operand.
-  // Row-contiguous: row = t/2, k offset (t%2)*16.  Column-contiguous: k = t/8,
-  // row offset (t%8)*16.
+  // Every thread stages 16 consecutive k of one row of each operand per slab.
+  // Row-contiguous operand: row = t/2, k = (t%2)*16 .. +15 (four 16-byte loads).
+  // Column-contiguous operand: row = t%128, k = (t/128)*16 .. +15 (sixteen
+  // 4-byte loads; consecutive lanes read consecutive rows, 256 bytes per load).
   float ra[16], rb[16];
-  auto gload = [&](int64_t k0) {
-    const float* pa = !A_KM ? A + (m0 + (t >> 1)) * lda + k0 + (t & 1) * 16
-                            : A + (k0 + (t >> 3)) * lda + m0 + (t & 7) * 16;
-    const float* pb = !B_KN ? B + (n0 + (t >> 1)) * ldb + k0 + (t & 1) * 16
-                            : B + (k0 + (t >> 3)) * ldb + n0 + (t & 7) * 16;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float4 va = *reinterpret_cast<const float4*>(pa + 4 * q);
-      const float4 vb = *reinterpret_cast<const float4*>(pb + 4 * q);
-      ra[4 * q] = va.x; ra[4 * q + 1] = va.y; ra[4 * q + 2] = va.z; ra[4 * q + 3] = va.w;
-      rb[4 * q] = vb.x; rb[4 * q + 1] = vb.y; rb[4 * q + 2] = vb.z; rb[4 * q + 3] = vb.w;
-    }
-  };
-  auto sstore_one = [&](unsigned short* base, const float (&r)[16], bool km) {
+  auto gload_one = [&](const float* __restrict__ P, int64_t ld, int64_t r0, bool km,
+                       int64_t k0, float (&r)[16]) {
     if (!km) {
-      // 16 consecutive k of one row: contiguous in every plane.
-      const int row = t >> 1, ko = (t & 1) * 16;
-      unsigned short p[16][NP];
+      const float* p = P + (r0 + (t >> 1)) * ld + k0 + (t & 1) * 16;
 #pragma unroll
-      for (int q = 0; q < 16; ++q) split_bf16<NP>(r[q], p[q]);
-#pragma unroll
-      for (int pl = 0; pl < NP; ++pl) {
-        unsigned short* d = base + (pl * TILE + row) * BF_LDK + ko;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) d[q] = p[q][pl];
+      for (int q = 0; q < 4; ++q) {
+        const float4 v = *reinterpret_cast<const float4*>(p + 4 * q);
+        r[4 * q] = v.x; r[4 * q + 1] = v.y; r[4 * q + 2] = v.z; r[4 * q + 3] = v.w;
       }
     } else {
-      const int k = t >> 3, ro = (t & 7) * 16;
+      const float* p = P + (k0 + (t >> 7) * 16) * ld + r0 + (t & 127);
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        unsigned short p[NP];
-        split_bf16<NP>(r[q], p);
+      for (int q = 0; q < 16; ++q) r[q] = p[q * ld];
+    }
+  };
+  auto gload = [&](int64_t k0) {
+    gload_one(A, lda, m0, A_KM, k0, ra);
+    gload_one(B, ldb, n0, B_KN, k0, rb);
+  };
+  // Split into bf16 planes and store as two 16-byte writes per plane into the
+  // k-contiguous layout the matrix-core operands read.
+  auto sstore_one = [&](unsigned short* base, const float (&r)[16], bool km) {
+    const int row = km ? (t & 127) : (t >> 1);
+    const int ko = km ? (t >> 7) * 16 : (t & 1) * 16;
+    uint4 pk[NP][2];
+    split_pack16<NP>(r, pk);
 #pragma unroll
-        for (int pl = 0; pl < NP; ++pl) base[(pl * TILE + ro + q) * BF_LDK + k] = p[pl];
-      }
+    for (int pl = 0; pl < NP; ++pl) {
+      uint4* d = reinterpret_cast<uint4*>(base + (pl * TILE + row) * BF_LDK + ko);
+      d[0] = pk[pl][0];
+      d[1] = pk[pl][1];
     }
   };
 

@@ -32,7 +32,8 @@ __global__ __launch_bounds__(256) void kmat_kernel(int kind, const double* __res
                                                    const double* __restrict__ theta, int nls,
                                                    double jitter, int diag_mode, TS* out,
                                                    int64_t ldo, int64_t rows_total,
-                                                   int64_t cols_total, int lower_only) {
+                                                   int64_t cols_total, int lower_only,
+                                                   double* __restrict__ diagres) {
   __shared__ double xi_s[KT][DCH + 1];
   __shared__ double zj_s[DCH][KT + 1];
 
@@ -104,7 +105,11 @@ __global__ __launch_bounds__(256) void kmat_kernel(int kind, const double* __res
     } else {
       v = (i == j && !(diag_mode & KM_ZERO_PAD)) ? 1.0 : 0.0;
     }
-    out[i * ldo + j] = (TS)v;
+    const TS sv = (TS)v;
+    out[i * ldo + j] = sv;
+    // The diagonal of Sigma is one value repeated, so its float32 rounding is a
+    // systematic shift; keep the float64 residual for the leaf factorisation.
+    if (diagres && i == j) diagres[i] = v - (double)sv;
   }
 }
 
@@ -114,7 +119,7 @@ template <typename TS>
 int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, int64_t m,
                          int d, const double* theta, int nls, double jitter, int diag_mode,
                          TS* out, int64_t ldo, int64_t rows_total, int64_t cols_total,
-                         bool lower_only, hipStream_t s) {
+                         bool lower_only, hipStream_t s, double* diagres) {
   if (rows_total <= 0 || cols_total <= 0) return GPK_OK;
   dim3 grid;
   if (lower_only) {
@@ -133,16 +138,17 @@ int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, 
     grid = dim3((unsigned)((cols_total + KT - 1) / KT), (unsigned)gy, 1);
   }
   hipLaunchKernelGGL(kmat_kernel<TS>, grid, dim3(256), 0, s, kind, x, n, z, m, d, theta, nls,
-                     jitter, diag_mode, out, ldo, rows_total, cols_total, lower_only ? 1 : 0);
+                     jitter, diag_mode, out, ldo, rows_total, cols_total, lower_only ? 1 : 0,
+                     diagres);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }
 
 template int launch_kernel_matrix<double>(int, const double*, int64_t, const double*, int64_t,
                                           int, const double*, int, double, int, double*,
-                                          int64_t, int64_t, int64_t, bool, hipStream_t);
+                                          int64_t, int64_t, int64_t, bool, hipStream_t, double*);
 template int launch_kernel_matrix<float>(int, const double*, int64_t, const double*, int64_t,
                                          int, const double*, int, double, int, float*, int64_t,
-                                         int64_t, int64_t, bool, hipStream_t);
+                                         int64_t, int64_t, bool, hipStream_t, double*);
 
 }  // namespace gpk

@@ -29,7 +29,8 @@ template <typename TS>
 __global__ __launch_bounds__(256) void leaf_kernel(TS* A, TS* X, int64_t ld, int64_t o,
                                                    double* __restrict__ leafinv,
                                                    double* __restrict__ logdet,
-                                                   int* __restrict__ info) {
+                                                   int* __restrict__ info,
+                                                   const double* __restrict__ diagres) {
   __shared__ double col_s[2][NB];
   __shared__ double dinv_s[NB];
   __shared__ double tr_s[NB][NB + 1];
@@ -45,6 +46,7 @@ __global__ __launch_bounds__(256) void leaf_kernel(TS* A, TS* X, int64_t ld, int
     for (int b = 0; b < 8; ++b) {
       const int i = rb + a, j = cb + b;
       s[a][b] = (j <= i) ? (double)row[b] : 0.0;
+      if (diagres && i == j) s[a][b] += diagres[o + i];
     }
   }
   if (cb == 0) {
@@ -139,16 +141,16 @@ __global__ __launch_bounds__(256) void leaf_kernel(TS* A, TS* X, int64_t ld, int
 
 template <typename TS>
 int launch_leaf(TS* A, TS* X, int64_t ld, int64_t o, double* leafinv, double* logdet, int* info,
-                hipStream_t s) {
+                const double* diagres, hipStream_t s) {
   hipLaunchKernelGGL(leaf_kernel<TS>, dim3(1), dim3(256), 0, s, A, X, ld, o, leafinv, logdet,
-                     info);
+                     info, diagres);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }
 
 template int launch_leaf<double>(double*, double*, int64_t, int64_t, double*, double*, int*,
-                                 hipStream_t);
+                                 const double*, hipStream_t);
 template int launch_leaf<float>(float*, float*, int64_t, int64_t, double*, double*, int*,
-                                hipStream_t);
+                                const double*, hipStream_t);
 
 }  // namespace gpk
