@@ -33,13 +33,12 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int TILE = 128;
 
+// Output accumulators of every main loop are float64: the bf16 products
+// accumulate in fp32 on the matrix cores over k-chunks of BF_FLUSH_SLABS
+// slabs and each chunk is added into float64 (a single fp32 chain over
+// k = 65,536 loses ~1e-3 relative on sums of positive terms).
 template <int AR>
 struct AccT {
-  typedef f32x4 vec;
-  typedef float scalar;
-};
-template <>
-struct AccT<AR_F64> {
   typedef f64x4 vec;
   typedef double scalar;
 };
@@ -76,6 +75,7 @@ constexpr int F64_LDP = TILE + 1;  // padded row length (doubles)
 constexpr size_t F64_SMEM = 2 /*A,B*/ * 2 /*buffers*/ * F64_BK * F64_LDP * sizeof(double);
 constexpr int BF_BK = 32;
 constexpr int BF_LDK = BF_BK + 8;  // padded k length (bf16)
+constexpr int BF_FLUSH_SLABS = 8;  // fp32 chunk length: 8 slabs = 256 k
 template <int AR>
 constexpr int bf_planes() { return AR == AR_BF16X6 ? 3 : 2; }
 template <int AR>
@@ -182,17 +182,31 @@ template <int AR, bool A_KM, bool B_KN>
 __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64_t lda,
                                               const float* __restrict__ B, int64_t ldb,
                                               int64_t m0, int64_t n0, int64_t kb, int64_t ke,
-                                              f32x4 (&acc)[4][4], char* smem) {
+                                              f64x4 (&acc64)[4][4], char* smem) {
   constexpr int NP = bf_planes<AR>();
   unsigned short* As = reinterpret_cast<unsigned short*>(smem);   // [NP][TILE][LDK]
   unsigned short* Bs = As + NP * TILE * BF_LDK;                    // [NP][TILE][LDK]
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w >> 1, wn = w & 1;
+  f32x4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < 4; ++j) {
+      acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+      acc64[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+    }
   if (kb >= ke) return;
+  auto flush = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)acc[i][j][r];
+        acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+  };
 
   // Every thread stages 16 consecutive k of one row of each operand per slab.
   // Row-contiguous operand: row = t/2, k = (t%2)*16 .. +15 (four 16-byte loads).
@@ -268,6 +282,7 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
         acc[i][j] = c;
       }
     __syncthreads();
+    if ((sl + 1) % BF_FLUSH_SLABS == 0 || sl + 1 == nslab) flush();
   }
 }
 

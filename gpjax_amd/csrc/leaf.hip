@@ -11,7 +11,7 @@
 // of that array is exactly [X-row-k numerators ; pivot ; L-column-k
 // numerators], so one broadcast of column k per step drives both updates.
 //
-// Thread t of 256 owns the 8 x 8 block rows 8*(t/16).., cols 8*(t%16)..;
+// Thread t of 1024 owns the 4 x 4 block rows 4*(t/32).., cols 4*(t%32)..;
 // column k+1 is published to a double-buffered LDS vector by its owners
 // right after their update, so each step costs one barrier.
 //
@@ -25,25 +25,31 @@
 namespace gpk {
 namespace {
 
+// 1024 threads, thread t owning the EB x EB block at rows EB*(t/32), cols
+// EB*(t%32): four waves per SIMD hide the per-column LDS and barrier latency.
+constexpr int LEAF_THREADS = 1024;
+constexpr int EB = 4;
+constexpr int BPR = NB / EB;
+
 template <typename TS>
-__global__ __launch_bounds__(256) void leaf_kernel(TS* A, TS* X, int64_t ld, int64_t o,
-                                                   double* __restrict__ leafinv,
-                                                   double* __restrict__ logdet,
-                                                   int* __restrict__ info,
-                                                   const double* __restrict__ diagres) {
+__global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_t ld, int64_t o,
+                                                            double* __restrict__ leafinv,
+                                                            double* __restrict__ logdet,
+                                                            int* __restrict__ info,
+                                                            const double* __restrict__ diagres) {
   __shared__ double col_s[2][NB];
   __shared__ double dinv_s[NB];
   __shared__ double tr_s[NB][NB + 1];
 
   const int t = threadIdx.x;
-  const int rb = (t >> 4) * 8;
-  const int cb = (t & 15) * 8;
-  double s[8][8];
+  const int rb = (t / BPR) * EB;
+  const int cb = (t % BPR) * EB;
+  double s[EB][EB];
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
+  for (int a = 0; a < EB; ++a) {
     const TS* row = A + (o + rb + a) * ld + o + cb;
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < EB; ++b) {
       const int i = rb + a, j = cb + b;
       s[a][b] = (j <= i) ? (double)row[b] : 0.0;
       if (diagres && i == j) s[a][b] += diagres[o + i];
@@ -51,7 +57,7 @@ __global__ __launch_bounds__(256) void leaf_kernel(TS* A, TS* X, int64_t ld, int
   }
   if (cb == 0) {
 #pragma unroll
-    for (int a = 0; a < 8; ++a) col_s[0][rb + a] = s[a][0];
+    for (int a = 0; a < EB; ++a) col_s[0][rb + a] = s[a][0];
   }
   __syncthreads();
 
@@ -67,46 +73,38 @@ __global__ __launch_bounds__(256) void leaf_kernel(TS* A, TS* X, int64_t ld, int
       ld_acc += log(dk);
       dinv_s[k] = inv;
     }
-    // Skip blocks with no element to touch: every update at step k has column > k
-    // or is column k itself.
-    if (cb + 7 >= k) {
-      double sc_c[8];
+    // Every update at step k touches column k or a column > k.
+    if (cb + EB - 1 >= k) {
+      double sc_c[EB];
 #pragma unroll
-      for (int b = 0; b < 8; ++b) {
+      for (int b = 0; b < EB; ++b) {
         const int j = cb + b;
         sc_c[b] = (j == k) ? inv : col[j] * inv;
       }
 #pragma unroll
-      for (int a = 0; a < 8; ++a) {
+      for (int a = 0; a < EB; ++a) {
         const int i = rb + a;
         const double sci = (i == k) ? inv : col[i] * inv;
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < EB; ++b) {
           const int j = cb + b;
-          if (j <= i) {
-            // lower: trailing matrix, becomes L
-            if (j == k) {
-              s[a][b] = (i == k) ? dk : sci;
-            } else if (j > k) {
-              s[a][b] -= sci * sc_c[b];
-            }
-          } else {
-            // strict upper: (i, j) holds the numerator of X[j][i]
-            if (j == k) {
-              s[a][b] = sci;
-            } else if (j > k && i <= k) {
-              s[a][b] -= sc_c[b] * sci;
-            }
+          if (j == k) {
+            // column k: L[i][k] below the diagonal, X[k][i] (stored at (i, k)) above
+            s[a][b] = (i == k) ? dk : sci;
+          } else if (j > k && (i >= j || i <= k)) {
+            // lower part: trailing update L[i][k] L[j][k];
+            // strict upper (i, j), i <= k: numerator of X[j][i] -= L[j][k] X[k][i]
+            s[a][b] -= sci * sc_c[b];
           }
         }
       }
     }
-    if (k + 1 < NB && cb <= k + 1 && k + 1 < cb + 8) {
+    if (k + 1 < NB && cb <= k + 1 && k + 1 < cb + EB) {
 #pragma unroll
-      for (int b = 0; b < 8; ++b) {
+      for (int b = 0; b < EB; ++b) {
         if (cb + b == k + 1) {
 #pragma unroll
-          for (int a = 0; a < 8; ++a) col_s[(k + 1) & 1][rb + a] = s[a][b];
+          for (int a = 0; a < EB; ++a) col_s[(k + 1) & 1][rb + a] = s[a][b];
         }
       }
     }
@@ -115,17 +113,17 @@ __global__ __launch_bounds__(256) void leaf_kernel(TS* A, TS* X, int64_t ld, int
 
   // L out (lower incl. diagonal), and X^T staged in LDS for a coalesced write.
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
+  for (int a = 0; a < EB; ++a) {
     TS* row = A + (o + rb + a) * ld + o + cb;
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < EB; ++b) {
       const int i = rb + a, j = cb + b;
       row[b] = (TS)((j <= i) ? s[a][b] : 0.0);
       if (j > i) tr_s[j][i] = s[a][b];
     }
   }
   __syncthreads();
-  for (int e = t; e < NB * NB; e += 256) {
+  for (int e = t; e < NB * NB; e += LEAF_THREADS) {
     const int i = e / NB, j = e % NB;
     const double v = (j < i) ? tr_s[i][j] : (j == i ? dinv_s[i] : 0.0);
     X[(o + i) * ld + o + j] = (TS)v;
@@ -142,8 +140,8 @@ __global__ __launch_bounds__(256) void leaf_kernel(TS* A, TS* X, int64_t ld, int
 template <typename TS>
 int launch_leaf(TS* A, TS* X, int64_t ld, int64_t o, double* leafinv, double* logdet, int* info,
                 const double* diagres, hipStream_t s) {
-  hipLaunchKernelGGL(leaf_kernel<TS>, dim3(1), dim3(256), 0, s, A, X, ld, o, leafinv, logdet,
-                     info, diagres);
+  hipLaunchKernelGGL(leaf_kernel<TS>, dim3(1), dim3(LEAF_THREADS), 0, s, A, X, ld, o, leafinv,
+                     logdet, info, diagres);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }
