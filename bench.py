@@ -91,6 +91,9 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed-arith", default="6,3,6", metavar="P,T,L",
+                    help="bf16 products per element for the Cholesky updates, the "
+                         "triangular inverse and the fused Sigma^-1 pass (3 or 6 each)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES bytes 5 times "
                          "(reference for rocprof FETCH_SIZE / WRITE_SIZE scaling) and exit")
@@ -116,6 +119,8 @@ def main() -> int:
     from gpjax_amd import _lib, workloads
 
     dev = torch.device("cuda:0")
+    arith = [int(v) for v in args.mixed_arith.split(",")]
+    _lib.call("gpk_set_mixed_arith", *arith)
     t0 = time.time()
     w = workloads.exact_large(n=args.n)
     print(f"# workload generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
@@ -187,7 +192,8 @@ def main() -> int:
         "dtype": "bf16x6-split/f32 with f64 panels", "data": "synthetic",
         "config": {"workload": "exact GP MLL+grad, N=65536 D=16 RBF-ARD, RFF targets "
                                "(gpjax_amd.workloads.exact_large)",
-                   "n": args.n, "d": 16, "params": 18, "precision": "mixed"},
+                   "n": args.n, "d": 16, "params": 18, "precision": "mixed",
+                   "bf16_products_potrf_trtri_lauum": arith},
         "phases_ms": {k: round(v[0] / max(v[1], 1), 2) for k, v in phases_m.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -79,7 +79,7 @@ struct Layout {
   size_t ts = 0;
   size_t off_L = 0, off_X = 0, off_T = 0, off_leafinv = 0, off_leafld = 0, off_leafinfo = 0;
   size_t off_r = 0, off_alpha = 0, off_u = 0, off_v = 0, off_res = 0, off_delta = 0;
-  size_t off_diagres = 0;
+  size_t off_dblk = 0;
   size_t off_part = 0, off_gpart = 0, off_scal = 0, total = 0;
 };
 
@@ -105,7 +105,7 @@ Layout make_layout(int64_t n, int64_t d, int prec) {
   L.off_v = o; o += al((size_t)np * sizeof(double));
   L.off_res = o; o += al((size_t)np * sizeof(double));
   L.off_delta = o; o += al((size_t)np * sizeof(double));
-  L.off_diagres = o; o += al((size_t)np * sizeof(double));
+  L.off_dblk = o; o += al(prec == GPK_PREC_FP64 ? 0 : (size_t)np * NB * sizeof(double));
   L.off_part = o; o += al((size_t)nchunk * np * sizeof(double));
   L.off_gpart = o; o += al((size_t)ntile * (d + 2) * sizeof(double));
   L.off_scal = o; o += al(16 * sizeof(double));
@@ -255,16 +255,16 @@ int mll_forward_t(int kind, int prec, const double* x, const double* y, int64_t 
                f64 ? (int)AR_F64 : g_ar_potrf,
                f64 ? (int)AR_F64 : g_ar_trtri,
                s};
-  // float32 storage: the leaves restore the float64 diagonal of Sigma
-  double* diagres = f64 ? nullptr : at<double>(ws, Lo.off_diagres);
-  F.diagres = diagres;
+  // float32 storage: the diagonal NB-blocks live in float64 on the side
+  double* dblk = f64 ? nullptr : at<double>(ws, Lo.off_dblk);
+  F.dblk = dblk;
 
   hipLaunchKernelGGL(prep_r_kernel, dim3(nblk(np)), dim3(256), 0, s, y, theta, nls, n, np, r);
   GPK_LAUNCH_CHECK();
   {
     PhaseScope ps(GPK_PHASE_GRAM, s);
     GPK_TRY(launch_kernel_matrix<TS>(kind, x, n, x, n, d, theta, nls, jitter, 2, L, np, np, np,
-                                     true, s, diagres));
+                                     true, s, dblk));
   }
   {
     PhaseScope ps(GPK_PHASE_POTRF, s);

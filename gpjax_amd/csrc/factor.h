@@ -39,11 +39,15 @@ struct Factor {
   int* leafinfo;     // n/NB leaf pivot status
   int ar_potrf, ar_trtri;
   hipStream_t s;
-  const double* diagres = nullptr;  // float64 residual of a float32-stored diagonal
+  // Optional float64 copy of the diagonal NB-blocks ([n/NB][NB][NB]): when set
+  // (float32 storage), SYRK updates those blocks in float64 and the leaves
+  // factor them from there, so no pivot block is ever rounded to float32.
+  double* dblk = nullptr;
 
+  double* dblk_at(int64_t o) const { return dblk ? dblk + (o / NB) * NB * NB : nullptr; }
   int leaf(int64_t o) {
     return launch_leaf<TS>(L, X, ld, o, leafinv + (o / NB) * NB * NB, leafld + o / NB,
-                           leafinfo + o / NB, diagres, s);
+                           leafinfo + o / NB, dblk_at(o), s);
   }
   // Solve Y L^T = B in place, B = L[row0 : row0+m, o : o+n], triangle L[o:o+n, o:o+n].
   int trsm(int64_t row0, int64_t m, int64_t o, int64_t n) {
@@ -67,7 +71,8 @@ struct Factor {
     GPK_TRY(trsm(o + h, n - h, o, h));
     GPK_TRY(launch_gemm<TS, TS, TS>(ar_potrf, false, false, L + (o + h) * ld + o, ld,
                                     L + (o + h) * ld + o, ld, L + (o + h) * ld + o + h, ld,
-                                    n - h, n - h, h, -1.0, true, true, KR_FULL, s));
+                                    n - h, n - h, h, -1.0, true, true, KR_FULL, s,
+                                    dblk_at(o + h)));
     return potrf(o + h, n - h);
   }
   // X = L^-1 below the leaves, whose diagonal blocks the leaves already wrote.

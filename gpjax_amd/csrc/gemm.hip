@@ -20,6 +20,7 @@ struct GemmParams {
   int beta_one;
   int lower_tiles;
   int krange;
+  double* cdiag;  // lower_tiles: diagonal tiles update this float64 [tile][NB][NB] instead of C
 };
 
 template <int AR, typename TA, typename TB, typename TC, bool A_KM, bool B_KN>
@@ -44,6 +45,23 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
   TC* C = static_cast<TC*>(p.C);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wm = w >> 1, wn = w & 1;
+  if (p.cdiag && p.lower_tiles && mt == nt) {
+    double* cd = p.cdiag + mt * TILE * TILE;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = wm * 64 + i * 16 + acc_row<AR>(lane, r);
+          const int colx = wn * 64 + j * 16 + (lane & 15);
+          double* c = cd + row * TILE + colx;
+          double v = p.alpha * (double)acc[i][j][r];
+          if (p.beta_one) v += *c;
+          *c = v;
+        }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -87,7 +105,7 @@ int dispatch_layout(bool a_km, bool b_kn, const GemmParams& p, int64_t M, int64_
 template <typename TA, typename TB, typename TC>
 int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const TB* B,
                 int64_t ldb, TC* C, int64_t ldc, int64_t M, int64_t N, int64_t K, double alpha,
-                bool beta_one, bool lower_tiles, int krange, hipStream_t s) {
+                bool beta_one, bool lower_tiles, int krange, hipStream_t s, double* cdiag) {
   if (M <= 0 || N <= 0) return GPK_OK;
   if (M % TILE || N % TILE || K % TILE || K < 0) {
     set_error("gemm: extents %lld x %lld x %lld not multiples of %d", (long long)M,
@@ -102,32 +120,28 @@ int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const
     set_error("gemm: M too large for one launch");
     return GPK_ERR_BAD_ARG;
   }
-  GemmParams p{A, lda, B, ldb, C, ldc, K, alpha, beta_one ? 1 : 0, lower_tiles ? 1 : 0, krange};
+  GemmParams p{A, lda, B, ldb, C, ldc, K, alpha, beta_one ? 1 : 0, lower_tiles ? 1 : 0, krange,
+               cdiag};
   if (arith == AR_F64) return dispatch_layout<AR_F64, TA, TB, TC>(a_km, b_kn, p, M, N, s);
   if constexpr (std::is_same<TA, float>::value && std::is_same<TB, float>::value &&
                 std::is_same<TC, float>::value) {
-    if (arith == AR_BF16X3) return dispatch_layout<AR_BF16X3, float, float, float>(a_km, b_kn, p, M, N, s);
-    if (arith == AR_BF16X6) return dispatch_layout<AR_BF16X6, float, float, float>(a_km, b_kn, p, M, N, s);
+    if (arith == AR_BF16X3)
+      return dispatch_layout<AR_BF16X3, float, float, float>(a_km, b_kn, p, M, N, s);
+    if (arith == AR_BF16X6)
+      return dispatch_layout<AR_BF16X6, float, float, float>(a_km, b_kn, p, M, N, s);
   }
   set_error("gemm: unsupported arithmetic %d for these operand types", arith);
   return GPK_ERR_BAD_ARG;
 }
 
-template int launch_gemm<double, double, double>(int, bool, bool, const double*, int64_t,
-                                                 const double*, int64_t, double*, int64_t,
-                                                 int64_t, int64_t, int64_t, double, bool, bool,
-                                                 int, hipStream_t);
-template int launch_gemm<float, float, float>(int, bool, bool, const float*, int64_t,
-                                              const float*, int64_t, float*, int64_t, int64_t,
-                                              int64_t, int64_t, double, bool, bool, int,
-                                              hipStream_t);
-template int launch_gemm<float, double, float>(int, bool, bool, const float*, int64_t,
-                                               const double*, int64_t, float*, int64_t,
-                                               int64_t, int64_t, int64_t, double, bool, bool,
-                                               int, hipStream_t);
-template int launch_gemm<float, double, double>(int, bool, bool, const float*, int64_t,
-                                                const double*, int64_t, double*, int64_t,
-                                                int64_t, int64_t, int64_t, double, bool, bool,
-                                                int, hipStream_t);
+#define GPK_GEMM_INST(TA, TB, TC)                                                            \
+  template int launch_gemm<TA, TB, TC>(int, bool, bool, const TA*, int64_t, const TB*,       \
+                                       int64_t, TC*, int64_t, int64_t, int64_t, int64_t, double, \
+                                       bool, bool, int, hipStream_t, double*);
+GPK_GEMM_INST(double, double, double)
+GPK_GEMM_INST(float, float, float)
+GPK_GEMM_INST(float, double, float)
+GPK_GEMM_INST(float, double, double)
+#undef GPK_GEMM_INST
 
 }  // namespace gpk

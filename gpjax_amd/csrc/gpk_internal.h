@@ -102,22 +102,21 @@ __device__ __forceinline__ void kval_fac(int kind, double d2, double& k, double&
 constexpr int KM_DIAG_MASK = 3;
 constexpr int KM_ZERO_PAD = 4;
 // lower_only: only 128-tiles on or below the diagonal (rows_total == cols_total).
-// diagres (optional): diagres[i] = (float64 diagonal value) - (stored value),
-// the rounding residual of a float32 store.
+// dblk (optional, lower_only mode): every diagonal NB x NB block is also
+// written in float64 to dblk[b][NB][NB].
 template <typename TS>
 int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, int64_t m,
                          int d, const double* theta, int nls, double jitter, int diag_mode,
                          TS* out, int64_t ldo, int64_t rows_total, int64_t cols_total,
-                         bool lower_only, hipStream_t s, double* diagres = nullptr);
+                         bool lower_only, hipStream_t s, double* dblk = nullptr);
 
 // Leaf: factor the NB x NB diagonal block at (o, o) of A in float64 in LDS,
 // write L (lower, zero upper) to A, L^-1 to X (lower, zero upper) and to
 // leafinv (float64), 2*sum(log diag) to *logdet, first bad pivot to *info.
-// diagres (optional, indexed by global row) is added to the block's diagonal
-// on load, restoring the float64 diagonal of a float32-stored matrix.
+// dsrc (optional): read the block from this float64 NB x NB array instead of A.
 template <typename TS>
 int launch_leaf(TS* A, TS* X, int64_t ld, int64_t o, double* leafinv, double* logdet, int* info,
-                const double* diagres, hipStream_t s);
+                const double* dsrc, hipStream_t s);
 
 // k-range modes of the tile GEMM.
 enum KRange { KR_FULL = 0, KR_GE_N = 1, KR_LE_M = 2, KR_GE_M = 3 };
@@ -131,7 +130,8 @@ enum Arith { AR_F64 = 0, AR_BF16X3 = 3, AR_BF16X6 = 6 };
 template <typename TA, typename TB, typename TC>
 int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const TB* B,
                 int64_t ldb, TC* C, int64_t ldc, int64_t M, int64_t N, int64_t K, double alpha,
-                bool beta_one, bool lower_tiles, int krange, hipStream_t s);
+                bool beta_one, bool lower_tiles, int krange, hipStream_t s,
+                double* cdiag = nullptr);
 
 // Fused LAUUM + adjoint: for every lower NB-tile of Sinv = X^T X (X = L^-1,
 // lower, Np x Np), reduce 1/2 tr(W dSigma/dtheta) contributions with

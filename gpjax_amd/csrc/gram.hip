@@ -33,7 +33,7 @@ __global__ __launch_bounds__(256) void kmat_kernel(int kind, const double* __res
                                                    double jitter, int diag_mode, TS* out,
                                                    int64_t ldo, int64_t rows_total,
                                                    int64_t cols_total, int lower_only,
-                                                   double* __restrict__ diagres) {
+                                                   double* __restrict__ dblk) {
   __shared__ double xi_s[KT][DCH + 1];
   __shared__ double zj_s[DCH][KT + 1];
 
@@ -105,11 +105,13 @@ __global__ __launch_bounds__(256) void kmat_kernel(int kind, const double* __res
     } else {
       v = (i == j && !(diag_mode & KM_ZERO_PAD)) ? 1.0 : 0.0;
     }
-    const TS sv = (TS)v;
-    out[i * ldo + j] = sv;
-    // The diagonal of Sigma is one value repeated, so its float32 rounding is a
-    // systematic shift; keep the float64 residual for the leaf factorisation.
-    if (diagres && i == j) diagres[i] = v - (double)sv;
+    out[i * ldo + j] = (TS)v;
+    // Diagonal NB-blocks also go to a float64 side buffer (lower_only mode):
+    // the factorisation keeps its pivots' blocks in float64.
+    if (dblk && (i / NB) == (j / NB)) {
+      const int64_t bk = i / NB;
+      dblk[bk * NB * NB + (i - bk * NB) * NB + (j - bk * NB)] = v;
+    }
   }
 }
 
@@ -119,7 +121,7 @@ template <typename TS>
 int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, int64_t m,
                          int d, const double* theta, int nls, double jitter, int diag_mode,
                          TS* out, int64_t ldo, int64_t rows_total, int64_t cols_total,
-                         bool lower_only, hipStream_t s, double* diagres) {
+                         bool lower_only, hipStream_t s, double* dblk) {
   if (rows_total <= 0 || cols_total <= 0) return GPK_OK;
   dim3 grid;
   if (lower_only) {
@@ -139,7 +141,7 @@ int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, 
   }
   hipLaunchKernelGGL(kmat_kernel<TS>, grid, dim3(256), 0, s, kind, x, n, z, m, d, theta, nls,
                      jitter, diag_mode, out, ldo, rows_total, cols_total, lower_only ? 1 : 0,
-                     diagres);
+                     dblk);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }

@@ -36,7 +36,7 @@ __global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_
                                                             double* __restrict__ leafinv,
                                                             double* __restrict__ logdet,
                                                             int* __restrict__ info,
-                                                            const double* __restrict__ diagres) {
+                                                            const double* __restrict__ dsrc) {
   __shared__ double col_s[2][NB];
   __shared__ double dinv_s[NB];
   __shared__ double tr_s[NB][NB + 1];
@@ -51,8 +51,8 @@ __global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_
 #pragma unroll
     for (int b = 0; b < EB; ++b) {
       const int i = rb + a, j = cb + b;
-      s[a][b] = (j <= i) ? (double)row[b] : 0.0;
-      if (diagres && i == j) s[a][b] += diagres[o + i];
+      const double v = dsrc ? dsrc[i * NB + j] : (double)row[b];
+      s[a][b] = (j <= i) ? v : 0.0;
     }
   }
   if (cb == 0) {
@@ -139,9 +139,9 @@ __global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_
 
 template <typename TS>
 int launch_leaf(TS* A, TS* X, int64_t ld, int64_t o, double* leafinv, double* logdet, int* info,
-                const double* diagres, hipStream_t s) {
+                const double* dsrc, hipStream_t s) {
   hipLaunchKernelGGL(leaf_kernel<TS>, dim3(1), dim3(LEAF_THREADS), 0, s, A, X, ld, o, leafinv,
-                     logdet, info, diagres);
+                     logdet, info, dsrc);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }
