@@ -433,9 +433,9 @@ size_t gpk_mll_workspace_bytes(int64_t n, int64_t d, int prec) {
 }
 
 int gpk_set_mixed_arith(int potrf_products, int trtri_products, int lauum_products) {
-  auto ok = [](int v) { return v == AR_BF16X3 || v == AR_BF16X6; };
+  auto ok = [](int v) { return v == AR_F64 || v == AR_BF16X3 || v == AR_BF16X6; };
   if (!ok(potrf_products) || !ok(trtri_products) || !ok(lauum_products)) {
-    set_error("mixed arithmetic: products must be 3 or 6");
+    set_error("mixed arithmetic: products must be 0 (float64), 3 or 6");
     return GPK_ERR_BAD_ARG;
   }
   g_ar_potrf = potrf_products;

@@ -8,7 +8,7 @@
 //               k-slabs of 16, double-buffered LDS (one barrier per slab).
 //   AR_BF16X3 : fp32 operands split on load into bf16 hi + lo planes;
 //               products hi*hi + hi*lo + lo*hi (Henry, Tang & Heinecke,
-//               ARITH 2019) on v_mfma_f32_16x16x32_bf16, fp32 accumulate.
+//               ARITH 2019) on v_mfma_f32_16x16x32_bf16.
 //   AR_BF16X6 : three planes hi + mid + lo, six products
 //               (hh, hm, mh, mm, hl, lh): about fp32-product accuracy.
 // Operand layouts: A(m, k) = A[m*lda + k] (row-contiguous) or A[k*lda + m]
@@ -29,14 +29,15 @@ namespace gpk {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 constexpr int TILE = 128;
 
-// Output accumulators of every main loop are float64: the bf16 products
-// accumulate in fp32 on the matrix cores over k-chunks of BF_FLUSH_SLABS
-// slabs and each chunk is added into float64 (a single fp32 chain over
-// k = 65,536 loses ~1e-3 relative on sums of positive terms).
+// Output accumulators of every main loop are float64.  In the bf16 loops the
+// matrix cores accumulate in fp32 over short k-chunks only (see
+// mainloop_bf16); each chunk is then added into float64.
 template <int AR>
 struct AccT {
   typedef f64x4 vec;
@@ -49,23 +50,24 @@ struct AccT {
 // NP planes of sixteen bf16, packed as two uint4 per plane.
 template <int NP>
 __device__ __forceinline__ void split_pack16(const float (&v)[16], uint4 (&out)[NP][2]) {
+  unsigned wq[NP][8];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    unsigned wq[NP][4];
+  for (int e = 0; e < 8; ++e) {
+    f32x2 r = {v[2 * e], v[2 * e + 1]};
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      float r0 = v[h * 8 + 2 * e], r1 = v[h * 8 + 2 * e + 1];
-#pragma unroll
-      for (int q = 0; q < NP; ++q) {
-        const __bf16 b0 = (__bf16)r0, b1 = (__bf16)r1;
-        wq[q][e] = (unsigned)__builtin_bit_cast(unsigned short, b0) |
-                   ((unsigned)__builtin_bit_cast(unsigned short, b1) << 16);
-        r0 -= (float)b0;
-        r1 -= (float)b1;
+    for (int q = 0; q < NP; ++q) {
+      const unsigned u = __builtin_bit_cast(unsigned, __builtin_convertvector(r, bf16x2));
+      wq[q][e] = u;
+      if (q + 1 < NP) {
+        r.x -= __builtin_bit_cast(float, u << 16);
+        r.y -= __builtin_bit_cast(float, u & 0xffff0000u);
       }
     }
+  }
 #pragma unroll
-    for (int q = 0; q < NP; ++q) out[q][h] = make_uint4(wq[q][0], wq[q][1], wq[q][2], wq[q][3]);
+  for (int q = 0; q < NP; ++q) {
+    out[q][0] = make_uint4(wq[q][0], wq[q][1], wq[q][2], wq[q][3]);
+    out[q][1] = make_uint4(wq[q][4], wq[q][5], wq[q][6], wq[q][7]);
   }
 }
 
@@ -75,13 +77,14 @@ constexpr int F64_LDP = TILE + 1;  // padded row length (doubles)
 constexpr size_t F64_SMEM = 2 /*A,B*/ * 2 /*buffers*/ * F64_BK * F64_LDP * sizeof(double);
 constexpr int BF_BK = 32;
 constexpr int BF_LDK = BF_BK + 8;  // padded k length (bf16)
-constexpr int BF_FLUSH_SLABS = 8;  // fp32 chunk length: 8 slabs = 256 k
+constexpr int BF_CHUNK = 4;        // slabs per fp32 chunk: 128 k
 template <int AR>
 constexpr int bf_planes() { return AR == AR_BF16X6 ? 3 : 2; }
 template <int AR>
 constexpr size_t smem_bytes() {
   return AR == AR_F64 ? F64_SMEM
-                      : (size_t)2 * bf_planes<AR>() * TILE * BF_LDK * sizeof(unsigned short);
+                      : (size_t)2 /*buffers*/ * 2 /*A,B*/ * bf_planes<AR>() * TILE * BF_LDK *
+                            sizeof(unsigned short);
 }
 
 template <typename T>
@@ -178,35 +181,59 @@ __device__ __forceinline__ void mainloop_f64(const TA* __restrict__ A, int64_t l
 // ---------------------------------------------------------------------------
 // bf16-split main loop (fp32 operands)
 // ---------------------------------------------------------------------------
+// Software pipeline over k-slabs of BF_BK with two LDS buffers and one
+// barrier per slab: while the matrix cores work on slab s (buffer s&1), the
+// registers holding slab s+1 are split into bf16 planes and stored into
+// buffer (s+1)&1, and the global loads of slab s+2 are in flight.
+//
+// Accumulation: the leading product hi*hi goes into its own fp32
+// accumulators (one fp32 rounding per slab), the correction products, 2^-8
+// and less of it, into a second set.  Both restart from zero every BF_CHUNK
+// slabs (128 k; every k-range is a multiple of 128) and are then added into
+// the float64 accumulators.  With all six products in one fp32 accumulator
+// over 256 k, the rounding of the running sums dominated the error of the
+// whole mixed-precision factorisation (float64 products of the same float32
+// operands are ~1000x closer to the float64 result).
+// Instruction interleaving for one slab of the bf16 loop, as scheduling-group
+// hints (masks: 0x008 MFMA, 0x002 VALU, 0x100 DS read, 0x200 DS write,
+// 0x020 VMEM read).  With one wave per SIMD the split of the next slab can
+// only hide under this slab's MFMAs if the two are interleaved: the plane-0
+// fragment reads go first, then every MFMA is followed by either one more
+// fragment read or a few VALU instructions, with the DS writes of the next
+// slab and the global loads of the one after spread over the rest.
+template <int AR, bool A_KM, bool B_KN, bool FLUSH>
+__device__ __forceinline__ void sched_bf16_slab() {
+  constexpr int NP = bf_planes<AR>();
+  constexpr int NM = 16 * (AR == AR_BF16X6 ? 6 : 3);     // MFMAs
+  constexpr int NR = 8 * NP;                              // ds_read_b128
+  constexpr int NW = 4 * NP;                              // ds_write_b128
+  constexpr int NG = (A_KM ? 16 : 4) + (B_KN ? 16 : 4);  // global loads
+  constexpr int NV = FLUSH ? 5 : 3;                       // VALU per MFMA slot
+  constexpr int W0 = NR;                                  // first DS write slot
+  constexpr int WS = (NM - W0) / NW > 0 ? (NM - W0) / NW : 1;
+  constexpr int G0 = NM - NG > W0 ? NM - NG : W0;         // first VMEM slot
+#pragma unroll
+  for (int q = 0; q < NM; ++q) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x002, NV, 0);
+  }
+}
+
 template <int AR, bool A_KM, bool B_KN>
 __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64_t lda,
                                               const float* __restrict__ B, int64_t ldb,
                                               int64_t m0, int64_t n0, int64_t kb, int64_t ke,
                                               f64x4 (&acc64)[4][4], char* smem) {
   constexpr int NP = bf_planes<AR>();
-  unsigned short* As = reinterpret_cast<unsigned short*>(smem);   // [NP][TILE][LDK]
-  unsigned short* Bs = As + NP * TILE * BF_LDK;                    // [NP][TILE][LDK]
+  constexpr int OPS = NP * TILE * BF_LDK;  // bf16 elements of one operand in one buffer
+  unsigned short* S = reinterpret_cast<unsigned short*>(smem);  // [2][A, B][NP][TILE][LDK]
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w >> 1, wn = w & 1;
-  f32x4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-      acc64[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
-    }
+    for (int j = 0; j < 4; ++j) acc64[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
   if (kb >= ke) return;
-  auto flush = [&]() {
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)acc[i][j][r];
-        acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-      }
-  };
 
   // Every thread stages 16 consecutive k of one row of each operand per slab.
   // Row-contiguous operand: row = t/2, k = (t%2)*16 .. +15 (four 16-byte loads).
@@ -246,44 +273,104 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
       d[1] = pk[pl][1];
     }
   };
+  auto sstore = [&](int buf) {
+    sstore_one(S + buf * 2 * OPS, ra, A_KM);
+    sstore_one(S + buf * 2 * OPS + OPS, rb, B_KN);
+  };
 
-  const int64_t nslab = (ke - kb) / BF_BK;
+  const int64_t nslab = (ke - kb) / BF_BK;  // a multiple of BF_CHUNK
   gload(kb);
-  for (int64_t sl = 0; sl < nslab; ++sl) {
-    sstore_one(As, ra, A_KM);
-    sstore_one(Bs, rb, B_KN);
-    __syncthreads();
-    if (sl + 1 < nslab) gload(kb + (sl + 1) * BF_BK);
-    bf16x8 av[NP][4], bv[NP][4];
+  sstore(0);
+  gload(kb + BF_BK);
+  __syncthreads();
+  const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+  f32x4 accc[4][4];
 #pragma unroll
-    for (int pl = 0; pl < NP; ++pl) {
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) accc[i][j] = zero;
+  for (int64_t c0 = 0; c0 < nslab; c0 += BF_CHUNK) {
+    f32x4 acch[4][4];
+#pragma unroll
+    for (int u = 0; u < BF_CHUNK; ++u) {
+      const unsigned short* As = S + (u & 1) * 2 * OPS;
+      const unsigned short* Bs = As + OPS;
+      // Fragments of planes 0 and 1; plane 2 (x6) later reuses plane 1's registers.
+      bf16x8 a0[4], b0[4], a1[4], b1[4];
+      auto frag = [&](const unsigned short* P, int pl, int base, int i) {
+        return *reinterpret_cast<const bf16x8*>(
+            P + (pl * TILE + base + i * 16 + (lane & 15)) * BF_LDK + 8 * (lane >> 4));
+      };
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a0[i] = frag(As, 0, wm * 64, i);
+        b0[i] = frag(Bs, 0, wn * 64, i);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a1[i] = frag(As, 1, wm * 64, i);
+        b1[i] = frag(Bs, 1, wn * 64, i);
+      }
+      // Leading products first: they need only plane 0, and at the chunk's
+      // last slab their float64 flush overlaps the correction products.
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        av[pl][i] = *reinterpret_cast<const bf16x8*>(
-            As + (pl * TILE + wm * 64 + i * 16 + (lane & 15)) * BF_LDK + 8 * (lane >> 4));
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        bv[pl][j] = *reinterpret_cast<const bf16x8*>(
-            Bs + (pl * TILE + wn * 64 + j * 16 + (lane & 15)) * BF_LDK + 8 * (lane >> 4));
-    }
+        for (int j = 0; j < 4; ++j)
+          acch[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[i], b0[j],
+                                                               u == 0 ? zero : acch[i][j], 0, 0, 0);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        f32x4 c = acc[i][j];
-        if (AR == AR_BF16X6) {
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[2][i], bv[0][j], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0][i], bv[2][j], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[1][i], bv[1][j], c, 0, 0, 0);
+        for (int j = 0; j < 4; ++j) {
+          f32x4 c = accc[i][j];
+          if (AR == AR_BF16X6) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[i], b1[j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[i], b0[j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[i], b1[j], c, 0, 0, 0);
+          accc[i][j] = c;
         }
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[1][i], bv[0][j], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0][i], bv[1][j], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0][i], bv[0][j], c, 0, 0, 0);
-        acc[i][j] = c;
+      if (AR == AR_BF16X6) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          a1[i] = frag(As, 2, wm * 64, i);
+          b1[i] = frag(Bs, 2, wn * 64, i);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            f32x4 c = accc[i][j];
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[i], b0[j], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[i], b1[j], c, 0, 0, 0);
+            accc[i][j] = c;
+          }
       }
-    __syncthreads();
-    if ((sl + 1) % BF_FLUSH_SLABS == 0 || sl + 1 == nslab) flush();
+      if (u == BF_CHUNK - 1) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)acch[i][j][r];
+      }
+      // Stage slab s+1 (at the last slab a repeat of it, stored and never read)
+      // and start loading slab s+2.
+      sstore((u + 1) & 1);
+      const int64_t s2 = c0 + u + 2 < nslab ? c0 + u + 2 : nslab - 1;
+      gload(kb + s2 * BF_BK);
+      if (u == BF_CHUNK - 1) sched_bf16_slab<AR, A_KM, B_KN, true>();
+      else sched_bf16_slab<AR, A_KM, B_KN, false>();
+      __syncthreads();
+    }
   }
+  // The correction products are at most 2^-8 of the leading ones, so their
+  // single fp32 chain over the whole k-range adds little error.
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)accc[i][j][r];
 }
 
 // Row (within a 16 x 16 block) of accumulator element r held by `lane`.

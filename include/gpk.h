@@ -157,7 +157,8 @@ int gpk_predict(int kind, int prec, const double* x, const double* y, int64_t n,
 
 /*
  * Products per element used by the GPK_PREC_MIXED stages (3 = bf16 hi/lo
- * split, 6 = hi/mid/lo split): Cholesky updates, triangular inverse, and the
+ * split, 6 = hi/mid/lo split, 0 = float64 products of the float32 operands):
+ * Cholesky updates, triangular inverse, and the
  * fused Sigma^-1 adjoint.  Process-wide; defaults 6, 3, 6.  No counterpart in
  * the reference (which has a single float64 path); it exists to trade speed
  * against the accuracy figures the benchmark reports.
