@@ -42,6 +42,22 @@ def lauum_flops(n: int, tile: int = 128) -> float:
     return 2.0 * tile ** 3 * nt * (nt + 1) * (nt + 2) / 6.0
 
 
+def profiled_traffic(n: int):
+    """HBM bytes per launch of the adjoint kernel at N = 65,536, from the committed
+    rocprofv3 FETCH_SIZE + WRITE_SIZE summary (scaled by the copy calibration)."""
+    if n != 65536:
+        return None
+    path = os.path.join(ROOT, "profiles", "adjoint_bf16x6_n65536_traffic.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        return None
+    return None
+
+
 def run_steps(post, D, precision, steps, params, report=False):
     """`steps` MLL + gradient evaluations; with report=True each step syncs and
     prints its wall time (used for the untimed warm-up steps only)."""
@@ -179,7 +195,8 @@ def main() -> int:
     roofline = {"kernel": "adjoint (fused L^-T L^-1 tile GEMM + gradient epilogue, bf16x6)",
                 "bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 4),
-                "traffic": None, "avg_launch_ms": round(adj_s * 1e3, 3)}
+                "traffic": profiled_traffic(args.n), "traffic_unit": "bytes per launch (HBM)",
+                "algorithmic_flops_per_launch": flops, "avg_launch_ms": round(adj_s * 1e3, 3)}
 
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
 

@@ -18,10 +18,17 @@ class Constant:
             self.constant = constant
         else:
             self.constant = torch.as_tensor(constant, dtype=torch.float64)
+        self._on_device: dict = {}
 
     def value_tensor(self, device) -> torch.Tensor:
-        c = self.constant.value if isinstance(self.constant, Parameter) else self.constant
-        return c.reshape(()).to(device, torch.float64)
+        if isinstance(self.constant, Parameter):
+            return self.constant.value.reshape(()).to(device, torch.float64)
+        # A fixed constant is copied to each device once, so that later calls
+        # (e.g. inside a HIP-graph capture) issue no host-to-device copy.
+        key = str(torch.device(device))
+        if key not in self._on_device:
+            self._on_device[key] = self.constant.reshape(()).to(device, torch.float64)
+        return self._on_device[key]
 
     def __call__(self, x: torch.Tensor) -> torch.Tensor:
         return torch.ones((x.shape[0], 1), dtype=torch.float64, device=x.device) * \
