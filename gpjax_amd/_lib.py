@@ -43,6 +43,9 @@ SIGNATURES = {
     "gpk_predict_workspace_bytes": (_S, [_L, _L, _L, _I, _I]),
     "gpk_predict": (_I, [_I, _I, _P, _P, _L, _L, _P, _L, _P, _I, _D, _D, _I, _I, _P, _S, _P, _P,
                          _P]),
+    "gpk_batched_workspace_bytes": (_S, [_P, _L, _L]),
+    "gpk_batched_plan": (_I, [_P, _L, _L, _P, _S]),
+    "gpk_batched_mll_grad": (_I, [_I, _P, _P, _P, _L, _L, _P, _I, _D, _I, _P, _S, _P, _P, _P]),
     "gpk_profile_enable": (_I, [_I]),
     "gpk_profile_collect": (_I, [_P, _P]),
 }

@@ -25,8 +25,12 @@ __global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __rest
                                                       const double* __restrict__ x, int64_t n,
                                                       int d, const double* __restrict__ theta,
                                                       int nls, const double* __restrict__ alpha,
-                                                      double* __restrict__ partial) {
+                                                      double* __restrict__ partial, Batch bt) {
   typedef typename AccT<AR>::scalar AS;
+  bmember(bt, d, x, n, theta);
+  X = bws(X, bt);
+  alpha = bws(alpha, bt);
+  partial = bws(partial, bt);
   constexpr size_t ML = smem_bytes<AR>();
   constexpr size_t XS = (size_t)2 * TILE * (DMAX + 1) * sizeof(double);
   constexpr size_t AB = (size_t)2 * TILE * sizeof(double);
@@ -145,12 +149,12 @@ __global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __rest
 template <int AR, typename TS>
 int launch_ar(int kind, const TS* X, int64_t ld, int64_t np, const double* x, int64_t n, int d,
               const double* theta, int nls, const double* alpha, double* partial,
-              hipStream_t s) {
+              hipStream_t s, const Batch& bt) {
   const int64_t nt = np / TILE;
-  dim3 grid((unsigned)(nt * (nt + 1) / 2));
+  const dim3 grid((unsigned)(nt * (nt + 1) / 2), 1, bt.count);
 #define GPK_ADJ(DM)                                                                        \
   hipLaunchKernelGGL((adjoint_kernel<AR, TS, DM>), grid, dim3(256), 0, s, kind, X, ld, np, x, \
-                     n, d, theta, nls, alpha, partial)
+                     n, d, theta, nls, alpha, partial, bt)
   if (d <= 1) GPK_ADJ(1);
   else if (d <= 2) GPK_ADJ(2);
   else if (d <= 4) GPK_ADJ(4);
@@ -167,7 +171,7 @@ int launch_ar(int kind, const TS* X, int64_t ld, int64_t np, const double* x, in
 template <typename TS>
 int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, const double* x,
                    int64_t n, int d, const double* theta, int nls, const double* alpha,
-                   double* partial, hipStream_t s) {
+                   double* partial, hipStream_t s, const Batch& bt) {
   if (d < 1 || d > MAX_D) {
     set_error("adjoint: input dimension %d outside [1, %d]", d, MAX_D);
     return GPK_ERR_BAD_ARG;
@@ -176,10 +180,15 @@ int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, con
     set_error("adjoint: padded size %lld not a multiple of %d", (long long)np, TILE);
     return GPK_ERR_BAD_ARG;
   }
-  if (arith == AR_F64) return launch_ar<AR_F64, TS>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s);
+  if (arith == AR_F64)
+    return launch_ar<AR_F64, TS>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s, bt);
   if constexpr (std::is_same<TS, float>::value) {
-    if (arith == AR_BF16X3) return launch_ar<AR_BF16X3, float>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s);
-    if (arith == AR_BF16X6) return launch_ar<AR_BF16X6, float>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s);
+    if (arith == AR_BF16X3)
+      return launch_ar<AR_BF16X3, float>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s,
+                                         bt);
+    if (arith == AR_BF16X6)
+      return launch_ar<AR_BF16X6, float>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s,
+                                         bt);
   }
   set_error("adjoint: unsupported arithmetic %d", arith);
   return GPK_ERR_BAD_ARG;
@@ -187,9 +196,9 @@ int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, con
 
 template int launch_adjoint<double>(int, int, const double*, int64_t, int64_t, const double*,
                                     int64_t, int, const double*, int, const double*, double*,
-                                    hipStream_t);
+                                    hipStream_t, const Batch&);
 template int launch_adjoint<float>(int, int, const float*, int64_t, int64_t, const double*,
                                    int64_t, int, const double*, int, const double*, double*,
-                                   hipStream_t);
+                                   hipStream_t, const Batch&);
 
 }  // namespace gpk

@@ -12,6 +12,7 @@
 //   5. logdet = sum of leaf log-diagonals, quad = r . alpha, mll.
 // Backward (gpk_mll_backward): the fused LAUUM + adjoint tile pass and a
 // deterministic reduction to the gradient.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -115,18 +116,31 @@ Layout make_layout(int64_t n, int64_t d, int prec) {
 
 // ---- small kernels ----
 __global__ void prep_r_kernel(const double* __restrict__ y, const double* __restrict__ theta,
-                              int nls, int64_t n, int64_t np, double* __restrict__ r) {
+                              int nls, int64_t n, int64_t np, double* __restrict__ r, Batch bt) {
+  if (bt.gp) {
+    const int64_t p = bt.gp[blockIdx.z];
+    y += bt.xoff[p];
+    n = bt.xoff[p + 1] - bt.xoff[p];
+    theta += p * bt.theta_stride;
+  }
+  r = bws(r, bt);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < np) r[i] = i < n ? y[i] - theta[nls + 2] : 0.0;
 }
 
 __global__ void sub_kernel(const double* __restrict__ a, const double* __restrict__ b,
-                           int64_t np, double* __restrict__ out) {
+                           int64_t np, double* __restrict__ out, Batch bt) {
+  a = bws(a, bt);
+  b = bws(b, bt);
+  out = bws(out, bt);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < np) out[i] = a[i] - b[i];
 }
 
-__global__ void add_kernel(double* __restrict__ a, const double* __restrict__ b, int64_t np) {
+__global__ void add_kernel(double* __restrict__ a, const double* __restrict__ b, int64_t np,
+                           Batch bt) {
+  a = bws(a, bt);
+  b = bws(b, bt);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < np) a[i] += b[i];
 }
@@ -147,8 +161,19 @@ __device__ double block_sum(double v, double* red) {
 __global__ __launch_bounds__(256) void finalize_fwd_kernel(
     const double* __restrict__ leafld, const int* __restrict__ leafinfo, int64_t nleaf,
     const double* __restrict__ r, const double* __restrict__ alpha,
-    const double* __restrict__ res, int64_t n, int64_t np, double* __restrict__ result) {
+    const double* __restrict__ res, int64_t n, int64_t np, double* __restrict__ result,
+    Batch bt) {
   __shared__ double red[4];
+  leafld = bws(leafld, bt);
+  leafinfo = bws(leafinfo, bt);
+  r = bws(r, bt);
+  alpha = bws(alpha, bt);
+  res = bws(res, bt);
+  if (bt.gp) {
+    const int64_t p = bt.gp[blockIdx.z];
+    n = bt.xoff[p + 1] - bt.xoff[p];
+    result += p * GPK_RESULT_LEN;
+  }
   double q = 0.0, rr = 0.0, ee = 0.0;
   for (int64_t i = threadIdx.x; i < np; i += 256) {
     if (i < n) q += r[i] * alpha[i];
@@ -180,8 +205,17 @@ __global__ __launch_bounds__(256) void finalize_fwd_kernel(
 // One workgroup per gradient slot: sum the tile partials in a fixed order.
 __global__ __launch_bounds__(256) void finalize_bwd_kernel(
     const double* __restrict__ gpart, int64_t ntile, int nls, const double* __restrict__ theta,
-    const double* __restrict__ alpha, int64_t n, double gbar, double* __restrict__ grad) {
+    const double* __restrict__ alpha, int64_t n, double gbar, double* __restrict__ grad,
+    Batch bt) {
   __shared__ double red[4];
+  gpart = bws(gpart, bt);
+  alpha = bws(alpha, bt);
+  if (bt.gp) {
+    const int64_t p = bt.gp[blockIdx.z];
+    n = bt.xoff[p + 1] - bt.xoff[p];
+    theta += p * bt.theta_stride;
+    grad += p * (nls + 3);
+  }
   const int q = blockIdx.x;
   const int nv = nls + 2;
   double s = 0.0;
@@ -233,8 +267,10 @@ int check_common(int kind, int prec, const void* x, int64_t n, int64_t d, const 
 template <typename TS>
 int mll_forward_t(int kind, int prec, const double* x, const double* y, int64_t n, int d,
                   const double* theta, int nls, double jitter, int iters, void* ws,
-                  const Layout& Lo, double* result, hipStream_t s) {
+                  const Layout& Lo, double* result, hipStream_t s,
+                  const Batch& bt = Batch()) {
   const int64_t np = Lo.np;
+  const dim3 gv(nblk(np), 1, bt.count);
   TS* L = at<TS>(ws, Lo.off_L);
   TS* X = at<TS>(ws, Lo.off_X);
   double* r = at<double>(ws, Lo.off_r);
@@ -258,13 +294,14 @@ int mll_forward_t(int kind, int prec, const double* x, const double* y, int64_t 
   // float32 storage: the diagonal NB-blocks live in float64 on the side
   double* dblk = f64 ? nullptr : at<double>(ws, Lo.off_dblk);
   F.dblk = dblk;
+  F.bt = bt;
 
-  hipLaunchKernelGGL(prep_r_kernel, dim3(nblk(np)), dim3(256), 0, s, y, theta, nls, n, np, r);
+  hipLaunchKernelGGL(prep_r_kernel, gv, dim3(256), 0, s, y, theta, nls, n, np, r, bt);
   GPK_LAUNCH_CHECK();
   {
     PhaseScope ps(GPK_PHASE_GRAM, s);
     GPK_TRY(launch_kernel_matrix<TS>(kind, x, n, x, n, d, theta, nls, jitter, 2, L, np, np, np,
-                                     true, s, dblk));
+                                     true, s, dblk, bt));
   }
   {
     PhaseScope ps(GPK_PHASE_POTRF, s);
@@ -276,39 +313,41 @@ int mll_forward_t(int kind, int prec, const double* x, const double* y, int64_t 
   }
   PhaseScope ps_solve(GPK_PHASE_SOLVE, s);
   // alpha0 = X^T X r
-  GPK_TRY(launch_trmv<TS>(X, np, np, r, u, false, part, s));
-  GPK_TRY(launch_trmv<TS>(X, np, np, u, alpha, true, part, s));
+  GPK_TRY(launch_trmv<TS>(X, np, np, r, u, false, part, s, bt));
+  GPK_TRY(launch_trmv<TS>(X, np, np, u, alpha, true, part, s, bt));
   for (int it = 0; it < iters; ++it) {
-    GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, alpha, v, part, s));
-    hipLaunchKernelGGL(sub_kernel, dim3(nblk(np)), dim3(256), 0, s, r, v, np, res);
+    GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, alpha, v, part, s, bt));
+    hipLaunchKernelGGL(sub_kernel, gv, dim3(256), 0, s, r, v, np, res, bt);
     GPK_LAUNCH_CHECK();
-    GPK_TRY(launch_trmv<TS>(X, np, np, res, u, false, part, s));
-    GPK_TRY(launch_trmv<TS>(X, np, np, u, delta, true, part, s));
-    hipLaunchKernelGGL(add_kernel, dim3(nblk(np)), dim3(256), 0, s, alpha, delta, np);
+    GPK_TRY(launch_trmv<TS>(X, np, np, res, u, false, part, s, bt));
+    GPK_TRY(launch_trmv<TS>(X, np, np, u, delta, true, part, s, bt));
+    hipLaunchKernelGGL(add_kernel, gv, dim3(256), 0, s, alpha, delta, np, bt);
     GPK_LAUNCH_CHECK();
   }
-  GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, alpha, v, part, s));
-  hipLaunchKernelGGL(sub_kernel, dim3(nblk(np)), dim3(256), 0, s, r, v, np, res);
+  GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, alpha, v, part, s, bt));
+  hipLaunchKernelGGL(sub_kernel, gv, dim3(256), 0, s, r, v, np, res, bt);
   GPK_LAUNCH_CHECK();
-  hipLaunchKernelGGL(finalize_fwd_kernel, dim3(1), dim3(256), 0, s, at<double>(ws, Lo.off_leafld),
-                     at<int>(ws, Lo.off_leafinfo), np / NB, r, alpha, res, n, np, result);
+  hipLaunchKernelGGL(finalize_fwd_kernel, dim3(1, 1, bt.count), dim3(256), 0, s,
+                     at<double>(ws, Lo.off_leafld), at<int>(ws, Lo.off_leafinfo), np / NB, r,
+                     alpha, res, n, np, result, bt);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }
 
 template <typename TS>
 int mll_backward_t(int kind, const double* x, int64_t n, int d, const double* theta, int nls,
-                   double gbar, void* ws, const Layout& Lo, double* grad, hipStream_t s) {
+                   double gbar, void* ws, const Layout& Lo, double* grad, hipStream_t s,
+                   const Batch& bt = Batch()) {
   const int64_t np = Lo.np;
   const bool f64 = std::is_same<TS, double>::value;
   double* gpart = at<double>(ws, Lo.off_gpart);
   PhaseScope ps(GPK_PHASE_ADJOINT, s);
   GPK_TRY(launch_adjoint<TS>(kind, f64 ? (int)AR_F64 : g_ar_lauum, at<TS>(ws, Lo.off_X), np, np,
-                             x, n, d, theta, nls, at<double>(ws, Lo.off_alpha), gpart, s));
+                             x, n, d, theta, nls, at<double>(ws, Lo.off_alpha), gpart, s, bt));
   const int64_t nleaf = np / NB;
-  hipLaunchKernelGGL(finalize_bwd_kernel, dim3((unsigned)(nls + 3)), dim3(256), 0, s, gpart,
-                     nleaf * (nleaf + 1) / 2, nls, theta, at<double>(ws, Lo.off_alpha), n, gbar,
-                     grad);
+  hipLaunchKernelGGL(finalize_bwd_kernel, dim3((unsigned)(nls + 3), 1, bt.count), dim3(256), 0, s,
+                     gpart, nleaf * (nleaf + 1) / 2, nls, theta, at<double>(ws, Lo.off_alpha), n,
+                     gbar, grad, bt);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }
@@ -386,6 +425,98 @@ int predict_t(int kind, int prec, const double* x, const double* y, int64_t n, i
     hipLaunchKernelGGL(pred_var_kernel, dim3(nblk(t)), dim3(256), 0, s, vec, theta, nls,
                        jitter_prior, t, cov);
     GPK_LAUNCH_CHECK();
+  }
+  return GPK_OK;
+}
+
+// ---- batched exact GPs ----
+// Problems are grouped by padded size np = round_up(n, 256) (stable in the
+// problem index); each group runs the exact-GP forward and backward once, with
+// one member per blockIdx.z.  Workspace: the member -> problem map and the row
+// offsets (uploaded by gpk_batched_plan), then per group `count` member
+// workspaces of one Layout each.
+struct BatchGroup {
+  int64_t np = 0, first = 0, count = 0;
+  size_t off = 0, stride = 0;
+  Layout lo;
+};
+struct BatchPlan {
+  std::vector<int> order;  // member slot -> problem index
+  std::vector<BatchGroup> groups;
+  size_t off_gp = 0, off_xoff = 0, total = 0;
+};
+
+int check_offsets(const int64_t* off, int64_t count, int64_t d) {
+  if (!off || count < 1) {
+    set_error("batched: need offsets and count >= 1");
+    return GPK_ERR_BAD_ARG;
+  }
+  if (d < 1 || d > MAX_D) {
+    set_error("batched: need 1 <= d <= %d (got %lld)", MAX_D, (long long)d);
+    return GPK_ERR_BAD_ARG;
+  }
+  if (off[0] != 0) {
+    set_error("batched: offsets[0] must be 0");
+    return GPK_ERR_BAD_ARG;
+  }
+  for (int64_t p = 0; p < count; ++p) {
+    if (off[p + 1] - off[p] < 1) {
+      set_error("batched: problem %lld has no rows", (long long)p);
+      return GPK_ERR_BAD_ARG;
+    }
+  }
+  return GPK_OK;
+}
+
+BatchPlan make_batch_plan(const int64_t* off, int64_t count, int64_t d) {
+  BatchPlan P;
+  P.order.resize((size_t)count);
+  for (int64_t p = 0; p < count; ++p) P.order[(size_t)p] = (int)p;
+  auto padded = [&](int p) { return round_up(off[p + 1] - off[p], 256); };
+  std::stable_sort(P.order.begin(), P.order.end(),
+                   [&](int a, int b) { return padded(a) < padded(b); });
+  size_t o = 0;
+  P.off_gp = o; o += al((size_t)count * sizeof(int));
+  P.off_xoff = o; o += al((size_t)(count + 1) * sizeof(int64_t));
+  for (int64_t i = 0; i < count;) {
+    BatchGroup g;
+    g.np = padded(P.order[(size_t)i]);
+    g.first = i;
+    while (i < count && padded(P.order[(size_t)i]) == g.np) ++i;
+    g.count = i - g.first;
+    g.lo = make_layout(g.np, d, GPK_PREC_FP64);
+    g.stride = al(g.lo.total);
+    g.off = o;
+    o += g.stride * (size_t)g.count;
+    P.groups.push_back(g);
+  }
+  P.total = o;
+  return P;
+}
+
+int batched_mll_grad(int kind, const double* x, const double* y, const int64_t* off,
+                     int64_t count, int d, const double* theta, int nls, double jitter, int iters,
+                     void* ws, double* result, double* grad, hipStream_t s) {
+  const BatchPlan P = make_batch_plan(off, count, d);
+  const int* gp = at<int>(ws, P.off_gp);
+  const int64_t* xoff = at<int64_t>(ws, P.off_xoff);
+  for (const BatchGroup& g : P.groups) {
+    if (g.count > 65535) {
+      set_error("batched: %lld problems of padded size %lld exceed one launch",
+                (long long)g.count, (long long)g.np);
+      return GPK_ERR_BAD_ARG;
+    }
+    Batch bt;
+    bt.ws_stride = (int64_t)g.stride;
+    bt.gp = gp + g.first;
+    bt.xoff = xoff;
+    bt.theta_stride = nls + 3;
+    bt.count = (unsigned)g.count;
+    void* wsg = static_cast<char*>(ws) + g.off;
+    // n is taken per member inside the kernels; np fixes the shapes.
+    GPK_TRY(mll_forward_t<double>(kind, GPK_PREC_FP64, x, y, g.np, d, theta, nls, jitter, iters,
+                                  wsg, g.lo, result, s, bt));
+    GPK_TRY(mll_backward_t<double>(kind, x, g.np, d, theta, nls, 1.0, wsg, g.lo, grad, s, bt));
   }
   return GPK_OK;
 }
@@ -523,6 +654,57 @@ int gpk_predict(int kind, int prec, const double* x, const double* y, int64_t n,
                              jitter_prior, full_cov, iters, ws, Lo, P, mean, cov, s);
   return predict_t<float>(kind, prec, x, y, n, (int)d, xt, t, theta, nls, jitter_post,
                           jitter_prior, full_cov, iters, ws, Lo, P, mean, cov, s);
+}
+
+size_t gpk_batched_workspace_bytes(const int64_t* offsets_host, int64_t count, int64_t d) {
+  if (check_offsets(offsets_host, count, d) != GPK_OK) return 0;
+  return make_batch_plan(offsets_host, count, d).total;
+}
+
+int gpk_batched_plan(const int64_t* offsets_host, int64_t count, int64_t d, void* ws,
+                     size_t ws_bytes) {
+  int rc = check_offsets(offsets_host, count, d);
+  if (rc) return rc;
+  const BatchPlan P = make_batch_plan(offsets_host, count, d);
+  if (!ws || ws_bytes < P.total) {
+    set_error("batched_plan: workspace %zu bytes < required %zu", ws_bytes, P.total);
+    return GPK_ERR_WORKSPACE;
+  }
+  if (reinterpret_cast<uintptr_t>(ws) % ALIGN) {
+    set_error("batched_plan: workspace must be %zu-byte aligned", ALIGN);
+    return GPK_ERR_BAD_ARG;
+  }
+  GPK_HIP_TRY(hipMemcpy(at<int>(ws, P.off_gp), P.order.data(), (size_t)count * sizeof(int),
+                        hipMemcpyHostToDevice));
+  GPK_HIP_TRY(hipMemcpy(at<int64_t>(ws, P.off_xoff), offsets_host,
+                        (size_t)(count + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+  return GPK_OK;
+}
+
+int gpk_batched_mll_grad(int kind, const double* x, const double* y,
+                         const int64_t* offsets_host, int64_t count, int64_t d,
+                         const double* theta, int nls, double jitter, int refine_iters, void* ws,
+                         size_t ws_bytes, double* result, double* grad, void* stream) {
+  int rc = check_offsets(offsets_host, count, d);
+  if (rc) return rc;
+  rc = check_common(kind, GPK_PREC_FP64, x, offsets_host[count], d, theta, nls);
+  if (rc) return rc;
+  if (!y || !ws || !result || !grad) {
+    set_error("batched_mll_grad: null y/ws/result/grad");
+    return GPK_ERR_BAD_ARG;
+  }
+  const size_t need = make_batch_plan(offsets_host, count, d).total;
+  if (ws_bytes < need) {
+    set_error("batched_mll_grad: workspace %zu bytes < required %zu", ws_bytes, need);
+    return GPK_ERR_WORKSPACE;
+  }
+  if (reinterpret_cast<uintptr_t>(ws) % ALIGN) {
+    set_error("batched_mll_grad: workspace must be %zu-byte aligned", ALIGN);
+    return GPK_ERR_BAD_ARG;
+  }
+  const int iters = refine_iters >= 0 ? refine_iters : 1;
+  return batched_mll_grad(kind, x, y, offsets_host, count, (int)d, theta, nls, jitter, iters, ws,
+                          result, grad, (hipStream_t)stream);
 }
 
 int gpk_profile_enable(int on) {

@@ -43,11 +43,14 @@ struct Factor {
   // (float32 storage), SYRK updates those blocks in float64 and the leaves
   // factor them from there, so no pivot block is ever rounded to float32.
   double* dblk = nullptr;
+  // Optional batch: all pointers above are member 0's; member z's are
+  // bt.ws_stride bytes further per z (same padded size for every member).
+  Batch bt = Batch();
 
   double* dblk_at(int64_t o) const { return dblk ? dblk + (o / NB) * NB * NB : nullptr; }
   int leaf(int64_t o) {
     return launch_leaf<TS>(L, X, ld, o, leafinv + (o / NB) * NB * NB, leafld + o / NB,
-                           leafinfo + o / NB, dblk_at(o), s);
+                           leafinfo + o / NB, dblk_at(o), s, bt);
   }
   // Solve Y L^T = B in place, B = L[row0 : row0+m, o : o+n], triangle L[o:o+n, o:o+n].
   int trsm(int64_t row0, int64_t m, int64_t o, int64_t n) {
@@ -55,13 +58,13 @@ struct Factor {
       const double* inv = leafinv + (o / NB) * NB * NB;
       return launch_gemm<TS, double, TS>(AR_F64, false, false, L + row0 * ld + o, ld, inv, NB,
                                          L + row0 * ld + o, ld, m, NB, NB, 1.0, false, false,
-                                         KR_FULL, s);
+                                         KR_FULL, s, nullptr, bt);
     }
     const int64_t h = split_point(n);
     GPK_TRY(trsm(row0, m, o, h));
     GPK_TRY(launch_gemm<TS, TS, TS>(ar_potrf, false, false, L + row0 * ld + o, ld,
                                     L + (o + h) * ld + o, ld, L + row0 * ld + o + h, ld, m,
-                                    n - h, h, -1.0, true, false, KR_FULL, s));
+                                    n - h, h, -1.0, true, false, KR_FULL, s, nullptr, bt));
     return trsm(row0, m, o + h, n - h);
   }
   int potrf(int64_t o, int64_t n) {
@@ -72,7 +75,7 @@ struct Factor {
     GPK_TRY(launch_gemm<TS, TS, TS>(ar_potrf, false, false, L + (o + h) * ld + o, ld,
                                     L + (o + h) * ld + o, ld, L + (o + h) * ld + o + h, ld,
                                     n - h, n - h, h, -1.0, true, true, KR_FULL, s,
-                                    dblk_at(o + h)));
+                                    dblk_at(o + h), bt));
     return potrf(o + h, n - h);
   }
   // X = L^-1 below the leaves, whose diagonal blocks the leaves already wrote.
@@ -84,11 +87,11 @@ struct Factor {
     // T = L21 X11   ((n-h) x h; X11 lower, so k >= column tile)
     GPK_TRY(launch_gemm<TS, TS, TS>(ar_trtri, false, true, L + (o + h) * ld + o, ld,
                                     X + o * ld + o, ld, T, h, n - h, h, h, 1.0, false, false,
-                                    KR_GE_N, s));
+                                    KR_GE_N, s, nullptr, bt));
     // X21 = -X22 T  (X22 lower, so k <= row tile)
     return launch_gemm<TS, TS, TS>(ar_trtri, false, true, X + (o + h) * ld + o + h, ld, T, h,
                                    X + (o + h) * ld + o, ld, n - h, h, n - h, -1.0, false,
-                                   false, KR_LE_M, s);
+                                   false, KR_LE_M, s, nullptr, bt);
   }
 };
 

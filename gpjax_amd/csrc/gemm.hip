@@ -21,11 +21,16 @@ struct GemmParams {
   int lower_tiles;
   int krange;
   double* cdiag;  // lower_tiles: diagonal tiles update this float64 [tile][NB][NB] instead of C
+  Batch bt;       // batch of same-shape problems, member blockIdx.z (operands in workspaces)
 };
 
 template <int AR, typename TA, typename TB, typename TC, bool A_KM, bool B_KN>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
   __shared__ __align__(16) char smem[smem_bytes<AR>()];
+  p.A = bws(p.A, p.bt);
+  p.B = bws(p.B, p.bt);
+  p.C = bws(p.C, p.bt);
+  p.cdiag = bws(p.cdiag, p.bt);
   int64_t mt, nt;
   if (p.lower_tiles) lower_tile(blockIdx.x, mt, nt);
   else { nt = blockIdx.x; mt = blockIdx.y; }
@@ -82,9 +87,9 @@ int launch_t(const GemmParams& p, int64_t M, int64_t N, hipStream_t s) {
   dim3 grid;
   if (p.lower_tiles) {
     const int64_t nt = M / TILE;
-    grid = dim3((unsigned)(nt * (nt + 1) / 2), 1, 1);
+    grid = dim3((unsigned)(nt * (nt + 1) / 2), 1, p.bt.count);
   } else {
-    grid = dim3((unsigned)(N / TILE), (unsigned)(M / TILE), 1);
+    grid = dim3((unsigned)(N / TILE), (unsigned)(M / TILE), p.bt.count);
   }
   hipLaunchKernelGGL((gemm_kernel<AR, TA, TB, TC, A_KM, B_KN>), grid, dim3(256), 0, s, p);
   GPK_LAUNCH_CHECK();
@@ -105,7 +110,8 @@ int dispatch_layout(bool a_km, bool b_kn, const GemmParams& p, int64_t M, int64_
 template <typename TA, typename TB, typename TC>
 int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const TB* B,
                 int64_t ldb, TC* C, int64_t ldc, int64_t M, int64_t N, int64_t K, double alpha,
-                bool beta_one, bool lower_tiles, int krange, hipStream_t s, double* cdiag) {
+                bool beta_one, bool lower_tiles, int krange, hipStream_t s, double* cdiag,
+                const Batch& bt) {
   if (M <= 0 || N <= 0) return GPK_OK;
   if (M % TILE || N % TILE || K % TILE || K < 0) {
     set_error("gemm: extents %lld x %lld x %lld not multiples of %d", (long long)M,
@@ -121,7 +127,7 @@ int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const
     return GPK_ERR_BAD_ARG;
   }
   GemmParams p{A, lda, B, ldb, C, ldc, K, alpha, beta_one ? 1 : 0, lower_tiles ? 1 : 0, krange,
-               cdiag};
+               cdiag, bt};
   if (arith == AR_F64) return dispatch_layout<AR_F64, TA, TB, TC>(a_km, b_kn, p, M, N, s);
   if constexpr (std::is_same<TA, float>::value && std::is_same<TB, float>::value &&
                 std::is_same<TC, float>::value) {
@@ -137,7 +143,7 @@ int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const
 #define GPK_GEMM_INST(TA, TB, TC)                                                            \
   template int launch_gemm<TA, TB, TC>(int, bool, bool, const TA*, int64_t, const TB*,       \
                                        int64_t, TC*, int64_t, int64_t, int64_t, int64_t, double, \
-                                       bool, bool, int, hipStream_t, double*);
+                                       bool, bool, int, hipStream_t, double*, const Batch&);
 GPK_GEMM_INST(double, double, double)
 GPK_GEMM_INST(float, float, float)
 GPK_GEMM_INST(float, double, float)

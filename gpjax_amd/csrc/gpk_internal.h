@@ -43,6 +43,44 @@ void set_error(const char* fmt, ...);
 inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
 // ---------------------------------------------------------------------------
+// Batches of independent problems of one padded size, one member per
+// blockIdx.z.  The workspace of member z sits ws_stride bytes after that of
+// member z - 1, so every workspace pointer of a kernel is shifted by
+// z * ws_stride.  Inputs and outputs are addressed through the member's
+// problem index p = gp[z]: rows xoff[p] .. xoff[p+1]-1 of the concatenated
+// x (and y), hyperparameter row theta + p * theta_stride, output row p.  The
+// default (gp == nullptr, ws_stride == 0, one member) is a single problem.
+// ---------------------------------------------------------------------------
+struct Batch {
+  int64_t ws_stride = 0;
+  const int* gp = nullptr;
+  const int64_t* xoff = nullptr;
+  int64_t theta_stride = 0;
+  unsigned count = 1;
+};
+
+template <typename T>
+__device__ __forceinline__ T* bws(T* p, const Batch& b) {
+  if (!p || !b.ws_stride) return p;
+  return reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) +
+                              (uintptr_t)((int64_t)blockIdx.z * b.ws_stride));
+}
+// Problem index of this member (0 for a single problem).
+__device__ __forceinline__ int64_t bprob(const Batch& b) {
+  return b.gp ? (int64_t)b.gp[blockIdx.z] : 0;
+}
+// Rebase this member's inputs: x rows, n and the hyperparameter row.
+template <typename PX, typename PT>
+__device__ __forceinline__ void bmember(const Batch& b, int d, PX& x, int64_t& n, PT& theta) {
+  if (!b.gp) return;
+  const int64_t p = b.gp[blockIdx.z];
+  const int64_t r0 = b.xoff[p];
+  x += r0 * d;
+  n = b.xoff[p + 1] - r0;
+  theta += p * b.theta_stride;
+}
+
+// ---------------------------------------------------------------------------
 // Stationary kernel formulas (unit variance), from rho^2 of the
 // lengthscale-scaled inputs.  kval() is k(rho); kfac() is F(rho) with
 // dK/dl_d = var * F(rho) * u_d / l_d, u_d = (x_d - z_d)^2 / l_d^2.
@@ -108,7 +146,8 @@ template <typename TS>
 int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, int64_t m,
                          int d, const double* theta, int nls, double jitter, int diag_mode,
                          TS* out, int64_t ldo, int64_t rows_total, int64_t cols_total,
-                         bool lower_only, hipStream_t s, double* dblk = nullptr);
+                         bool lower_only, hipStream_t s, double* dblk = nullptr,
+                         const Batch& bt = Batch());
 
 // Leaf: factor the NB x NB diagonal block at (o, o) of A in float64 in LDS,
 // write L (lower, zero upper) to A, L^-1 to X (lower, zero upper) and to
@@ -116,7 +155,7 @@ int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, 
 // dsrc (optional): read the block from this float64 NB x NB array instead of A.
 template <typename TS>
 int launch_leaf(TS* A, TS* X, int64_t ld, int64_t o, double* leafinv, double* logdet, int* info,
-                const double* dsrc, hipStream_t s);
+                const double* dsrc, hipStream_t s, const Batch& bt = Batch());
 
 // k-range modes of the tile GEMM.
 enum KRange { KR_FULL = 0, KR_GE_N = 1, KR_LE_M = 2, KR_GE_M = 3 };
@@ -131,7 +170,7 @@ template <typename TA, typename TB, typename TC>
 int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const TB* B,
                 int64_t ldb, TC* C, int64_t ldc, int64_t M, int64_t N, int64_t K, double alpha,
                 bool beta_one, bool lower_tiles, int krange, hipStream_t s,
-                double* cdiag = nullptr);
+                double* cdiag = nullptr, const Batch& bt = Batch());
 
 // Fused LAUUM + adjoint: for every lower NB-tile of Sinv = X^T X (X = L^-1,
 // lower, Np x Np), reduce 1/2 tr(W dSigma/dtheta) contributions with
@@ -140,18 +179,18 @@ int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const
 template <typename TS>
 int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, const double* x,
                    int64_t n, int d, const double* theta, int nls, const double* alpha,
-                   double* partial, hipStream_t s);
+                   double* partial, hipStream_t s, const Batch& bt = Batch());
 
 // y = X r (trans = false) or y = X^T r (trans = true), X lower triangular
 // np x np (only k <= i is read); float64 accumulation through `partial`.
 template <typename TS>
 int launch_trmv(const TS* X, int64_t ld, int64_t np, const double* r, double* y, bool trans,
-                double* partial, hipStream_t s);
+                double* partial, hipStream_t s, const Batch& bt = Batch());
 
 // v = Sigma a, Sigma recomputed from x in float64 (identity pad for i >= n).
 int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
                         const double* theta, int nls, double jitter, const double* a, double* v,
-                        double* partial, hipStream_t s);
+                        double* partial, hipStream_t s, const Batch& bt = Batch());
 
 // y_j = sum_i M[i][j] v_i (v non-null) or sum_i M[i][j]^2 (v null), dense
 // rows x cols float64; partial holds ceil(rows/1024) x cols doubles.

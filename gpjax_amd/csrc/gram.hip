@@ -33,9 +33,16 @@ __global__ __launch_bounds__(256) void kmat_kernel(int kind, const double* __res
                                                    double jitter, int diag_mode, TS* out,
                                                    int64_t ldo, int64_t rows_total,
                                                    int64_t cols_total, int lower_only,
-                                                   double* __restrict__ dblk) {
+                                                   double* __restrict__ dblk, Batch bt) {
   __shared__ double xi_s[KT][DCH + 1];
   __shared__ double zj_s[DCH][KT + 1];
+  if (bt.gp) {  // batch member: its own inputs (a Gram matrix, z = x)
+    bmember(bt, d, x, n, theta);
+    z = x;
+    m = n;
+  }
+  out = bws(out, bt);
+  dblk = bws(dblk, bt);
 
   int64_t ti, tj;
   if (lower_only) {
@@ -121,7 +128,7 @@ template <typename TS>
 int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, int64_t m,
                          int d, const double* theta, int nls, double jitter, int diag_mode,
                          TS* out, int64_t ldo, int64_t rows_total, int64_t cols_total,
-                         bool lower_only, hipStream_t s, double* dblk) {
+                         bool lower_only, hipStream_t s, double* dblk, const Batch& bt) {
   if (rows_total <= 0 || cols_total <= 0) return GPK_OK;
   dim3 grid;
   if (lower_only) {
@@ -130,27 +137,29 @@ int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, 
       return GPK_ERR_BAD_ARG;
     }
     const int64_t nt = rows_total / NB;
-    grid = dim3((unsigned)(nt * (nt + 1) / 2), 4, 1);
+    grid = dim3((unsigned)(nt * (nt + 1) / 2), 4, bt.count);
   } else {
     const int64_t gy = (rows_total + KT - 1) / KT;
     if (gy > 65535) {
       set_error("kernel matrix: %lld rows exceed one launch", (long long)rows_total);
       return GPK_ERR_BAD_ARG;
     }
-    grid = dim3((unsigned)((cols_total + KT - 1) / KT), (unsigned)gy, 1);
+    grid = dim3((unsigned)((cols_total + KT - 1) / KT), (unsigned)gy, bt.count);
   }
   hipLaunchKernelGGL(kmat_kernel<TS>, grid, dim3(256), 0, s, kind, x, n, z, m, d, theta, nls,
                      jitter, diag_mode, out, ldo, rows_total, cols_total, lower_only ? 1 : 0,
-                     dblk);
+                     dblk, bt);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }
 
 template int launch_kernel_matrix<double>(int, const double*, int64_t, const double*, int64_t,
                                           int, const double*, int, double, int, double*,
-                                          int64_t, int64_t, int64_t, bool, hipStream_t, double*);
+                                          int64_t, int64_t, int64_t, bool, hipStream_t, double*,
+                                          const Batch&);
 template int launch_kernel_matrix<float>(int, const double*, int64_t, const double*, int64_t,
                                          int, const double*, int, double, int, float*, int64_t,
-                                         int64_t, int64_t, bool, hipStream_t, double*);
+                                         int64_t, int64_t, bool, hipStream_t, double*,
+                                         const Batch&);
 
 }  // namespace gpk

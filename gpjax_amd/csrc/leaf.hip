@@ -36,10 +36,17 @@ __global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_
                                                             double* __restrict__ leafinv,
                                                             double* __restrict__ logdet,
                                                             int* __restrict__ info,
-                                                            const double* __restrict__ dsrc) {
+                                                            const double* __restrict__ dsrc,
+                                                            Batch bt) {
   __shared__ double col_s[2][NB];
   __shared__ double dinv_s[NB];
   __shared__ double tr_s[NB][NB + 1];
+  A = bws(A, bt);
+  X = bws(X, bt);
+  leafinv = bws(leafinv, bt);
+  logdet = bws(logdet, bt);
+  info = bws(info, bt);
+  dsrc = bws(dsrc, bt);
 
   const int t = threadIdx.x;
   const int rb = (t / BPR) * EB;
@@ -139,16 +146,16 @@ __global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_
 
 template <typename TS>
 int launch_leaf(TS* A, TS* X, int64_t ld, int64_t o, double* leafinv, double* logdet, int* info,
-                const double* dsrc, hipStream_t s) {
-  hipLaunchKernelGGL(leaf_kernel<TS>, dim3(1), dim3(LEAF_THREADS), 0, s, A, X, ld, o, leafinv,
-                     logdet, info, dsrc);
+                const double* dsrc, hipStream_t s, const Batch& bt) {
+  hipLaunchKernelGGL(leaf_kernel<TS>, dim3(1, 1, bt.count), dim3(LEAF_THREADS), 0, s, A, X, ld,
+                     o, leafinv, logdet, info, dsrc, bt);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }
 
 template int launch_leaf<double>(double*, double*, int64_t, int64_t, double*, double*, int*,
-                                 const double*, hipStream_t);
+                                 const double*, hipStream_t, const Batch&);
 template int launch_leaf<float>(float*, float*, int64_t, int64_t, double*, double*, int*,
-                                const double*, hipStream_t);
+                                const double*, hipStream_t, const Batch&);
 
 }  // namespace gpk

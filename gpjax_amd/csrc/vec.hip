@@ -16,7 +16,10 @@ constexpr int TRMV_RC = 1024;  // rows per chunk, transposed
 template <typename TS>
 __global__ __launch_bounds__(256) void trmv_n_kernel(const TS* __restrict__ X, int64_t ld,
                                                      int64_t np, const double* __restrict__ r,
-                                                     double* __restrict__ partial) {
+                                                     double* __restrict__ partial, Batch bt) {
+  X = bws(X, bt);
+  r = bws(r, bt);
+  partial = bws(partial, bt);
   const int64_t c = blockIdx.y;
   const int64_t row0 = (int64_t)blockIdx.x * 16;
   const int64_t k0 = c * TRMV_CW;
@@ -38,7 +41,9 @@ __global__ __launch_bounds__(256) void trmv_n_kernel(const TS* __restrict__ X, i
 }
 
 __global__ void trmv_n_reduce(const double* __restrict__ partial, int64_t np,
-                              double* __restrict__ y) {
+                              double* __restrict__ y, Batch bt) {
+  partial = bws(partial, bt);
+  y = bws(y, bt);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const int64_t nc = i / TRMV_CW;
@@ -51,7 +56,10 @@ __global__ void trmv_n_reduce(const double* __restrict__ partial, int64_t np,
 template <typename TS>
 __global__ __launch_bounds__(256) void trmv_t_kernel(const TS* __restrict__ X, int64_t ld,
                                                      int64_t np, const double* __restrict__ r,
-                                                     double* __restrict__ partial) {
+                                                     double* __restrict__ partial, Batch bt) {
+  X = bws(X, bt);
+  r = bws(r, bt);
+  partial = bws(partial, bt);
   const int64_t c = blockIdx.y;
   const int64_t col0 = (int64_t)blockIdx.x * 256;
   const int64_t i0 = c * TRMV_RC;
@@ -68,7 +76,9 @@ __global__ __launch_bounds__(256) void trmv_t_kernel(const TS* __restrict__ X, i
 }
 
 __global__ void trmv_t_reduce(const double* __restrict__ partial, int64_t np, int64_t nchunks,
-                              double* __restrict__ y) {
+                              double* __restrict__ y, Batch bt) {
+  partial = bws(partial, bt);
+  y = bws(y, bt);
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= np) return;
   double s = 0.0;
@@ -86,9 +96,12 @@ __global__ __launch_bounds__(256) void sigma_mv_kernel(int kind, const double* _
                                                        const double* __restrict__ theta,
                                                        int nls, double jitter,
                                                        const double* __restrict__ a,
-                                                       double* __restrict__ partial) {
+                                                       double* __restrict__ partial, Batch bt) {
   __shared__ double xj_s[SMV_JT][DMAX + 1];
   __shared__ double aj_s[SMV_JT];
+  bmember(bt, d, x, n, theta);
+  a = bws(a, bt);
+  partial = bws(partial, bt);
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t c = blockIdx.y;
   const int64_t j0c = c * SMV_CW;
@@ -132,7 +145,9 @@ __global__ __launch_bounds__(256) void sigma_mv_kernel(int kind, const double* _
 }
 
 __global__ void sum_chunks(const double* __restrict__ partial, int64_t np, int64_t nchunks,
-                           double* __restrict__ y) {
+                           double* __restrict__ y, Batch bt) {
+  partial = bws(partial, bt);
+  y = bws(y, bt);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   double s = 0.0;
@@ -196,47 +211,46 @@ __global__ void mfma_selftest_kernel(double* out64, float* outbf) {
 
 template <typename TS>
 int launch_trmv(const TS* X, int64_t ld, int64_t np, const double* r, double* y, bool trans,
-                double* partial, hipStream_t s) {
+                double* partial, hipStream_t s, const Batch& bt) {
   if (np % 256) {
     set_error("trmv: size %lld not a multiple of 256", (long long)np);
     return GPK_ERR_BAD_ARG;
   }
+  const dim3 gvec((unsigned)((np + 255) / 256), 1, bt.count);
   if (!trans) {
     const int64_t nc = (np + TRMV_CW - 1) / TRMV_CW;
-    hipLaunchKernelGGL(trmv_n_kernel<TS>, dim3((unsigned)(np / 16), (unsigned)nc), dim3(256), 0,
-                       s, X, ld, np, r, partial);
+    hipLaunchKernelGGL(trmv_n_kernel<TS>, dim3((unsigned)(np / 16), (unsigned)nc, bt.count),
+                       dim3(256), 0, s, X, ld, np, r, partial, bt);
     GPK_LAUNCH_CHECK();
-    hipLaunchKernelGGL(trmv_n_reduce, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s,
-                       partial, np, y);
+    hipLaunchKernelGGL(trmv_n_reduce, gvec, dim3(256), 0, s, partial, np, y, bt);
   } else {
     const int64_t nc = (np + TRMV_RC - 1) / TRMV_RC;
-    hipLaunchKernelGGL(trmv_t_kernel<TS>, dim3((unsigned)(np / 256), (unsigned)nc), dim3(256), 0,
-                       s, X, ld, np, r, partial);
+    hipLaunchKernelGGL(trmv_t_kernel<TS>, dim3((unsigned)(np / 256), (unsigned)nc, bt.count),
+                       dim3(256), 0, s, X, ld, np, r, partial, bt);
     GPK_LAUNCH_CHECK();
-    hipLaunchKernelGGL(trmv_t_reduce, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s,
-                       partial, np, nc, y);
+    hipLaunchKernelGGL(trmv_t_reduce, gvec, dim3(256), 0, s, partial, np, nc, y, bt);
   }
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }
 
 template int launch_trmv<double>(const double*, int64_t, int64_t, const double*, double*, bool,
-                                 double*, hipStream_t);
+                                 double*, hipStream_t, const Batch&);
 template int launch_trmv<float>(const float*, int64_t, int64_t, const double*, double*, bool,
-                                double*, hipStream_t);
+                                double*, hipStream_t, const Batch&);
 
 int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
                         const double* theta, int nls, double jitter, const double* a, double* v,
-                        double* partial, hipStream_t s) {
+                        double* partial, hipStream_t s, const Batch& bt) {
   if (d < 1 || d > MAX_D) {
     set_error("sigma matvec: input dimension %d outside [1, %d]", d, MAX_D);
     return GPK_ERR_BAD_ARG;
   }
   const int64_t nc = (np + SMV_CW - 1) / SMV_CW;
-  dim3 grid((unsigned)((np + 255) / 256), (unsigned)nc);
+  dim3 grid((unsigned)((np + 255) / 256), (unsigned)nc, bt.count);
 #define GPK_SMV(DM)                                                                          \
   hipLaunchKernelGGL(sigma_mv_kernel<DM>, grid, dim3(256), 0, s, kind, x, n, np, d, theta, nls, \
-                     jitter, a, partial)
+                     jitter, a, partial, bt)
   if (d <= 1) GPK_SMV(1);
   else if (d <= 2) GPK_SMV(2);
   else if (d <= 4) GPK_SMV(4);
@@ -245,8 +259,8 @@ int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
   else GPK_SMV(32);
 #undef GPK_SMV
   GPK_LAUNCH_CHECK();
-  hipLaunchKernelGGL(sum_chunks, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, partial, np,
-                     nc, v);
+  hipLaunchKernelGGL(sum_chunks, dim3((unsigned)((np + 255) / 256), 1, bt.count), dim3(256), 0, s,
+                     partial, np, nc, v, bt);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }

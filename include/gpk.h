@@ -156,6 +156,43 @@ int gpk_predict(int kind, int prec, const double* x, const double* y, int64_t n,
                 double* mean, double* cov, void* stream);
 
 /*
+ * Many independent exact GPs in one call: `count` problems sharing the kernel
+ * kind, the input dimension d and the lengthscale layout nls, each with its own
+ * rows, targets and hyperparameters.  Each problem's MLL and gradient are those
+ * of gpk_mll_forward / gpk_mll_backward: conjugate_mll and its jax.grad
+ * (gpjax/objectives.py:36-107; tests/test_objectives.py:83-85) applied per
+ * problem, as a vmap over problems would.  float64.
+ *
+ * offsets_host: HOST array of count + 1 row offsets (offsets[0] = 0): problem p
+ *   owns rows offsets[p] .. offsets[p+1]-1 (at least one) of the concatenated
+ *   x [offsets[count], d] and y [offsets[count]] (device).
+ * theta: device [count][nls + 3], each row in gpk_mll_forward's layout.
+ * Problems are grouped by padded size; one group runs as one set of launches
+ * with one problem per blockIdx.z.
+ */
+size_t gpk_batched_workspace_bytes(const int64_t* offsets_host, int64_t count, int64_t d);
+
+/*
+ * Writes the batch's problem map into ws (synchronous host-to-device copies).
+ * Call once for a given (offsets, ws) before gpk_batched_mll_grad, outside any
+ * HIP-graph capture.
+ */
+int gpk_batched_plan(const int64_t* offsets_host, int64_t count, int64_t d, void* ws,
+                     size_t ws_bytes);
+
+/*
+ * MLL diagnostics and gradient of every problem: result [count][GPK_RESULT_LEN]
+ * (slots as gpk_mll_forward), grad [count][nls + 3] (d MLL / d theta in theta's
+ * layout), both device.  Launches only (stream-ordered, no host sync, no
+ * allocation), so it may be captured in a HIP graph once gpk_batched_plan has
+ * run on ws.  refine_iters < 0 selects the default (1).
+ */
+int gpk_batched_mll_grad(int kind, const double* x, const double* y, const int64_t* offsets_host,
+                         int64_t count, int64_t d, const double* theta, int nls, double jitter,
+                         int refine_iters, void* ws, size_t ws_bytes, double* result,
+                         double* grad, void* stream);
+
+/*
  * Products per element used by the GPK_PREC_MIXED stages (3 = bf16 hi/lo
  * split, 6 = hi/mid/lo split, 0 = float64 products of the float32 operands):
  * Cholesky updates, triangular inverse, and the

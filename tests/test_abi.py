@@ -75,7 +75,59 @@ def test_workspace_too_small_is_reported():
 def test_mixed_arith_option_validation():
     lib = _lib.lib()
     assert lib.gpk_set_mixed_arith(4, 3, 6) == 1
+    assert lib.gpk_set_mixed_arith(0, 3, 0) == 0   # float64 products of float32 operands
     assert lib.gpk_set_mixed_arith(6, 3, 6) == 0
+
+
+def _offsets(*rows):
+    arr = (ctypes.c_int64 * (len(rows) + 1))()
+    acc = 0
+    arr[0] = 0
+    for i, r in enumerate(rows):
+        acc += r
+        arr[i + 1] = acc
+    return arr
+
+
+def test_batched_workspace_groups_by_padded_size():
+    lib = _lib.lib()
+    d = 4
+    off = _offsets(10, 290, 10, 256)           # padded sizes 256, 512, 256, 256
+    got = lib.gpk_batched_workspace_bytes(off, 4, d)
+    per = {np_: lib.gpk_mll_workspace_bytes(np_, d, 0) for np_ in (256, 512)}
+    meta = 256 + 256                            # member map and row offsets, 256-byte aligned
+    assert got == 3 * per[256] + per[512] + meta
+
+
+@pytest.mark.parametrize("rows,count,d", [
+    ((5, 0, 7), 3, 2),       # a problem without rows
+    ((5, 7), 0, 2),          # no problems
+    ((5, 7), 2, 0),          # d < 1
+    ((5, 7), 2, 99),         # d above the fused-kernel limit
+])
+def test_batched_validation_without_device(rows, count, d):
+    lib = _lib.lib()
+    off = _offsets(*rows)
+    assert lib.gpk_batched_workspace_bytes(off, count, d) == 0
+    dummy = ctypes.c_void_p(0x1000)
+    rc = lib.gpk_batched_mll_grad(0, dummy, dummy, off, count, d, dummy, 1, 1e-6, -1, dummy,
+                                  1 << 40, dummy, dummy, None)
+    assert rc == 1
+    assert "batched" in lib.gpk_last_error().decode()
+
+
+def test_batched_workspace_and_offsets_checks():
+    lib = _lib.lib()
+    dummy = ctypes.c_void_p(0x1000)
+    off = _offsets(5, 7)
+    rc = lib.gpk_batched_plan(off, 2, 3, dummy, 64)
+    assert rc == 5 and "workspace" in lib.gpk_last_error().decode()
+    rc = lib.gpk_batched_mll_grad(0, dummy, dummy, off, 2, 3, dummy, 1, 1e-6, -1, dummy, 64,
+                                  dummy, dummy, None)
+    assert rc == 5
+    bad = _offsets(5, 7)
+    bad[0] = 1
+    assert lib.gpk_batched_plan(bad, 2, 3, dummy, 1 << 40) == 1
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):

@@ -1,0 +1,128 @@
+"""Many independent exact GPs in one call (``gpk_batched_mll_grad``).
+
+Each problem's MLL and gradient must equal what the single-problem path
+(``gpk_mll_forward`` / ``gpk_mll_backward``) gives for it -- the same kernels
+run with one problem per blockIdx.z, so agreement is to 1e-12 -- and the
+float64 oracle (``conjugate_mll_grad``: LU slogdet / solve and the closed-form
+gradient of ``gpjax/objectives.py:36-107``) to 1e-10 (MLL) and 1e-8 (gradient,
+norm-wise).  Sizes cover n = 1, ragged sizes around the 256-row padding
+boundaries and the largest problem of the many-small-GPs workload (1,024).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import gpjax_amd as gpx
+from gpjax_amd import workloads
+from oracle import gp_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [1, 5, 31, 128, 255, 256, 257, 400, 700, 1024]
+
+
+def _problems(sizes, d, seed):
+    rng = np.random.default_rng(seed)
+    Xs = [rng.uniform(0.0, 1.0, (n, d)) for n in sizes]
+    ys = [np.sin(6 * X[:, 0]) + X[:, -1] ** 2 + 0.05 * rng.standard_normal(len(X)) for X in Xs]
+    ls = np.exp(rng.uniform(np.log(0.2), np.log(1.0), (len(sizes), d)))
+    var = np.exp(rng.uniform(np.log(0.5), np.log(2.0), len(sizes)))
+    sd = np.exp(rng.uniform(np.log(0.03), np.log(0.3), len(sizes)))
+    return Xs, ys, ls, var, sd
+
+
+def _single(cuda, kind, X, y, ls, var, sd):
+    KC = {"rbf": gpx.kernels.RBF, "matern52": gpx.kernels.Matern52,
+          "matern32": gpx.kernels.Matern32, "matern12": gpx.kernels.Matern12}[kind]
+    k = KC(lengthscale=ls, variance=var)
+    post = gpx.gps.Prior(mean_function=gpx.mean_functions.Zero(), kernel=k, jitter=1e-6) * \
+        gpx.likelihoods.Gaussian(num_datapoints=len(X), obs_stddev=sd)
+    D = gpx.Dataset(X=torch.tensor(X, device=cuda), y=torch.tensor(y[:, None], device=cuda))
+    mll = gpx.objectives.conjugate_mll(post, D, precision="fp64")
+    mll.backward()
+    g = np.concatenate([np.atleast_1d(k.lengthscale.value.grad.cpu().numpy()),
+                        [float(k.variance.value.grad), float(post.likelihood.obs_stddev.value.grad)]])
+    return float(mll.detach()), g
+
+
+@pytest.mark.parametrize("kind,ard", [("rbf", True), ("matern52", False), ("matern12", True)])
+def test_batched_equals_single_problem_path(cuda, kind, ard):
+    d = 3
+    Xs, ys, ls, var, sd = _problems(SIZES, d, seed=11)
+    lsb = ls if ard else ls[:, 0]
+    m = gpx.batched.BatchedConjugateMLL(Xs, ys, kernel=kind, lengthscale=lsb, variance=var,
+                                        obs_stddev=sd, ard=ard)
+    mll = m.mll()
+    mll.sum().backward()
+    mll = mll.detach().cpu().numpy()
+    gls = m.lengthscale.value.grad.cpu().numpy()
+    gvar = m.variance.value.grad.cpu().numpy()
+    gsd = m.obs_stddev.value.grad.cpu().numpy()
+    info = m.last_result[:, 4].cpu().numpy()
+    assert np.all(info == 0)
+    for p, (X, y) in enumerate(zip(Xs, ys)):
+        ref_mll, ref_g = _single(cuda, kind, X, y, lsb[p], var[p], sd[p])
+        got_g = np.concatenate([np.atleast_1d(gls[p]), [gvar[p], gsd[p]]])
+        assert abs(mll[p] - ref_mll) <= 1e-12 * abs(ref_mll), (p, len(X))
+        assert np.linalg.norm(got_g - ref_g) <= 1e-12 * np.linalg.norm(ref_g), (p, len(X))
+
+
+def test_batched_against_oracle(cuda):
+    d = 4
+    sizes = [1, 17, 200, 300]
+    Xs, ys, ls, var, sd = _problems(sizes, d, seed=5)
+    m = gpx.batched.BatchedConjugateMLL(Xs, ys, kernel="rbf", lengthscale=ls, variance=var,
+                                        obs_stddev=sd)
+    mll = m.mll()
+    mll.sum().backward()
+    mll = mll.detach().cpu().numpy()
+    for p in range(len(sizes)):
+        ref = O.conjugate_mll_grad("rbf", Xs[p], ys[p], ls[p], var[p], sd[p], 1e-6)
+        assert abs(mll[p] - ref["mll"]) <= 1e-10 * abs(ref["mll"])
+        got = np.concatenate([m.lengthscale.value.grad[p].cpu().numpy(),
+                              [float(m.variance.value.grad[p]), float(m.obs_stddev.value.grad[p])]])
+        want = np.concatenate([ref["lengthscale"], [ref["variance"], ref["obs_stddev"]]])
+        assert np.linalg.norm(got - want) <= 1e-8 * np.linalg.norm(want), p
+
+
+def test_many_small_gps_workload_slice(cuda):
+    """512 problems of the many-small-GPs recipe (sizes 32..1024, D = 4, noise down
+    to 1e-3): every problem finite and factorised, sampled problems equal to the
+    single-problem path."""
+    w = workloads.batched(num=512)
+    m = gpx.batched.BatchedConjugateMLL(w["Xs"], w["ys"], kernel="rbf",
+                                        lengthscale=w["lengthscale"], variance=w["variance"],
+                                        obs_stddev=w["obs_stddev"], jitter=w["jitter"])
+    mll = m.mll()
+    mll.sum().backward()
+    assert torch.isfinite(mll).all()
+    assert torch.isfinite(m.lengthscale.value.grad).all()
+    assert bool((m.last_result[:, 4] == 0).all())
+    got = mll.detach().cpu().numpy()
+    for p in [0, 7, 100, 511]:
+        ref, _ = _single(cuda, "rbf", w["Xs"][p], w["ys"][p], w["lengthscale"][p],
+                         w["variance"][p], w["obs_stddev"][p])
+        assert abs(got[p] - ref) <= 1e-12 * abs(ref), p
+
+
+def test_batched_fit_graph_replay_matches_eager(cuda):
+    Xs, ys, ls, var, sd = _problems([40, 300, 600, 90], 2, seed=3)
+
+    def model():
+        return gpx.batched.BatchedConjugateMLL(Xs, ys, kernel="rbf", lengthscale=0.5,
+                                               variance=1.0, obs_stddev=0.2)
+
+    def objective(mdl, data):
+        return mdl.mll().sum()
+
+    m1, h1 = gpx.fit(model=model(), objective=objective, train_data=None, num_iters=10,
+                     learning_rate=0.05)
+    torch.cuda.synchronize()
+    m2, h2 = gpx.fit(model=model(), objective=objective, train_data=None, num_iters=10,
+                     learning_rate=0.05, capture_graph=True, warmup=3)
+    torch.cuda.synchronize()
+    assert torch.allclose(h1.cpu(), h2.cpu(), rtol=1e-12, atol=0.0)
+    assert float(h1[-1]) < float(h1[0])
+    assert torch.allclose(m1.lengthscale.value.detach(), m2.lengthscale.value.detach(),
+                          rtol=1e-12)
