@@ -84,16 +84,130 @@ def grads(params):
     return torch.cat([p.value.grad.reshape(-1) for p in params]).double().cpu()
 
 
-def cpu_baseline(n: int) -> dict:
+def cpu_baseline(n: int, workload: str = "exact_large") -> dict:
     threads = os.cpu_count() or 1
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OPENBLAS_NUM_THREADS=str(threads),
                MKL_NUM_THREADS=str(threads))
     cmd = [sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"), "--n", str(n),
-           "--threads", str(threads)]
+           "--threads", str(threads), "--workload", workload]
     proc = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
     if proc.returncode != 0:
         return {"value": None, "error": proc.stderr[-500:]}
     return json.loads(proc.stdout.strip().splitlines()[-1])
+
+
+def timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, out
+
+
+def other_workload(args) -> dict:
+    """The other configurations, one JSON line each (not the headline):
+    matern  -- exact MLL + gradient, N = 16,384, D = 8, Matern-5/2 ARD, mixed and FP64;
+    batched -- 4,096 independent exact GPs (N_i in 32..1024, D = 4, RBF ARD), MLL +
+               gradient of all of them per step;
+    sparse  -- collapsed ELBO + gradient (theta and the 2,048 x 8 inducing inputs),
+               N = 1e6, D = 8, Matern-3/2 ARD, FP64."""
+    import torch
+
+    import gpjax_amd as gpx
+    from gpjax_amd import workloads
+    dev = torch.device("cuda:0")
+    line = {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "data": "synthetic"}
+    if args.workload == "batched":
+        w = workloads.batched()
+        m = gpx.batched.BatchedConjugateMLL(w["Xs"], w["ys"], kernel="rbf",
+                                            lengthscale=w["lengthscale"], variance=w["variance"],
+                                            obs_stddev=w["obs_stddev"], jitter=w["jitter"],
+                                            device=dev)
+        params = [m.lengthscale, m.variance, m.obs_stddev]
+
+        def step():
+            for p in params:
+                p.value.grad = None
+            mll = m.mll()
+            mll.sum().backward()
+            return mll
+
+        t, mll = timed(step, args.steps, args.warmup)
+        sizes = [int(s) for s in w["sizes"]]
+        line.update(metric="batched MLL+grad evals/s (4096 exact GPs, N_i in 32..1024, D=4)",
+                    value=round(1.0 / t, 4), unit="evals/s", ms_per_step=round(t * 1e3, 2),
+                    dtype="f64", config={"workload": "batched: 4096 RBF-ARD GPs "
+                                         "(gpjax_amd.workloads.batched)",
+                                         "problems": len(sizes), "sum_n": sum(sizes),
+                                         "sum_n3": float(sum(s ** 3 for s in sizes))},
+                    all_finite=bool(torch.isfinite(mll).all()))
+        cpu_n = 32
+    elif args.workload == "sparse":
+        w = workloads.sparse()
+        k = gpx.kernels.Matern32(lengthscale=w["lengthscale"], variance=w["variance"])
+        post = gpx.gps.Prior(mean_function=gpx.mean_functions.Zero(), kernel=k) * \
+            gpx.likelihoods.Gaussian(num_datapoints=len(w["X"]), obs_stddev=w["obs_stddev"])
+        q = gpx.variational_families.CollapsedVariationalGaussian(posterior=post,
+                                                                 inducing_inputs=w["Z"])
+        D = gpx.Dataset(X=torch.tensor(w["X"], device=dev),
+                        y=torch.tensor(w["y"][:, None], device=dev))
+        params = [k.lengthscale, k.variance, post.likelihood.obs_stddev, q.inducing_inputs]
+
+        def step():
+            for p in params:
+                p.value.grad = None
+            e = gpx.objectives.collapsed_elbo(q, D)
+            e.backward()
+            return e
+
+        t, e = timed(step, args.steps, args.warmup)
+        line.update(metric="collapsed ELBO+grad evals/s at N=1e6, M=2048", value=round(1.0 / t, 4),
+                    unit="evals/s", ms_per_step=round(t * 1e3, 2), dtype="f64",
+                    config={"workload": "sparse: N=1e6, D=8, M=2048, Matern-3/2 ARD "
+                                        "(gpjax_amd.workloads.sparse)", "n": len(w["X"]),
+                            "m": 2048, "d": 8}, elbo=float(e.detach()))
+        cpu_n = 16384
+    else:  # matern
+        w = workloads.exact_matern()
+        n = len(w["X"])
+        k = gpx.kernels.Matern52(lengthscale=w["lengthscale"], variance=w["variance"])
+        post = gpx.gps.Prior(mean_function=gpx.mean_functions.Zero(), kernel=k,
+                             jitter=w["jitter"]) * \
+            gpx.likelihoods.Gaussian(num_datapoints=n, obs_stddev=w["obs_stddev"])
+        D = gpx.Dataset(X=torch.tensor(w["X"], device=dev),
+                        y=torch.tensor(w["y"][:, None], device=dev))
+        params = [k.lengthscale, k.variance, post.likelihood.obs_stddev]
+
+        def make(prec):
+            def step():
+                for p in params:
+                    p.value.grad = None
+                mll = gpx.objectives.conjugate_mll(post, D, precision=prec)
+                mll.backward()
+                return float(mll.detach()), grads(params)
+            return step
+
+        t_m, (mll_m, g_m) = timed(make("mixed"), args.steps, args.warmup)
+        t_d, (mll_d, g_d) = timed(make("fp64"), args.steps, args.warmup)
+        rel_g = float(torch.linalg.norm(g_m - g_d) / torch.linalg.norm(g_d))
+        line.update(metric="MLL+grad evals/s at N=16,384 (Matern-5/2 ARD, D=8), mixed",
+                    value=round(1.0 / t_m, 4), unit="evals/s", ms_per_step=round(t_m * 1e3, 2),
+                    dtype="bf16x6-split/f32 with f64 panels",
+                    config={"workload": "exact GP MLL+grad, N=16384 D=8 Matern-5/2 ARD "
+                                        "(gpjax_amd.workloads.exact_matern)", "n": n, "d": 8},
+                    fp64_gpu={"evals_per_s": round(1.0 / t_d, 4),
+                              "ms_per_step": round(t_d * 1e3, 2)},
+                    speedup_vs_fp64=round(t_d / t_m, 3),
+                    accuracy={"rel_mll_vs_fp64": abs(mll_m - mll_d) / abs(mll_d),
+                              "rel_grad_vs_fp64": rel_g})
+        cpu_n = 4096
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(cpu_n, args.workload)
+    return line
 
 
 def main() -> int:
@@ -113,11 +227,17 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES bytes 5 times "
                          "(reference for rocprof FETCH_SIZE / WRITE_SIZE scaling) and exit")
+    ap.add_argument("--workload", default="exact_large",
+                    choices=["exact_large", "matern", "batched", "sparse"],
+                    help="exact_large (default) is the headline; the others print their own line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no "
               "multi-GPU path", file=sys.stderr)
         return 2
+    if args.workload != "exact_large":
+        print(json.dumps(other_workload(args)), flush=True)
+        return 0
     if args.copy_calibration:
         import torch
         src = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device="cuda:0")

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <type_traits>
 
 #include "../../include/gpk.h"
 
@@ -59,11 +60,14 @@ struct Batch {
   unsigned count = 1;
 };
 
+// The shift is byte arithmetic on the pointer itself (not through an integer),
+// so the compiler keeps the global address space and emits global (not flat)
+// loads and stores.
 template <typename T>
 __device__ __forceinline__ T* bws(T* p, const Batch& b) {
   if (!p || !b.ws_stride) return p;
-  return reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) +
-                              (uintptr_t)((int64_t)blockIdx.z * b.ws_stride));
+  typedef typename std::conditional<std::is_const<T>::value, const char, char>::type Byte;
+  return reinterpret_cast<T*>(reinterpret_cast<Byte*>(p) + (int64_t)blockIdx.z * b.ws_stride);
 }
 // Problem index of this member (0 for a single problem).
 __device__ __forceinline__ int64_t bprob(const Batch& b) {

@@ -22,13 +22,71 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 
 
+def _median_time(fn, runs):
+    fn()
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times)
+
+
+def other_workload(a) -> dict:
+    """Bounded samples of the other configurations, extrapolated to full size:
+    matern (N = 16,384, D = 8) by N^3; batched (4,096 GPs) by the sum of N_i^3
+    over the problems timed against the whole batch; sparse (N = 1e6, M = 2,048)
+    linearly in N (its cost is M^2 N + O(M^3))."""
+    import numpy as np
+
+    from gpjax_amd import workloads
+    from oracle import gp_oracle as O
+
+    if a.workload == "matern":
+        w = workloads.exact_matern(n=a.n)
+        t = _median_time(lambda: O.conjugate_mll_grad(
+            w["kind"], w["X"], w["y"], w["lengthscale"], w["variance"], w["obs_stddev"],
+            w["jitter"]), a.runs)
+        scale = (a.n / 16384) ** 3
+        sample = f"float64 oracle MLL + gradient on exact_matern with N={a.n}; rate scaled to N=16384 by N^3"
+    elif a.workload == "batched":
+        w = workloads.batched()
+        k = a.n  # number of problems timed
+
+        def run():
+            for p in range(k):
+                O.conjugate_mll_grad("rbf", w["Xs"][p], w["ys"][p], w["lengthscale"][p],
+                                     w["variance"][p], w["obs_stddev"][p], w["jitter"])
+        t = _median_time(run, a.runs)
+        sizes = np.asarray(w["sizes"], dtype=np.float64)
+        scale = float((sizes[:k] ** 3).sum() / (sizes ** 3).sum())
+        sample = (f"float64 oracle MLL + gradient of the first {k} of the 4096 problems; rate "
+                  "scaled to the whole batch by sum(N_i^3)")
+    else:  # sparse
+        w = workloads.sparse(n=a.n, m=2048)
+        t = _median_time(lambda: O.collapsed_elbo_grad(
+            w["kind"], w["X"], w["y"], w["Z"], w["lengthscale"], w["variance"],
+            w["obs_stddev"], w["jitter"]), a.runs)
+        scale = a.n / 1_000_000
+        sample = (f"float64 oracle collapsed ELBO + gradient (theta and Z) with N={a.n}, M=2048; "
+                  "rate scaled to N=1e6 linearly")
+    return {"value": 1.0 / t * scale, "unit": "evals/s", "cores": a.threads, "kind": "port",
+            "sample": f"{sample}; median {t:.3f}s of {a.runs} runs after 1 warm-up, "
+                      f"{a.threads} threads", "sample_seconds_per_eval": t}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=6144)
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
     ap.add_argument("--runs", type=int, default=3)
     ap.add_argument("--target-n", type=int, default=65536)
+    ap.add_argument("--workload", default="exact_large",
+                    choices=["exact_large", "matern", "batched", "sparse"])
     a = ap.parse_args()
+    if a.workload != "exact_large":
+        print(json.dumps(other_workload(a)))
+        return
 
     import numpy as np  # noqa: F401  (BLAS threads fixed by the caller's environment)
 

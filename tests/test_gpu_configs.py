@@ -1,0 +1,123 @@
+"""The benchmark configurations other than the headline, at their full sizes.
+
+* C1 -- N = 2,000, D = 1, RBF (``workloads.exact_small``): FP64 MLL and gradient
+  against the float64 oracle (1e-10 / 1e-8).
+* C2 -- N = 16,384, D = 8, Matern-5/2 ARD (``workloads.exact_matern``): the FP64
+  path against the oracle on the same recipe at N = 2,048, and at the full
+  N = 16,384 the mixed path against the FP64 one under the headline gates
+  (|dMLL|/|MLL| <= 1e-6, gradient <= 1e-4 relative).
+* C4 -- N = 1e6, M = 2,048, D = 8, Matern-3/2 ARD (``workloads.sparse``): no
+  oracle finishes at this size, so size-independent properties: the ELBO is a
+  sum over data points (permuting the rows leaves value and gradient unchanged
+  up to summation order, 1e-9), and its analytic gradient matches central
+  differences of the device ELBO in the noise scale and the signal variance
+  (step 1e-4 relative, 1e-6 relative agreement).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import gpjax_amd as gpx
+from gpjax_amd import workloads
+from oracle import gp_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _exact(cuda, w, kcls):
+    k = kcls(lengthscale=w["lengthscale"], variance=w["variance"])
+    post = gpx.gps.Prior(mean_function=gpx.mean_functions.Zero(), kernel=k,
+                         jitter=w["jitter"]) * \
+        gpx.likelihoods.Gaussian(num_datapoints=len(w["X"]), obs_stddev=w["obs_stddev"])
+    D = gpx.Dataset(X=torch.tensor(w["X"], device=cuda),
+                    y=torch.tensor(w["y"][:, None], device=cuda))
+    return post, D, k
+
+
+def _mll_grad(post, D, k, precision):
+    for p in (k.lengthscale, k.variance, post.likelihood.obs_stddev):
+        p.value.grad = None
+    mll = gpx.objectives.conjugate_mll(post, D, precision=precision)
+    mll.backward()
+    g = np.concatenate([np.atleast_1d(k.lengthscale.value.grad.cpu().numpy()),
+                        [float(k.variance.value.grad), float(post.likelihood.obs_stddev.value.grad)]])
+    return float(mll.detach()), g
+
+
+def _oracle(w, kind):
+    r = O.conjugate_mll_grad(kind, w["X"], w["y"], w["lengthscale"], w["variance"],
+                             w["obs_stddev"], w["jitter"])
+    return r["mll"], np.concatenate([np.atleast_1d(r["lengthscale"]),
+                                     [r["variance"], r["obs_stddev"]]])
+
+
+def test_c1_exact_small_fp64(cuda):
+    w = workloads.exact_small()
+    post, D, k = _exact(cuda, w, gpx.kernels.RBF)
+    mll, g = _mll_grad(post, D, k, "fp64")
+    ref_mll, ref_g = _oracle(w, "rbf")
+    assert abs(mll - ref_mll) <= 1e-10 * abs(ref_mll)
+    assert np.linalg.norm(g - ref_g) <= 1e-8 * np.linalg.norm(ref_g)
+
+
+def test_c2_recipe_fp64_against_oracle(cuda):
+    w = workloads.exact_matern(n=2048)
+    post, D, k = _exact(cuda, w, gpx.kernels.Matern52)
+    mll, g = _mll_grad(post, D, k, "fp64")
+    ref_mll, ref_g = _oracle(w, "matern52")
+    assert abs(mll - ref_mll) <= 1e-10 * abs(ref_mll)
+    assert np.linalg.norm(g - ref_g) <= 1e-8 * np.linalg.norm(ref_g)
+
+
+def test_c2_full_size_mixed_within_gates(cuda):
+    w = workloads.exact_matern()
+    post, D, k = _exact(cuda, w, gpx.kernels.Matern52)
+    mll_d, g_d = _mll_grad(post, D, k, "fp64")
+    mll_m, g_m = _mll_grad(post, D, k, "mixed")
+    rel_mll = abs(mll_m - mll_d) / abs(mll_d)
+    rel_g = np.linalg.norm(g_m - g_d) / np.linalg.norm(g_d)
+    print(f"C2 N=16384 mixed vs fp64: rel_mll={rel_mll:.2e} rel_grad={rel_g:.2e}")
+    assert rel_mll <= 1e-6
+    assert rel_g <= 1e-4
+
+
+def _sparse_model(cuda, w, X, y, sd=None, var=None):
+    k = gpx.kernels.Matern32(lengthscale=w["lengthscale"],
+                             variance=w["variance"] if var is None else var)
+    post = gpx.gps.Prior(mean_function=gpx.mean_functions.Zero(), kernel=k) * \
+        gpx.likelihoods.Gaussian(num_datapoints=len(X),
+                                 obs_stddev=w["obs_stddev"] if sd is None else sd)
+    q = gpx.variational_families.CollapsedVariationalGaussian(posterior=post,
+                                                             inducing_inputs=w["Z"])
+    D = gpx.Dataset(X=torch.tensor(X, device=cuda), y=torch.tensor(y[:, None], device=cuda))
+    return q, D, k, post
+
+
+def _elbo(cuda, w, X, y, **kw):
+    q, D, k, post = _sparse_model(cuda, w, X, y, **kw)
+    e = gpx.objectives.collapsed_elbo(q, D)
+    e.backward()
+    g = np.concatenate([k.lengthscale.value.grad.cpu().numpy(),
+                        [float(k.variance.value.grad), float(post.likelihood.obs_stddev.value.grad)]])
+    gz = q.inducing_inputs.value.grad.cpu().numpy()
+    return float(e.detach()), g, gz
+
+
+def test_c4_full_size_elbo_properties(cuda):
+    w = workloads.sparse()
+    X, y = w["X"], w["y"]
+    e, g, gz = _elbo(cuda, w, X, y)
+    assert np.isfinite(e) and np.all(np.isfinite(g)) and np.all(np.isfinite(gz))
+    perm = np.random.default_rng(0).permutation(len(X))
+    e_p, g_p, gz_p = _elbo(cuda, w, X[perm], y[perm])
+    assert abs(e_p - e) <= 1e-9 * abs(e)
+    assert np.linalg.norm(g_p - g) <= 1e-9 * np.linalg.norm(g)
+    assert np.max(np.abs(gz_p - gz)) <= 1e-9 * np.max(np.abs(gz))
+    # central differences of the device ELBO: noise scale and signal variance
+    for name, base, idx in [("sd", w["obs_stddev"], -1), ("var", w["variance"], -2)]:
+        h = 1e-4 * base
+        ep = _elbo(cuda, w, X, y, **{name: base + h})[0]
+        em = _elbo(cuda, w, X, y, **{name: base - h})[0]
+        fd = (ep - em) / (2 * h)
+        assert abs(fd - g[idx]) <= 1e-6 * abs(g[idx]) + 1e-9 * abs(e), (name, fd, g[idx])
