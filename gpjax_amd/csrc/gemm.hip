@@ -32,8 +32,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
   p.C = bws(p.C, p.bt);
   p.cdiag = bws(p.cdiag, p.bt);
   int64_t mt, nt;
-  if (p.lower_tiles) lower_tile(blockIdx.x, mt, nt);
-  else { nt = blockIdx.x; mt = blockIdx.y; }
+  if (p.lower_tiles) {
+    lower_tile(blockIdx.x, mt, nt);
+  } else {
+    nt = blockIdx.x;
+    // k <= row tile: the last tile rows carry the most work, so dispatch them first.
+    mt = p.krange == KR_LE_M ? gridDim.y - 1 - blockIdx.y : blockIdx.y;
+  }
   int64_t kb, ke;
   k_range(p.krange, mt, nt, p.K, kb, ke);
   const int64_t m0 = mt * TILE, n0 = nt * TILE;

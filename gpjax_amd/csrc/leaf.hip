@@ -40,6 +40,7 @@ __global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_
                                                             Batch bt) {
   __shared__ double col_s[2][NB];
   __shared__ double dinv_s[NB];
+  __shared__ double dk_s[NB];
   __shared__ double tr_s[NB][NB + 1];
   A = bws(A, bt);
   X = bws(X, bt);
@@ -68,16 +69,16 @@ __global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_
   }
   __syncthreads();
 
-  double ld_acc = 0.0;
   int bad = -1;
   for (int k = 0; k < NB; ++k) {
     const double* col = col_s[k & 1];
     const double piv = col[k];
     const double dk = sqrt(piv);
     const double inv = 1.0 / dk;
+    // The logarithms are taken after the sweep, off the per-column critical path.
     if (t == 0) {
       if (bad < 0 && !(piv > 0.0)) bad = k;
-      ld_acc += log(dk);
+      dk_s[k] = dk;
       dinv_s[k] = inv;
     }
     // Every update at step k touches column k or a column > k.
@@ -129,6 +130,7 @@ __global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_
       if (j > i) tr_s[j][i] = s[a][b];
     }
   }
+  if (t < NB) dk_s[t] = log(dk_s[t]);
   __syncthreads();
   for (int e = t; e < NB * NB; e += LEAF_THREADS) {
     const int i = e / NB, j = e % NB;
@@ -137,6 +139,8 @@ __global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_
     leafinv[e] = v;
   }
   if (t == 0) {
+    double ld_acc = 0.0;  // in column order, as the sweep produced the pivots
+    for (int k = 0; k < NB; ++k) ld_acc += dk_s[k];
     *logdet = 2.0 * ld_acc;
     *info = bad >= 0 ? (int)(o + bad + 1) : 0;
   }
