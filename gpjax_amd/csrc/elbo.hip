@@ -838,6 +838,100 @@ int check_elbo(int kind, int prec, const double* x, const double* y, int64_t n, 
   return GPK_OK;
 }
 
+// ---- predictive of the collapsed posterior (variational_families.py:786-870) ----
+// With C = Kzz + Kzx Kxz / s = Lz (I + A A^T) Lz^T (A = Lz^-1 Kzx / sd, Kzz with
+// jitter), the reference's quantities are
+//   mean = c + Ktz C^-1 Kzx (y - c) / s                      (:840-856)
+//   cov  = Ktt - |Lz^-1 Kzt|^2 + |L_C^-1 Kzt|^2 + jitter I   (:850-866)
+// where |M|^2 = M^T M and L_C^-1 = L^-1 Lz^-1 (L = chol(I + A A^T)).  The
+// forward leaves X_K = Lz^-1, X_C = L_C^-1 and C^-1 b (b = Kzx (y - c)) in the
+// workspace; Kzt is formed once, zero-padded to mp x tp.
+struct ElboPredLayout {
+  int64_t tp = 0;
+  size_t off_kzt = 0, off_a1 = 0, off_a2 = 0, off_ktt = 0, off_part = 0, off_v1 = 0, off_v2 = 0;
+  size_t off_res = 0, total = 0;
+};
+
+ElboPredLayout make_elbo_pred_layout(const ElboLayout& Lo, int64_t t, int full_cov) {
+  ElboPredLayout P;
+  P.tp = round_up(t, NB);
+  const int64_t mp = Lo.mp, tp = P.tp;
+  size_t o = Lo.total;
+  P.off_kzt = o; o += al((size_t)mp * tp * sizeof(double));
+  P.off_a1 = o; o += al((size_t)mp * tp * sizeof(double));
+  P.off_a2 = o; o += al((size_t)mp * tp * sizeof(double));
+  P.off_ktt = o; o += al((size_t)(full_cov ? tp * tp : 1) * sizeof(double));
+  P.off_part = o; o += al((size_t)((mp + 1023) / 1024) * tp * sizeof(double));
+  P.off_v1 = o; o += al((size_t)tp * sizeof(double));
+  P.off_v2 = o; o += al((size_t)tp * sizeof(double));
+  P.off_res = o; o += al(GPK_RESULT_LEN * sizeof(double));
+  P.total = o;
+  return P;
+}
+
+__global__ void sgpr_mean_kernel(const double* __restrict__ v, const double* __restrict__ theta,
+                                 int nls, int64_t t, double* __restrict__ mean) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double sd = theta[nls + 1];
+  if (j < t) mean[j] = theta[nls + 2] + v[j] / (sd * sd);
+}
+
+// var_j = k(t_j, t_j) + jitter - |Lz^-1 Kzt|^2_j + |L_C^-1 Kzt|^2_j; k(0) = variance.
+__global__ void sgpr_var_kernel(const double* __restrict__ v1, const double* __restrict__ v2,
+                                const double* __restrict__ theta, int nls, double jitter,
+                                int64_t t, double* __restrict__ var_out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < t) var_out[j] = (theta[nls] + jitter) - v1[j] + v2[j];
+}
+
+int elbo_predict_impl(int kind, const double* x, const double* y, int64_t n, const double* z,
+                      int64_t m, int d, const double* xt, int64_t t, const double* theta, int nls,
+                      double jitter, int full_cov, void* ws, const ElboLayout& Lo,
+                      const ElboPredLayout& P, double* mean, double* cov, hipStream_t s) {
+  const int64_t mp = Lo.mp, tp = P.tp;
+  GPK_TRY(elbo_forward_impl(kind, x, y, n, z, m, d, theta, nls, jitter, ws, Lo,
+                            at<double>(ws, P.off_res), s));
+  double* kzt = at<double>(ws, P.off_kzt);
+  double* a1 = at<double>(ws, P.off_a1);
+  double* a2 = at<double>(ws, P.off_a2);
+  double* part = at<double>(ws, P.off_part);
+  double* v1 = at<double>(ws, P.off_v1);
+  double* v2 = at<double>(ws, P.off_v2);
+  const double* XK = at<double>(ws, Lo.off_XK);
+  const double* XC = at<double>(ws, Lo.off_XC);
+  GPK_TRY(launch_kernel_matrix<double>(kind, z, m, xt, t, d, theta, nls, 0.0, KM_ZERO_PAD, kzt,
+                                       tp, mp, tp, false, s));
+  // mean: c + Kzt^T (C^-1 b) / s
+  GPK_TRY(launch_colreduce(kzt, tp, mp, tp, at<double>(ws, Lo.off_c), v1, part, s));
+  hipLaunchKernelGGL(sgpr_mean_kernel, dim3(cdiv(t, 256)), dim3(256), 0, s, v1, theta, nls, t,
+                     mean);
+  GPK_LAUNCH_CHECK();
+  // A1 = Lz^-1 Kzt, A2 = L_C^-1 Kzt (lower-triangular left factors: k <= row tile)
+  GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, true, XK, mp, kzt, tp, a1, tp, mp,
+                                              tp, mp, 1.0, false, false, KR_LE_M, s));
+  GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, true, XC, mp, kzt, tp, a2, tp, mp,
+                                              tp, mp, 1.0, false, false, KR_LE_M, s));
+  if (full_cov) {
+    double* ktt = at<double>(ws, P.off_ktt);
+    GPK_TRY(launch_kernel_matrix<double>(kind, xt, t, xt, t, d, theta, nls, jitter,
+                                         1 | KM_ZERO_PAD, ktt, tp, tp, tp, false, s));
+    GPK_TRY(launch_gemm<double, double, double>(AR_F64, true, true, a1, tp, a1, tp, ktt, tp, tp,
+                                                tp, mp, -1.0, true, false, KR_FULL, s));
+    GPK_TRY(launch_gemm<double, double, double>(AR_F64, true, true, a2, tp, a2, tp, ktt, tp, tp,
+                                                tp, mp, 1.0, true, false, KR_FULL, s));
+    GPK_HIP_TRY(hipMemcpy2DAsync(cov, (size_t)t * sizeof(double), ktt, (size_t)tp * sizeof(double),
+                                 (size_t)t * sizeof(double), (size_t)t, hipMemcpyDeviceToDevice,
+                                 s));
+  } else {
+    GPK_TRY(launch_colreduce(a1, tp, mp, tp, nullptr, v1, part, s));
+    GPK_TRY(launch_colreduce(a2, tp, mp, tp, nullptr, v2, part, s));
+    hipLaunchKernelGGL(sgpr_var_kernel, dim3(cdiv(t, 256)), dim3(256), 0, s, v1, v2, theta, nls,
+                       jitter, t, cov);
+    GPK_LAUNCH_CHECK();
+  }
+  return GPK_OK;
+}
+
 }  // namespace
 }  // namespace gpk
 
@@ -891,6 +985,37 @@ int gpk_elbo_backward(int kind, int prec, const double* x, const double* y, int6
   }
   return elbo_backward_impl(kind, x, y, n, z, m, (int)d, theta, nls, gbar_host, ws, Lo, grad,
                             grad_z, (hipStream_t)stream);
+}
+
+size_t gpk_elbo_predict_workspace_bytes(int64_t n, int64_t m, int64_t d, int64_t t,
+                                        int full_cov) {
+  (void)d;
+  if (n < 1 || m < 1 || t < 1) return 0;
+  return make_elbo_pred_layout(make_elbo_layout(n, m), t, full_cov).total;
+}
+
+int gpk_elbo_predict(int kind, int prec, const double* x, const double* y, int64_t n,
+                     const double* z, int64_t m, int64_t d, const double* xt, int64_t t,
+                     const double* theta, int nls, double jitter, int full_cov, void* ws,
+                     size_t ws_bytes, double* mean, double* cov, void* stream) {
+  int rc = check_elbo(kind, prec, x, y, n, z, m, d, theta, nls);
+  if (rc) return rc;
+  if (!xt || t < 1 || !ws || !mean || !cov) {
+    set_error("collapsed predict: null xt/ws/mean/cov or t < 1");
+    return GPK_ERR_BAD_ARG;
+  }
+  const ElboLayout Lo = make_elbo_layout(n, m);
+  const ElboPredLayout P = make_elbo_pred_layout(Lo, t, full_cov);
+  if (ws_bytes < P.total) {
+    set_error("collapsed predict: workspace %zu bytes < required %zu", ws_bytes, P.total);
+    return GPK_ERR_WORKSPACE;
+  }
+  if (reinterpret_cast<uintptr_t>(ws) % 256) {
+    set_error("collapsed predict: workspace must be 256-byte aligned");
+    return GPK_ERR_BAD_ARG;
+  }
+  return elbo_predict_impl(kind, x, y, n, z, m, (int)d, xt, t, theta, nls, jitter, full_cov, ws,
+                           Lo, P, mean, cov, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -138,6 +138,24 @@ int gpk_elbo_backward(int kind, int prec, const double* x, const double* y, int6
                       double gbar_host, void* ws, size_t ws_bytes, double* grad, double* grad_z,
                       void* stream);
 
+/* Bytes of workspace gpk_elbo_predict needs for (n, m, d, t test points, full_cov). */
+size_t gpk_elbo_predict_workspace_bytes(int64_t n, int64_t m, int64_t d, int64_t t,
+                                        int full_cov);
+
+/*
+ * Predictive of the collapsed posterior at t test inputs xt [t, d]
+ * (CollapsedVariationalGaussian.predict, gpjax/variational_families.py:786-870):
+ *   mean = c + Kzt^T C^-1 Kzx (y - c) / s,   C = Kzz + jitter I + Kzx Kxz / s,
+ *   cov  = Ktt - (Lz^-1 Kzt)^T (Lz^-1 Kzt) + (L_C^-1 Kzt)^T (L_C^-1 Kzt) + jitter I,
+ * s = obs_sd^2, Lz = chol(Kzz + jitter I), L_C = chol(C) = Lz chol(I + A A^T).
+ * Runs the ELBO forward first (same statistics, no n x m matrix).  mean [t];
+ * cov [t, t] (full_cov) or its diagonal [t].  Float64 throughout.
+ */
+int gpk_elbo_predict(int kind, int prec, const double* x, const double* y, int64_t n,
+                     const double* z, int64_t m, int64_t d, const double* xt, int64_t t,
+                     const double* theta, int nls, double jitter, int full_cov, void* ws,
+                     size_t ws_bytes, double* mean, double* cov, void* stream);
+
 /* Bytes of workspace gpk_predict needs for (n, d, t test points, prec, full_cov). */
 size_t gpk_predict_workspace_bytes(int64_t n, int64_t d, int64_t t, int prec, int full_cov);
 

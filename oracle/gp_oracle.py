@@ -339,6 +339,85 @@ def predict(kind, X, y, T, ls, var, obs_sd, jitter_post=1e-6, jitter_prior=1e-6,
     return mu, cov
 
 
+def collapsed_predict(kind, X, y, Z, T, ls, var, obs_sd, jitter=1e-6, mean=0.0):
+    """``CollapsedVariationalGaussian.predict`` (``variational_families.py:786-870``):
+    latent mean [T] and covariance [T, T] of the collapsed (Titsias) posterior."""
+    X = np.asarray(X, np.float64)
+    Z = np.asarray(Z, np.float64)
+    T = np.asarray(T, np.float64)
+    m = Z.shape[0]
+    noise_var = float(obs_sd) ** 2                                # :807
+    Kzx = cross_covariance(kind, Z, X, ls, var)                   # :815
+    Kzz = add_jitter(gram(kind, Z, ls, var), jitter)              # :816-818
+    Lz = np.linalg.cholesky(Kzz)                                  # :822
+    Lz_inv_Kzx = sla.solve_triangular(Lz, Kzx, lower=True)        # :825
+    A = Lz_inv_Kzx / float(obs_sd)                                # :828
+    L = np.linalg.cholesky(np.eye(m) + A @ A.T)                   # :831-834
+    diff = np.asarray(y, np.float64).reshape(-1) - float(mean)    # :836-837
+    w = sla.cho_solve((L, True), Lz_inv_Kzx @ diff)               # :840
+    Kzz_inv_Kzx_diff = sla.solve_triangular(Lz.T, w, lower=False)  # :843
+    Ktt = gram(kind, T, ls, var)                                  # :845
+    Kzt = cross_covariance(kind, Z, T, ls, var)                   # :846
+    Lz_inv_Kzt = sla.solve_triangular(Lz, Kzt, lower=True)        # :850
+    L_inv_Lz_inv_Kzt = sla.solve_triangular(L, Lz_inv_Kzt, lower=True)  # :853
+    mu = float(mean) + (Kzt.T / noise_var) @ Kzz_inv_Kzx_diff     # :856
+    cov = Ktt - Lz_inv_Kzt.T @ Lz_inv_Kzt + L_inv_Lz_inv_Kzt.T @ L_inv_Lz_inv_Kzt  # :859-863
+    return mu, add_jitter(cov, jitter)                            # :864-866
+
+
+def conjugate_loocv(kind, X, y, ls, var, obs_sd, jitter=1e-6, mean=0.0) -> float:
+    """``objectives.conjugate_loocv`` (``objectives.py:156-178``): sum over i of
+    log N(y_i; mu_i, 1 / [Sigma^-1]_ii), mu_i = y_i - [Sigma^-1 r]_i / [Sigma^-1]_ii,
+    Sigma = K + (obs_sd^2 + jitter) I, r = y - c; Sigma^-1 by ``inv`` (``:171``)."""
+    X = np.asarray(X, np.float64)
+    n = X.shape[0]
+    y = np.asarray(y, np.float64).reshape(-1)
+    S = gram(kind, X, ls, var) + np.eye(n) * (float(obs_sd) ** 2 + jitter)  # :163-168
+    r = y - float(mean)
+    Sinv_r = np.linalg.solve(S, r)                                # :170
+    dg = np.diag(np.linalg.inv(S))                                # :171-172
+    mu = float(mean) + r - Sinv_r / dg                            # :174
+    sd = np.sqrt(1.0 / dg)                                        # :175
+    z = (y - mu) / sd
+    return float(np.sum(-0.5 * z * z - np.log(sd) - 0.5 * LOG2PI))  # :177-178
+
+
+def conjugate_loocv_grad(kind, X, y, ls, var, obs_sd, jitter=1e-6, mean=0.0) -> dict:
+    """Gradient of :func:`conjugate_loocv` in constrained coordinates (``jax.grad``
+    in the reference, ``tests/test_objectives.py:123-126``).
+
+    With alpha = Sigma^-1 r and d_i = [Sigma^-1]_ii the objective is
+    sum_i [-1/2 log 2pi + 1/2 log d_i - 1/2 alpha_i^2 / d_i], so
+    d LOOCV = 1/2 tr(W dSigma) + sum(w) dc with v = alpha / d, w = Sigma^-1 v,
+    b = 1/(2d) + alpha^2/(2 d^2) and W = alpha w^T + w alpha^T - 2 Sigma^-1 diag(b) Sigma^-1
+    (Rasmussen & Williams 2006, sec. 5.4.2 and eq. 5.13).
+    """
+    X = np.asarray(X, np.float64)
+    n, D = X.shape
+    lsv = _as_ls(ls, D)
+    xs = X / lsv
+    d2 = squared_distance(xs, xs)
+    ku = unit_kernel_from_d2(kind, d2)
+    S = float(var) * ku + np.eye(n) * (float(obs_sd) ** 2 + jitter)
+    r = np.asarray(y, np.float64).reshape(-1) - float(mean)
+    Sinv = np.linalg.inv(S)
+    alpha = Sinv @ r
+    dg = np.diag(Sinv).copy()
+    val = float(np.sum(-0.5 * LOG2PI + 0.5 * np.log(dg) - 0.5 * alpha ** 2 / dg))
+    w = Sinv @ (alpha / dg)
+    b = 0.5 / dg + 0.5 * alpha ** 2 / dg ** 2
+    W = np.outer(alpha, w) + np.outer(w, alpha) - 2.0 * (Sinv * b[None, :]) @ Sinv
+    F = float(var) * unit_kernel_dlog_factor(kind, d2)
+    WF = W * F
+    g_ls = np.empty(D)
+    for d in range(D):
+        dd = X[:, d][:, None] / lsv[d] - X[:, d][None, :] / lsv[d]
+        g_ls[d] = 0.5 * float(np.sum(WF * dd * dd)) / lsv[d]
+    g_l = g_ls if np.ndim(ls) else float(np.sum(g_ls))
+    return {"loocv": val, "lengthscale": g_l, "variance": 0.5 * float(np.sum(W * ku)),
+            "obs_stddev": float(obs_sd) * float(np.trace(W)), "mean": float(np.sum(w))}
+
+
 def gaussian_likelihood_predict(cov: np.ndarray, obs_sd: float) -> np.ndarray:
     """Noisy predictive covariance, diag += obs_sd^2 (``likelihoods.py:182-186``)."""
     return cov + np.eye(cov.shape[0]) * (float(obs_sd) ** 2)

@@ -72,6 +72,27 @@ def test_workspace_too_small_is_reported():
     assert "workspace" in lib.gpk_last_error().decode()
 
 
+def test_elbo_predict_workspace_and_validation():
+    lib = _lib.lib()
+    base = lib.gpk_elbo_workspace_bytes(5000, 300, 4, 0)
+    full = lib.gpk_elbo_predict_workspace_bytes(5000, 300, 4, 1000, 1)
+    diag = lib.gpk_elbo_predict_workspace_bytes(5000, 300, 4, 1000, 0)
+    # Kzt and the two solves (3 x 512 x 1024 doubles), plus 1024^2 doubles for Ktt
+    assert base + 3 * 512 * 1024 * 8 <= diag < full
+    assert full - diag >= 1024 * 1024 * 8 - 256
+    assert lib.gpk_elbo_predict_workspace_bytes(5000, 300, 4, 0, 1) == 0
+    dummy = ctypes.c_void_p(0x1000)
+    rc = lib.gpk_elbo_predict(0, 0, dummy, dummy, 100, dummy, 10, 2, dummy, 5, dummy, 1, 1e-6,
+                              1, dummy, 16, dummy, dummy, None)
+    assert rc == 5 and "workspace" in lib.gpk_last_error().decode()
+    rc = lib.gpk_elbo_predict(0, 1, dummy, dummy, 100, dummy, 10, 2, dummy, 5, dummy, 1, 1e-6,
+                              1, dummy, 1 << 40, dummy, dummy, None)
+    assert rc == 1   # the collapsed path is float64 only
+    rc = lib.gpk_elbo_predict(0, 0, dummy, dummy, 100, dummy, 10, 2, None, 5, dummy, 1, 1e-6,
+                              1, dummy, 1 << 40, dummy, dummy, None)
+    assert rc == 1 and "xt" in lib.gpk_last_error().decode()
+
+
 def test_mixed_arith_option_validation():
     lib = _lib.lib()
     assert lib.gpk_set_mixed_arith(4, 3, 6) == 1

@@ -81,3 +81,55 @@ def test_elbo_equals_mll_when_z_is_x(cuda):
     elbo = gpx.objectives.collapsed_elbo(q, D).item()
     mll = gpx.objectives.conjugate_mll(post, D).item()
     assert abs(elbo - mll) / abs(mll) < 1e-6
+
+
+@pytest.mark.parametrize("kind,n,m,d,t,ard,full", [("matern32", 3000, 300, 8, 257, True, True),
+                                                   ("rbf", 500, 40, 2, 100, True, False),
+                                                   ("matern52", 1234, 129, 3, 300, False, True),
+                                                   ("matern12", 700, 64, 1, 1000, True, False)])
+def test_collapsed_predict_parity(cuda, kind, n, m, d, t, ard, full):
+    """gpk_elbo_predict against the oracle's restatement of
+    CollapsedVariationalGaussian.predict (pinned in tests/test_oracle.py by the
+    Z = X invariant and the textbook dense form).  Float64: mean and covariance
+    within 1e-8 of their largest entry; the diagonal-only call equals the
+    diagonal of the full one within 1e-12."""
+    lsr = (0.05, 0.15) if kind == "rbf" else (0.2, 0.6)
+    X, y, Z, ls, var, sd, mean, k, post, q, D = _setup(cuda, kind, n, m, d, seed=n + t, ard=ard,
+                                                       lsr=lsr)
+    T = np.random.default_rng(t).uniform(-0.1, 1.1, (t, d))
+    Tt = torch.tensor(T, device=cuda)
+    pred = q.predict(Tt, D, full_covariance=full)
+    mu_ref, cov_ref = O.collapsed_predict(kind, X, y, Z, T, ls, var, sd, 1e-6, mean)
+    mu = pred.mean.cpu().numpy()
+    cov = pred.covariance().cpu().numpy()
+    assert np.max(np.abs(mu - mu_ref)) <= 1e-8 * np.max(np.abs(mu_ref))
+    if full:
+        assert np.max(np.abs(cov - cov_ref)) <= 1e-8 * np.max(np.abs(cov_ref))
+        dg = q.predict(Tt, D, full_covariance=False).covariance().diagonal().cpu().numpy()
+        assert np.max(np.abs(dg - np.diag(cov))) <= 1e-12 * np.max(np.abs(dg))
+    else:
+        assert cov.shape == (t, t) and np.count_nonzero(cov - np.diag(np.diag(cov))) == 0
+        dref = np.diag(cov_ref)
+        assert np.max(np.abs(np.diag(cov) - dref)) <= 1e-8 * np.max(np.abs(dref))
+
+
+def test_collapsed_predict_equals_exact_when_z_is_x(cuda):
+    """Z = X makes the collapsed posterior the exact one (up to the jitter on
+    Kzz): the device's collapsed predictive matches ConjugatePosterior.predict."""
+    rng = np.random.default_rng(5)
+    n, d, t = 300, 2, 64
+    X = rng.uniform(-2, 2, (n, d))
+    y = np.sin(X[:, 0]) + 0.1 * rng.standard_normal(n)
+    k = gpx.kernels.Matern52(lengthscale=np.array([0.9, 1.3]), variance=1.1)
+    post = gpx.gps.Prior(mean_function=gpx.mean_functions.Zero(), kernel=k) * \
+        gpx.likelihoods.Gaussian(num_datapoints=n, obs_stddev=0.3)
+    D = gpx.Dataset(X=torch.tensor(X, device=cuda), y=torch.tensor(y[:, None], device=cuda))
+    q = gpx.variational_families.CollapsedVariationalGaussian(posterior=post, inducing_inputs=X,
+                                                              jitter=1e-10)
+    T = torch.tensor(rng.uniform(-2.5, 2.5, (t, d)), device=cuda)
+    pc = q.predict(T, D)
+    post.jitter = 0.0
+    post.prior.jitter = 1e-10
+    pe = post.predict(T, D, precision="fp64")
+    assert torch.allclose(pc.mean, pe.mean, rtol=1e-6, atol=1e-8)
+    assert torch.allclose(pc.covariance(), pe.covariance(), rtol=1e-6, atol=1e-8)

@@ -14,7 +14,11 @@ something that does not share its code path:
   * collapsed ELBO: the reference's invariant Z = X => ELBO == MLL within 1e-6
     relative (``tests/test_objectives.py:193-199``) and a dense evaluation
     log N(y; m, s^2 I + Q) - tr(K - Q) / (2 s^2);
-  * predict: dense solve-based formulas, and exact interpolation as noise -> 0.
+  * predict: dense solve-based formulas, and exact interpolation as noise -> 0;
+  * collapsed predict: Z = X gives the exact predictive, Titsias' dense form;
+  * LOOCV: brute-force leave-one-out conditioning (N - 1 point solves per
+    held-out point), the N = 1 closed form; its gradient against torch
+    autograd of that brute force and central differences.
 A planted-mistake test shows the gradient pins reject a dropped factor.
 """
 
@@ -321,3 +325,138 @@ def test_predict_interpolates_without_noise():
     mu, cov = O.predict("rbf", X, y, X, 0.7, 1.0, 1e-5, 1e-10, 1e-10)
     np.testing.assert_allclose(mu, y, atol=1e-6)
     assert np.max(np.abs(np.diag(cov))) < 1e-6
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_collapsed_predict_equals_exact_when_z_is_x(kind):
+    """With Z = X the collapsed posterior is the exact one: Q = K, so mean and
+    covariance equal ConjugatePosterior.predict's (up to the jitter on Kzz)."""
+    X, y = _data(40, 2, 21)
+    T = np.random.default_rng(22).uniform(-2.5, 2.5, (12, 2))
+    ls, var, sd, c = np.array([0.9, 1.4]), 1.1, 0.3, -0.2
+    mu, cov = O.collapsed_predict(kind, X, y, X, T, ls, var, sd, 1e-10, c)
+    mu_ex, cov_ex = O.predict(kind, X, y, T, ls, var, sd, 0.0, 1e-10, c)
+    np.testing.assert_allclose(mu, mu_ex, rtol=1e-6, atol=1e-8)
+    np.testing.assert_allclose(cov, cov_ex, rtol=1e-6, atol=1e-8)
+
+
+def test_collapsed_predict_dense_form():
+    """Titsias' predictive in its textbook dense form:
+    mean = c + Ktz (Kzz + Kzx Kxz / s)^-1 Kzx (y - c) / s,
+    cov = Ktt - Ktz Kzz^-1 Kzt + Ktz (Kzz + Kzx Kxz / s)^-1 Kzt + jitter I."""
+    X, y = _data(60, 3, 23)
+    Z = X[:17] + 0.05
+    T = np.random.default_rng(24).uniform(-2, 2, (9, 3))
+    ls, var, sd, c, jit = np.array([0.8, 1.1, 1.5]), 0.9, 0.25, 0.1, 1e-6
+    mu, cov = O.collapsed_predict("matern32", X, y, Z, T, ls, var, sd, jit, c)
+    s = sd ** 2
+    Kzz = O.gram("matern32", Z, ls, var) + jit * np.eye(17)
+    Kzx = O.cross_covariance("matern32", Z, X, ls, var)
+    Kzt = O.cross_covariance("matern32", Z, T, ls, var)
+    Sig = Kzz + Kzx @ Kzx.T / s
+    mu_ref = c + Kzt.T @ np.linalg.solve(Sig, Kzx @ (y - c)) / s
+    cov_ref = (O.gram("matern32", T, ls, var) - Kzt.T @ np.linalg.solve(Kzz, Kzt)
+               + Kzt.T @ np.linalg.solve(Sig, Kzt) + jit * np.eye(9))
+    np.testing.assert_allclose(mu, mu_ref, rtol=1e-8, atol=1e-10)
+    np.testing.assert_allclose(cov, cov_ref, rtol=1e-7, atol=1e-10)
+
+
+def _brute_loocv(kind, X, y, ls, var, sd, c, jitter=1e-6):
+    """Leave each point out, condition the remaining N-1 on the noisy Gaussian,
+    and score the held-out y_i under its noisy predictive (Rasmussen & Williams
+    2006, eqs. 5.10-5.12 before their closed form)."""
+    n = X.shape[0]
+    S = O.gram(kind, X, ls, var) + np.eye(n) * (sd ** 2 + jitter)
+    tot = 0.0
+    for i in range(n):
+        o = np.r_[0:i, i + 1:n]
+        if o.size:
+            sol = np.linalg.solve(S[np.ix_(o, o)], np.c_[y[o] - c, S[o, i]])
+            mu = c + S[i, o] @ sol[:, 0]
+            v = S[i, i] - S[i, o] @ sol[:, 1]
+        else:
+            mu, v = c, S[i, i]
+        tot += st.norm(mu, math.sqrt(v)).logpdf(y[i])
+    return tot
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("n,d", [(1, 1), (2, 2), (10, 3), (33, 2)])
+def test_loocv_vs_brute_force(kind, n, d):
+    X, y = _data(n, d, 7 * n + d)
+    ls, var, sd, c = np.linspace(0.7, 1.5, d), 1.2, 0.3, 0.2
+    ref = _brute_loocv(kind, X, y, ls, var, sd, c)
+    assert O.conjugate_loocv(kind, X, y, ls, var, sd, 1e-6, c) == pytest.approx(ref, rel=1e-10)
+
+
+def test_loocv_n1_closed_form():
+    """One point: the held-out predictive is the prior, N(c, var + jitter + sd^2)."""
+    X, y = np.array([[0.3]]), np.array([1.7])
+    var, sd, jit, c = 1.2, 0.4, 1e-6, 0.5
+    s = var + jit + sd ** 2
+    ref = -0.5 * (math.log(2 * math.pi) + math.log(s) + (y[0] - c) ** 2 / s)
+    assert O.conjugate_loocv("rbf", X, y, 1.0, var, sd, jit, c) == pytest.approx(ref, rel=1e-14)
+
+
+def _torch_brute_loocv(kind, X, y, ls, var, sd, c, jitter=1e-6):
+    xs = X / ls
+    diff = xs[:, None, :] - xs[None, :, :]
+    d2 = (diff * diff).sum(-1)
+    if kind == "rbf":
+        K = var * torch.exp(-0.5 * d2)
+    else:
+        r = torch.sqrt(torch.clamp(d2, min=1e-36))
+        if kind == "matern12":
+            K = var * torch.exp(-r)
+        elif kind == "matern32":
+            K = var * (1 + math.sqrt(3) * r) * torch.exp(-math.sqrt(3) * r)
+        else:
+            K = var * (1 + math.sqrt(5) * r + 5.0 / 3.0 * r * r) * torch.exp(-math.sqrt(5) * r)
+    n = X.shape[0]
+    S = K + torch.eye(n, dtype=torch.float64) * (sd ** 2 + jitter)
+    tot = 0.0
+    for i in range(n):
+        o = [k for k in range(n) if k != i]
+        Soo = S[o][:, o]
+        mu = c + S[i, o] @ torch.linalg.solve(Soo, y[o] - c)
+        v = S[i, i] - S[i, o] @ torch.linalg.solve(Soo, S[o, i])
+        tot = tot - 0.5 * (math.log(2 * math.pi) + torch.log(v) + (y[i] - mu) ** 2 / v)
+    return tot
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("ard", [False, True])
+def test_loocv_grad_vs_torch_autograd_of_brute_force(kind, ard):
+    X, y = _data(16, 3, 8)
+    ls = np.array([0.6, 1.1, 1.9]) if ard else 0.9
+    var, sd, c = 1.3, 0.25, 0.1
+    g = O.conjugate_loocv_grad(kind, X, y, ls, var, sd, 1e-6, c)
+    tl = torch.tensor(ls, dtype=torch.float64, requires_grad=True)
+    tv = torch.tensor(var, dtype=torch.float64, requires_grad=True)
+    ts = torch.tensor(sd, dtype=torch.float64, requires_grad=True)
+    tc = torch.tensor(c, dtype=torch.float64, requires_grad=True)
+    v = _torch_brute_loocv(kind, torch.tensor(X), torch.tensor(y), tl, tv, ts, tc)
+    v.backward()
+    assert g["loocv"] == pytest.approx(v.item(), rel=1e-10)
+    ref = np.concatenate([np.atleast_1d(tl.grad.numpy()),
+                          [tv.grad.item(), ts.grad.item(), tc.grad.item()]])
+    np.testing.assert_allclose(_grad_vec(g, ard), ref, rtol=1e-8, atol=1e-10)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_loocv_grad_vs_central_differences(kind):
+    X, y = _data(30, 2, 9)
+    ls, var, sd, c = np.array([0.8, 1.4]), 1.1, 0.3, -0.2
+    g = O.conjugate_loocv_grad(kind, X, y, ls, var, sd, 1e-6, c)
+    f = lambda a, b, s, m: O.conjugate_loocv(kind, X, y, a, b, s, 1e-6, m)
+    h = 1e-6
+    fd = []
+    for i in range(2):
+        e = np.zeros(2)
+        e[i] = h
+        fd.append((f(ls + e, var, sd, c) - f(ls - e, var, sd, c)) / (2 * h))
+    fd.append((f(ls, var + h, sd, c) - f(ls, var - h, sd, c)) / (2 * h))
+    fd.append((f(ls, var, sd + h, c) - f(ls, var, sd - h, c)) / (2 * h))
+    fd.append((f(ls, var, sd, c + h) - f(ls, var, sd, c - h)) / (2 * h))
+    np.testing.assert_allclose(_grad_vec(g, True), np.array(fd), rtol=2e-6, atol=1e-7)
+    assert g["loocv"] == pytest.approx(f(ls, var, sd, c), rel=1e-12)
