@@ -40,6 +40,9 @@ SIGNATURES = {
     "gpk_elbo_workspace_bytes": (_S, [_L, _L, _L, _I]),
     "gpk_elbo_forward": (_I, [_I, _I, _P, _P, _L, _P, _L, _L, _P, _I, _D, _P, _S, _P, _P]),
     "gpk_elbo_backward": (_I, [_I, _I, _P, _P, _L, _P, _L, _L, _P, _I, _D, _P, _S, _P, _P, _P]),
+    "gpk_loocv_workspace_bytes": (_S, [_L, _L, _I]),
+    "gpk_loocv_forward": (_I, [_I, _I, _P, _P, _L, _L, _P, _I, _D, _I, _P, _S, _P, _P]),
+    "gpk_loocv_backward": (_I, [_I, _I, _P, _L, _L, _P, _I, _D, _P, _S, _P, _P]),
     "gpk_elbo_predict_workspace_bytes": (_S, [_L, _L, _L, _L, _I]),
     "gpk_elbo_predict": (_I, [_I, _I, _P, _P, _L, _P, _L, _L, _P, _L, _P, _I, _D, _I, _P, _S,
                               _P, _P, _P]),

@@ -14,6 +14,12 @@
 // with wt = 2 off the diagonal tile (symmetry) and 1 on it.  Sigma^-1 and
 // dK/dtheta are never written to memory.  Each tile writes its nls + 2 sums to
 // `partial`; a deterministic reduction follows.
+//
+// The same pass serves any objective whose derivative has the form
+// 1/2 tr(W dSigma) with W = alpha beta^T + beta alpha^T - Y^T Y: `beta` set
+// selects that form (beta null: alpha alpha^T), and `dense` makes Y a dense
+// matrix (k over all rows) instead of the lower-triangular X.  The
+// leave-one-out objective uses it with Y = diag(c) Sigma^-1.
 #include "gemm_core.h"
 
 namespace gpk {
@@ -25,15 +31,17 @@ __global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __rest
                                                       const double* __restrict__ x, int64_t n,
                                                       int d, const double* __restrict__ theta,
                                                       int nls, const double* __restrict__ alpha,
+                                                      const double* __restrict__ beta, int dense,
                                                       double* __restrict__ partial, Batch bt) {
   typedef typename AccT<AR>::scalar AS;
   bmember(bt, d, x, n, theta);
   X = bws(X, bt);
   alpha = bws(alpha, bt);
+  beta = bws(beta, bt);
   partial = bws(partial, bt);
   constexpr size_t ML = smem_bytes<AR>();
   constexpr size_t XS = (size_t)2 * TILE * (DMAX + 1) * sizeof(double);
-  constexpr size_t AB = (size_t)2 * TILE * sizeof(double);
+  constexpr size_t AB = (size_t)4 * TILE * sizeof(double);
   constexpr size_t WB = (size_t)4 * 16 * 64 * sizeof(AS);
   constexpr size_t EP = XS + AB + WB;
   __shared__ __align__(16) char smem[ML > EP ? ML : EP];
@@ -42,11 +50,12 @@ __global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __rest
   int64_t mt, nt;
   lower_tile(blockIdx.x, mt, nt);
   const int64_t m0 = mt * TILE, n0 = nt * TILE;
+  const int64_t kb = dense ? 0 : m0;
   typename AccT<AR>::vec acc[4][4];
   if constexpr (AR == AR_F64) {
-    mainloop_f64<TS, TS, true, true>(X, ld, X, ld, m0, n0, m0, np, acc, smem);
+    mainloop_f64<TS, TS, true, true>(X, ld, X, ld, m0, n0, kb, np, acc, smem);
   } else {
-    mainloop_bf16<AR, true, true>(X, ld, X, ld, m0, n0, m0, np, acc, smem);
+    mainloop_bf16<AR, true, true>(X, ld, X, ld, m0, n0, kb, np, acc, smem);
   }
   __syncthreads();
 
@@ -57,7 +66,9 @@ __global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __rest
   double* xn = xm + TILE * (DMAX + 1);                 // [TILE][DMAX+1]
   double* am = xn + TILE * (DMAX + 1);                 // [TILE]
   double* an = am + TILE;                              // [TILE]
-  AS* wbuf = reinterpret_cast<AS*>(an + TILE);         // [4 waves][16][64 lanes]
+  double* bm = an + TILE;                              // [TILE]
+  double* bn = bm + TILE;                              // [TILE]
+  AS* wbuf = reinterpret_cast<AS*>(bn + TILE);         // [4 waves][16][64 lanes]
   const int t = threadIdx.x;
   for (int e = t; e < TILE * DMAX; e += 256) {
     const int r = e / DMAX, q = e % DMAX;
@@ -71,6 +82,10 @@ __global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __rest
   if (t < TILE) {
     am[t] = alpha[m0 + t];
     an[t] = alpha[n0 + t];
+    if (beta) {
+      bm[t] = beta[m0 + t];
+      bn[t] = beta[n0 + t];
+    }
   }
   __syncthreads();
 
@@ -98,7 +113,8 @@ __global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __rest
       const int lj = wn * 64 + j * 16 + (lane & 15);
       const int64_t gi = m0 + li, gj = n0 + lj;
       if (gi >= n || gj >= n) continue;
-      const double wij = am[li] * an[lj] - (double)mine[e * 64];
+      const double wij = (beta ? am[li] * bn[lj] + bm[li] * an[lj] : am[li] * an[lj]) -
+                         (double)mine[e * 64];
       const double* pi = xm + li * (DMAX + 1);
       const double* pj = xn + lj * (DMAX + 1);
       double d2 = 0.0;
@@ -149,12 +165,12 @@ __global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __rest
 template <int AR, typename TS>
 int launch_ar(int kind, const TS* X, int64_t ld, int64_t np, const double* x, int64_t n, int d,
               const double* theta, int nls, const double* alpha, double* partial,
-              hipStream_t s, const Batch& bt) {
+              hipStream_t s, const Batch& bt, const double* beta, bool dense) {
   const int64_t nt = np / TILE;
   const dim3 grid((unsigned)(nt * (nt + 1) / 2), 1, bt.count);
 #define GPK_ADJ(DM)                                                                        \
   hipLaunchKernelGGL((adjoint_kernel<AR, TS, DM>), grid, dim3(256), 0, s, kind, X, ld, np, x, \
-                     n, d, theta, nls, alpha, partial, bt)
+                     n, d, theta, nls, alpha, beta, dense ? 1 : 0, partial, bt)
   if (d <= 1) GPK_ADJ(1);
   else if (d <= 2) GPK_ADJ(2);
   else if (d <= 4) GPK_ADJ(4);
@@ -171,7 +187,8 @@ int launch_ar(int kind, const TS* X, int64_t ld, int64_t np, const double* x, in
 template <typename TS>
 int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, const double* x,
                    int64_t n, int d, const double* theta, int nls, const double* alpha,
-                   double* partial, hipStream_t s, const Batch& bt) {
+                   double* partial, hipStream_t s, const Batch& bt, const double* beta,
+                   bool dense) {
   if (d < 1 || d > MAX_D) {
     set_error("adjoint: input dimension %d outside [1, %d]", d, MAX_D);
     return GPK_ERR_BAD_ARG;
@@ -181,14 +198,15 @@ int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, con
     return GPK_ERR_BAD_ARG;
   }
   if (arith == AR_F64)
-    return launch_ar<AR_F64, TS>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s, bt);
+    return launch_ar<AR_F64, TS>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s, bt,
+                                 beta, dense);
   if constexpr (std::is_same<TS, float>::value) {
     if (arith == AR_BF16X3)
       return launch_ar<AR_BF16X3, float>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s,
-                                         bt);
+                                         bt, beta, dense);
     if (arith == AR_BF16X6)
       return launch_ar<AR_BF16X6, float>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s,
-                                         bt);
+                                         bt, beta, dense);
   }
   set_error("adjoint: unsupported arithmetic %d", arith);
   return GPK_ERR_BAD_ARG;
@@ -196,9 +214,9 @@ int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, con
 
 template int launch_adjoint<double>(int, int, const double*, int64_t, int64_t, const double*,
                                     int64_t, int, const double*, int, const double*, double*,
-                                    hipStream_t, const Batch&);
+                                    hipStream_t, const Batch&, const double*, bool);
 template int launch_adjoint<float>(int, int, const float*, int64_t, int64_t, const double*,
                                    int64_t, int, const double*, int, const double*, double*,
-                                   hipStream_t, const Batch&);
+                                   hipStream_t, const Batch&, const double*, bool);
 
 }  // namespace gpk

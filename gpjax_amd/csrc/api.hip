@@ -429,6 +429,156 @@ int predict_t(int kind, int prec, const double* x, const double* y, int64_t n, i
   return GPK_OK;
 }
 
+// ---- leave-one-out predictive probability: conjugate_loocv (objectives.py:156-178) ----
+// With alpha = Sigma^-1 r and d_i = [Sigma^-1]_ii (column sums of squares of
+// X = L^-1; the reference inverts Sigma densely, objectives.py:171):
+//   LOOCV = sum_{i<n} -1/2 log 2pi + 1/2 log d_i - 1/2 alpha_i^2 / d_i.
+// Its derivative is 1/2 tr(W dSigma) + sum(w) dc with v = alpha / d,
+// w = Sigma^-1 v, c_i^2 = 1/d_i + alpha_i^2/d_i^2 and
+//   W = alpha w^T + w alpha^T - Y^T Y,   Y = diag(c) Sigma^-1,
+// which the adjoint pass reduces against dSigma/dtheta with Y dense.
+// Sigma^-1 = X^T X is formed (lower tiles, mirrored and row-scaled in place)
+// in the buffer that held L, which the backward no longer needs.
+struct LooLayout {
+  size_t off_d = 0, off_v = 0, off_w = 0, off_c = 0, off_tmp = 0, off_mres = 0, total = 0;
+};
+
+LooLayout make_loo_layout(const Layout& Lo) {
+  LooLayout P;
+  const size_t V = al((size_t)Lo.np * sizeof(double));
+  size_t o = Lo.total;
+  P.off_d = o; o += V;
+  P.off_v = o; o += V;
+  P.off_w = o; o += V;
+  P.off_c = o; o += V;
+  P.off_tmp = o; o += V;
+  P.off_mres = o; o += al((size_t)GPK_RESULT_LEN * sizeof(double));
+  P.total = o;
+  return P;
+}
+
+__global__ __launch_bounds__(256) void loo_fwd_kernel(const double* __restrict__ dg,
+                                                      const double* __restrict__ alpha,
+                                                      const double* __restrict__ mres, int64_t n,
+                                                      int64_t np, double* __restrict__ v,
+                                                      double* __restrict__ c,
+                                                      double* __restrict__ result) {
+  __shared__ double red[4];
+  double sl = 0.0, sq = 0.0;
+  for (int64_t i = threadIdx.x; i < np; i += 256) {
+    if (i < n) {
+      const double di = dg[i], a = alpha[i];
+      const double q = a * a / di;
+      sl += log(di);
+      sq += q;
+      v[i] = a / di;
+      c[i] = sqrt((1.0 + q) / di);
+    } else {
+      v[i] = 0.0;
+      c[i] = 0.0;
+    }
+  }
+  sl = block_sum(sl, red);
+  sq = block_sum(sq, red);
+  if (threadIdx.x == 0) {
+    const double log2pi = 1.8378770664093453;
+    const double info = mres[GPK_RES_INFO];
+    const double val = -0.5 * ((double)n * log2pi - sl + sq);
+    result[GPK_RES_MLL] = info != 0.0 ? __builtin_nan("") : val;
+    result[GPK_RES_LOGDET] = sl;
+    result[GPK_RES_QUAD] = sq;
+    result[GPK_RES_RESID] = mres[GPK_RES_RESID];
+    result[GPK_RES_INFO] = info;
+    for (int k = 5; k < GPK_RESULT_LEN; ++k) result[k] = 0.0;
+  }
+}
+
+// Y <- diag(c) S with S symmetric held in its lower 64-tiles (upper tiles are
+// overwritten).  One workgroup per lower tile (I, J): it reads only tile (I, J)
+// and writes tiles (I, J) and (J, I), so the update in place has no race.
+constexpr int MT = 64;
+// Lower-triangle tile t -> (I, J), I >= J, row-major over I.
+__device__ __forceinline__ void tri_tile(int64_t t, int64_t& I, int64_t& J) {
+  int64_t a = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while (a * (a + 1) / 2 > t) --a;
+  while ((a + 1) * (a + 2) / 2 <= t) ++a;
+  I = a;
+  J = t - a * (a + 1) / 2;
+}
+
+template <typename TS>
+__global__ __launch_bounds__(256) void loo_scale_mirror_kernel(TS* __restrict__ Y, int64_t ld,
+                                                               const double* __restrict__ c) {
+  __shared__ TS tl[MT][MT + 1];
+  int64_t I, J;
+  tri_tile(blockIdx.x, I, J);
+  const int64_t r0 = I * MT, c0 = J * MT;
+  for (int e = threadIdx.x; e < MT * MT; e += 256) {
+    const int r = e / MT, q = e % MT;
+    tl[r][q] = Y[(r0 + r) * ld + c0 + q];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < MT * MT; e += 256) {
+    const int r = e / MT, q = e % MT;
+    if (I == J) {
+      const TS sv = r >= q ? tl[r][q] : tl[q][r];
+      Y[(r0 + r) * ld + c0 + q] = (TS)(c[r0 + r] * (double)sv);
+    } else {
+      Y[(r0 + r) * ld + c0 + q] = (TS)(c[r0 + r] * (double)tl[r][q]);
+      Y[(c0 + r) * ld + r0 + q] = (TS)(c[c0 + r] * (double)tl[q][r]);
+    }
+  }
+}
+
+template <typename TS>
+int loocv_forward_t(int kind, int prec, const double* x, const double* y, int64_t n, int d,
+                    const double* theta, int nls, double jitter, int iters, void* ws,
+                    const Layout& Lo, const LooLayout& P, double* result, hipStream_t s) {
+  const int64_t np = Lo.np;
+  GPK_TRY(mll_forward_t<TS>(kind, prec, x, y, n, d, theta, nls, jitter, iters, ws, Lo,
+                            at<double>(ws, P.off_mres), s));
+  double* dg = at<double>(ws, P.off_d);
+  GPK_TRY(launch_colreduce<TS>(at<TS>(ws, Lo.off_X), np, np, np, nullptr, dg,
+                               at<double>(ws, Lo.off_part), s, true));
+  hipLaunchKernelGGL(loo_fwd_kernel, dim3(1), dim3(256), 0, s, dg, at<double>(ws, Lo.off_alpha),
+                     at<double>(ws, P.off_mres), n, np, at<double>(ws, P.off_v),
+                     at<double>(ws, P.off_c), result);
+  GPK_LAUNCH_CHECK();
+  return GPK_OK;
+}
+
+template <typename TS>
+int loocv_backward_t(int kind, const double* x, int64_t n, int d, const double* theta, int nls,
+                     double gbar, void* ws, const Layout& Lo, const LooLayout& P, double* grad,
+                     hipStream_t s) {
+  const int64_t np = Lo.np;
+  const bool f64 = std::is_same<TS, double>::value;
+  const int ar = f64 ? (int)AR_F64 : g_ar_lauum;
+  const TS* X = at<TS>(ws, Lo.off_X);
+  TS* Y = at<TS>(ws, Lo.off_L);
+  double* part = at<double>(ws, Lo.off_part);
+  double* w = at<double>(ws, P.off_w);
+  // w = Sigma^-1 v = X^T (X v)
+  GPK_TRY(launch_trmv<TS>(X, np, np, at<double>(ws, P.off_v), at<double>(ws, P.off_tmp), false,
+                          part, s));
+  GPK_TRY(launch_trmv<TS>(X, np, np, at<double>(ws, P.off_tmp), w, true, part, s));
+  // Y = diag(c) X^T X
+  GPK_TRY(launch_gemm<TS, TS, TS>(ar, true, true, X, np, X, np, Y, np, np, np, np, 1.0, false,
+                                  true, KR_GE_M, s));
+  const int64_t mt = np / MT;
+  hipLaunchKernelGGL(loo_scale_mirror_kernel<TS>, dim3((unsigned)(mt * (mt + 1) / 2)), dim3(256),
+                     0, s, Y, np, at<double>(ws, P.off_c));
+  GPK_LAUNCH_CHECK();
+  double* gpart = at<double>(ws, Lo.off_gpart);
+  GPK_TRY(launch_adjoint<TS>(kind, ar, Y, np, np, x, n, d, theta, nls,
+                             at<double>(ws, Lo.off_alpha), gpart, s, Batch(), w, true));
+  const int64_t nleaf = np / NB;
+  hipLaunchKernelGGL(finalize_bwd_kernel, dim3((unsigned)(nls + 3)), dim3(256), 0, s, gpart,
+                     nleaf * (nleaf + 1) / 2, nls, theta, w, n, gbar, grad, Batch());
+  GPK_LAUNCH_CHECK();
+  return GPK_OK;
+}
+
 // ---- batched exact GPs ----
 // Problems are grouped by padded size np = round_up(n, 256) (stable in the
 // problem index); each group runs the exact-GP forward and backward once, with
@@ -654,6 +804,61 @@ int gpk_predict(int kind, int prec, const double* x, const double* y, int64_t n,
                              jitter_prior, full_cov, iters, ws, Lo, P, mean, cov, s);
   return predict_t<float>(kind, prec, x, y, n, (int)d, xt, t, theta, nls, jitter_post,
                           jitter_prior, full_cov, iters, ws, Lo, P, mean, cov, s);
+}
+
+size_t gpk_loocv_workspace_bytes(int64_t n, int64_t d, int prec) {
+  if (n < 1 || d < 1) return 0;
+  return make_loo_layout(make_layout(n, d, prec)).total;
+}
+
+int gpk_loocv_forward(int kind, int prec, const double* x, const double* y, int64_t n, int64_t d,
+                      const double* theta, int nls, double jitter, int refine_iters, void* ws,
+                      size_t ws_bytes, double* result, void* stream) {
+  int rc = check_common(kind, prec, x, n, d, theta, nls);
+  if (rc) return rc;
+  if (!y || !ws || !result) {
+    set_error("loocv_forward: null y/ws/result");
+    return GPK_ERR_BAD_ARG;
+  }
+  const Layout Lo = make_layout(n, d, prec);
+  const LooLayout P = make_loo_layout(Lo);
+  if (ws_bytes < P.total) {
+    set_error("loocv_forward: workspace %zu bytes < required %zu", ws_bytes, P.total);
+    return GPK_ERR_WORKSPACE;
+  }
+  if (reinterpret_cast<uintptr_t>(ws) % ALIGN) {
+    set_error("loocv_forward: workspace must be %zu-byte aligned", ALIGN);
+    return GPK_ERR_BAD_ARG;
+  }
+  const int iters = refine_iters >= 0 ? refine_iters : (prec == GPK_PREC_FP64 ? 1 : 4);
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == GPK_PREC_FP64)
+    return loocv_forward_t<double>(kind, prec, x, y, n, (int)d, theta, nls, jitter, iters, ws,
+                                   Lo, P, result, s);
+  return loocv_forward_t<float>(kind, prec, x, y, n, (int)d, theta, nls, jitter, iters, ws, Lo,
+                                P, result, s);
+}
+
+int gpk_loocv_backward(int kind, int prec, const double* x, int64_t n, int64_t d,
+                       const double* theta, int nls, double gbar_host, void* ws, size_t ws_bytes,
+                       double* grad, void* stream) {
+  int rc = check_common(kind, prec, x, n, d, theta, nls);
+  if (rc) return rc;
+  if (!ws || !grad) {
+    set_error("loocv_backward: null ws/grad");
+    return GPK_ERR_BAD_ARG;
+  }
+  const Layout Lo = make_layout(n, d, prec);
+  const LooLayout P = make_loo_layout(Lo);
+  if (ws_bytes < P.total) {
+    set_error("loocv_backward: workspace %zu bytes < required %zu", ws_bytes, P.total);
+    return GPK_ERR_WORKSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == GPK_PREC_FP64)
+    return loocv_backward_t<double>(kind, x, n, (int)d, theta, nls, gbar_host, ws, Lo, P, grad,
+                                    s);
+  return loocv_backward_t<float>(kind, x, n, (int)d, theta, nls, gbar_host, ws, Lo, P, grad, s);
 }
 
 size_t gpk_batched_workspace_bytes(const int64_t* offsets_host, int64_t count, int64_t d) {

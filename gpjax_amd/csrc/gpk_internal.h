@@ -180,10 +180,13 @@ int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const
 // lower, Np x Np), reduce 1/2 tr(W dSigma/dtheta) contributions with
 // W = alpha alpha^T - Sinv against kernel tiles recomputed from x.
 // partial: [num_tiles][nls + 2] float64.
+// beta (optional): W = alpha beta^T + beta alpha^T - X^T X instead; dense: X is
+// a dense Np x Np matrix (every k) rather than lower triangular (k >= row).
 template <typename TS>
 int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, const double* x,
                    int64_t n, int d, const double* theta, int nls, const double* alpha,
-                   double* partial, hipStream_t s, const Batch& bt = Batch());
+                   double* partial, hipStream_t s, const Batch& bt = Batch(),
+                   const double* beta = nullptr, bool dense = false);
 
 // y = X r (trans = false) or y = X^T r (trans = true), X lower triangular
 // np x np (only k <= i is read); float64 accumulation through `partial`.
@@ -197,9 +200,11 @@ int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
                         double* partial, hipStream_t s, const Batch& bt = Batch());
 
 // y_j = sum_i M[i][j] v_i (v non-null) or sum_i M[i][j]^2 (v null), dense
-// rows x cols float64; partial holds ceil(rows/1024) x cols doubles.
-int launch_colreduce(const double* M, int64_t ld, int64_t rows, int64_t cols, const double* v,
-                     double* y, double* partial, hipStream_t s);
+// rows x cols (float64 or float32 entries, float64 sums); partial holds
+// ceil(rows/1024) x cols doubles.  lower: only rows i >= j enter column j.
+template <typename TM>
+int launch_colreduce(const TM* M, int64_t ld, int64_t rows, int64_t cols, const double* v,
+                     double* y, double* partial, hipStream_t s, bool lower = false);
 
 int launch_mfma_selftest(double* out64, float* outbf, hipStream_t s);
 

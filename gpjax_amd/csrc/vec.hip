@@ -155,20 +155,23 @@ __global__ void sum_chunks(const double* __restrict__ partial, int64_t np, int64
   y[i] = s;
 }
 
-// y_j partial over rows [c*RC, (c+1)*RC) of a dense rows x cols float64 matrix:
-// sum_i M[i][j] v_i, or sum_i M[i][j]^2 when v is null.  One thread per column.
-__global__ __launch_bounds__(256) void colreduce_kernel(const double* __restrict__ M, int64_t ld,
+// y_j partial over rows [c*RC, (c+1)*RC) of a dense rows x cols matrix:
+// sum_i M[i][j] v_i, or sum_i M[i][j]^2 when v is null (rows i >= j only when
+// lower).  One thread per column, float64 sums.
+template <typename TM>
+__global__ __launch_bounds__(256) void colreduce_kernel(const TM* __restrict__ M, int64_t ld,
                                                         int64_t rows, int64_t cols,
-                                                        const double* __restrict__ v,
+                                                        const double* __restrict__ v, int lower,
                                                         double* __restrict__ partial) {
   const int64_t c = blockIdx.y;
   const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (j >= cols) return;
-  const int64_t i0 = c * TRMV_RC;
+  int64_t i0 = c * TRMV_RC;
   const int64_t i1 = (i0 + TRMV_RC < rows) ? i0 + TRMV_RC : rows;
+  if (lower && i0 < j) i0 = j;
   double acc = 0.0;
   for (int64_t i = i0; i < i1; ++i) {
-    const double m = M[i * ld + j];
+    const double m = (double)M[i * ld + j];
     acc += v ? m * v[i] : m * m;
   }
   partial[c * cols + j] = acc;
@@ -265,18 +268,24 @@ int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
   return GPK_OK;
 }
 
-int launch_colreduce(const double* M, int64_t ld, int64_t rows, int64_t cols, const double* v,
-                     double* y, double* partial, hipStream_t s) {
+template <typename TM>
+int launch_colreduce(const TM* M, int64_t ld, int64_t rows, int64_t cols, const double* v,
+                     double* y, double* partial, hipStream_t s, bool lower) {
   if (rows <= 0 || cols <= 0) return GPK_OK;
   const int64_t nc = (rows + TRMV_RC - 1) / TRMV_RC;
-  hipLaunchKernelGGL(colreduce_kernel, dim3((unsigned)((cols + 255) / 256), (unsigned)nc),
-                     dim3(256), 0, s, M, ld, rows, cols, v, partial);
+  hipLaunchKernelGGL(colreduce_kernel<TM>, dim3((unsigned)((cols + 255) / 256), (unsigned)nc),
+                     dim3(256), 0, s, M, ld, rows, cols, v, lower ? 1 : 0, partial);
   GPK_LAUNCH_CHECK();
   hipLaunchKernelGGL(colreduce_sum, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s,
                      partial, cols, nc, y);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }
+
+template int launch_colreduce<double>(const double*, int64_t, int64_t, int64_t, const double*,
+                                      double*, double*, hipStream_t, bool);
+template int launch_colreduce<float>(const float*, int64_t, int64_t, int64_t, const double*,
+                                     double*, double*, hipStream_t, bool);
 
 int launch_mfma_selftest(double* out64, float* outbf, hipStream_t s) {
   hipLaunchKernelGGL(mfma_selftest_kernel, dim3(1), dim3(64), 0, s, out64, outbf);

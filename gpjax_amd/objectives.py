@@ -32,23 +32,32 @@ def _precision_id(precision: str | None) -> int:
     return _lib.PRECISIONS[p]
 
 
+# Exact-GP objectives sharing one factorisation and one adjoint pass:
+# (workspace size, forward, backward) entry points of include/gpk.h.
+_EXACT = {"mll": ("gpk_mll_workspace_bytes", "gpk_mll_forward", "gpk_mll_backward"),
+          "loocv": ("gpk_loocv_workspace_bytes", "gpk_loocv_forward", "gpk_loocv_backward")}
+
+
 class _ConjugateMLL(torch.autograd.Function):
-    """Forward: gpk_mll_forward.  Backward: gpk_mll_backward (fused adjoint)."""
+    """Forward: gpk_mll_forward (or gpk_loocv_forward).  Backward: the matching
+    backward entry point (fused adjoint)."""
 
     @staticmethod
-    def forward(ctx, ls, var, sd, c, X, y, kind_id, prec_id, jitter, refine, nls, info_out):
+    def forward(ctx, ls, var, sd, c, X, y, kind_id, prec_id, jitter, refine, nls, info_out,
+                which="mll"):
         n, d = X.shape
         dev = X.device
+        ws_fn, fwd_fn, bwd_fn = _EXACT[which]
         theta = torch.cat([ls.reshape(-1)[:nls], var.reshape(1), sd.reshape(1),
                            c.reshape(1)]).to(dev, torch.float64).contiguous()
-        nbytes = _lib.lib().gpk_mll_workspace_bytes(n, d, prec_id)
+        nbytes = getattr(_lib.lib(), ws_fn)(n, d, prec_id)
         ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
         res = torch.empty(_lib.RESULT_LEN, dtype=torch.float64, device=dev)
         it = -1 if refine is None else int(refine)
-        _lib.call("gpk_mll_forward", kind_id, prec_id, _lib.ptr(X), _lib.ptr(y), n, d,
+        _lib.call(fwd_fn, kind_id, prec_id, _lib.ptr(X), _lib.ptr(y), n, d,
                   _lib.ptr(theta), nls, float(jitter), it, _lib.ptr(ws), nbytes, _lib.ptr(res),
                   _lib.stream_ptr(dev))
-        ctx.ws, ctx.nbytes, ctx.theta = ws, nbytes, theta
+        ctx.ws, ctx.nbytes, ctx.theta, ctx.bwd_fn = ws, nbytes, theta, bwd_fn
         ctx.X, ctx.kind_id, ctx.prec_id, ctx.nls = X, kind_id, prec_id, nls
         ctx.shapes = (ls.shape, var.shape, sd.shape, c.shape)
         if info_out is not None:
@@ -61,7 +70,7 @@ class _ConjugateMLL(torch.autograd.Function):
         n, d = X.shape
         nls = ctx.nls
         grad = torch.empty(nls + 3, dtype=torch.float64, device=X.device)
-        _lib.call("gpk_mll_backward", ctx.kind_id, ctx.prec_id, _lib.ptr(X), n, d,
+        _lib.call(ctx.bwd_fn, ctx.kind_id, ctx.prec_id, _lib.ptr(X), n, d,
                   _lib.ptr(ctx.theta), nls, 1.0, _lib.ptr(ctx.ws), ctx.nbytes, _lib.ptr(grad),
                   _lib.stream_ptr(X.device))
         g = grad * gbar
@@ -69,7 +78,7 @@ class _ConjugateMLL(torch.autograd.Function):
         ls_g = g[:nls].reshape(ls_s) if math.prod(ls_s) == nls else g[:nls].sum().expand(ls_s)
         ctx.ws = None
         return (ls_g, g[nls].reshape(var_s), g[nls + 1].reshape(sd_s), g[nls + 2].reshape(c_s),
-                None, None, None, None, None, None, None, None)
+                None, None, None, None, None, None, None, None, None)
 
 
 def _mean_constant(mean_function, device) -> torch.Tensor:
@@ -79,13 +88,13 @@ def _mean_constant(mean_function, device) -> torch.Tensor:
 
 
 def conjugate_mll_with_info(posterior, data, precision: str | None = None,
-                            refine_iters: int | None = None):
+                            refine_iters: int | None = None, _which: str = "mll"):
     """(mll, result) where result is the device buffer of ``gpk_mll_forward``
     (slots: mll, logdet, quad, relative refinement residual, info)."""
     kernel = posterior.prior.kernel
     X = kernel.slice_input(data.X)
     if not X.is_cuda:
-        raise ValueError("conjugate_mll: data must live on a GPU device")
+        raise ValueError(f"conjugate_{_which}: data must live on a GPU device")
     y = data.y.reshape(-1).to(torch.float64).contiguous()
     n, d = X.shape
     if y.shape[0] != n:
@@ -97,13 +106,28 @@ def conjugate_mll_with_info(posterior, data, precision: str | None = None,
                               posterior.likelihood.obs_stddev.value, c, X, y, kernel.kind_id,
                               _precision_id(precision), float(posterior.prior.jitter),
                               config.refine_iters if refine_iters is None else refine_iters,
-                              nls, info)
+                              nls, info, _which)
     return mll, info["result"]
 
 
 def conjugate_mll(posterior, data, precision: str | None = None):
     """Log marginal likelihood (``gpjax/objectives.py:36-107``), differentiable."""
     return conjugate_mll_with_info(posterior, data, precision)[0]
+
+
+def conjugate_loocv_with_info(posterior, data, precision: str | None = None,
+                              refine_iters: int | None = None):
+    """(loocv, result) where result is the device buffer of ``gpk_loocv_forward``
+    (slots: loocv, sum log d_i, sum alpha_i^2 / d_i, refinement residual, info)."""
+    return conjugate_mll_with_info(posterior, data, precision, refine_iters, _which="loocv")
+
+
+def conjugate_loocv(posterior, data, precision: str | None = None):
+    """Leave-one-out log predictive probability (``gpjax/objectives.py:110-178``),
+    Rasmussen & Williams sec. 5.4.2, differentiable in the kernel hyperparameters,
+    obs_stddev and a trainable constant mean.  Sigma = K + (obs_sd^2 + jitter) I
+    with the prior's jitter (``objectives.py:163-168``)."""
+    return conjugate_loocv_with_info(posterior, data, precision)[0]
 
 
 class _CollapsedELBO(torch.autograd.Function):

@@ -173,6 +173,35 @@ int gpk_predict(int kind, int prec, const double* x, const double* y, int64_t n,
                 double jitter_prior, int full_cov, int refine_iters, void* ws, size_t ws_bytes,
                 double* mean, double* cov, void* stream);
 
+/* Bytes of workspace gpk_loocv_forward / gpk_loocv_backward need. */
+size_t gpk_loocv_workspace_bytes(int64_t n, int64_t d, int prec);
+
+/*
+ * Leave-one-out log predictive probability, gpjax/objectives.py:110-178:
+ *   sum_i log N(y_i; y_i - alpha_i / d_i, 1 / d_i),
+ * alpha = Sigma^-1 (y - c), d = diag(Sigma^-1), Sigma = K + (jitter + obs_sd^2) I
+ * (the reference inverts Sigma densely, objectives.py:171; here d is the column
+ * sums of squares of L^-1).  Same factorisation, refinement and precisions as
+ * gpk_mll_forward.  result slots: [GPK_RES_MLL] = LOOCV, [1] = sum log d_i,
+ * [2] = sum alpha_i^2 / d_i, [GPK_RES_RESID] = relative residual of alpha,
+ * [GPK_RES_INFO] = pivot status.
+ */
+int gpk_loocv_forward(int kind, int prec, const double* x, const double* y, int64_t n, int64_t d,
+                      const double* theta, int nls, double jitter, int refine_iters, void* ws,
+                      size_t ws_bytes, double* result, void* stream);
+
+/*
+ * Gradient of the LOOCV objective (scaled by gbar_host) w.r.t. theta into grad
+ * [nls + 3] (jax.grad of conjugate_loocv in the reference,
+ * tests/test_objectives.py:123-126).  Must follow gpk_loocv_forward on the same
+ * workspace and stream: forms Sigma^-1 = L^-T L^-1 in the buffer that held L and
+ * reduces W = alpha w^T + w alpha^T - Sigma^-1 diag(c^2) Sigma^-1 against
+ * dSigma/dtheta (w = Sigma^-1 (alpha / d), c^2 = 1/d + alpha^2/d^2).
+ */
+int gpk_loocv_backward(int kind, int prec, const double* x, int64_t n, int64_t d,
+                       const double* theta, int nls, double gbar_host, void* ws, size_t ws_bytes,
+                       double* grad, void* stream);
+
 /*
  * Many independent exact GPs in one call: `count` problems sharing the kernel
  * kind, the input dimension d and the lengthscale layout nls, each with its own

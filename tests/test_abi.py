@@ -72,6 +72,22 @@ def test_workspace_too_small_is_reported():
     assert "workspace" in lib.gpk_last_error().decode()
 
 
+def test_loocv_workspace_and_validation():
+    lib = _lib.lib()
+    for prec in (0, 1):
+        base = lib.gpk_mll_workspace_bytes(3000, 3, prec)
+        loo = lib.gpk_loocv_workspace_bytes(3000, 3, prec)
+        # five padded vectors and the result slots on top of the exact-GP workspace
+        assert base + 5 * 3072 * 8 <= loo <= base + 5 * 3072 * 8 + 6 * 256
+    assert lib.gpk_loocv_workspace_bytes(0, 3, 0) == 0
+    dummy = ctypes.c_void_p(0x1000)
+    rc = lib.gpk_loocv_forward(0, 0, dummy, dummy, 100, 2, dummy, 1, 1e-6, -1, dummy, 16, dummy,
+                               None)
+    assert rc == 5 and "workspace" in lib.gpk_last_error().decode()
+    rc = lib.gpk_loocv_backward(0, 0, dummy, 100, 2, dummy, 3, 1.0, dummy, 1 << 40, dummy, None)
+    assert rc == 1 and "nls" in lib.gpk_last_error().decode()
+
+
 def test_elbo_predict_workspace_and_validation():
     lib = _lib.lib()
     base = lib.gpk_elbo_workspace_bytes(5000, 300, 4, 0)
