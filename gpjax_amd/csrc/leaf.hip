@@ -11,9 +11,13 @@
 // of that array is exactly [X-row-k numerators ; pivot ; L-column-k
 // numerators], so one broadcast of column k per step drives both updates.
 //
-// Thread t of 1024 owns the 4 x 4 block rows 4*(t/32).., cols 4*(t%32)..;
-// column k+1 is published to a double-buffered LDS vector by its owners
-// right after their update, so each step costs one barrier.
+// Thread t of 1024 owns the 4 x 4 block rows 4*(t%32).., cols 4*(t/32)..,
+// so one wave holds two whole column blocks: at step k only the waves holding
+// a column >= k have work, and the others go straight to the barrier.  Column
+// k+1 is published to a double-buffered LDS vector by its owners right after
+// their update, together with sqrt and reciprocal of the new pivot (computed
+// once, by the owner of the diagonal entry), so each step costs one barrier.
+// The update is written with selects only (no per-entry branches).
 //
 // Outputs: L (lower, zero strict upper) into A, X = L^-1 (lower, zero strict
 // upper) into X, X in float64 into leafinv, 2*sum(log L_kk) into *logdet and
@@ -25,8 +29,8 @@
 namespace gpk {
 namespace {
 
-// 1024 threads, thread t owning the EB x EB block at rows EB*(t/32), cols
-// EB*(t%32): four waves per SIMD hide the per-column LDS and barrier latency.
+// 1024 threads, thread t owning the EB x EB block at rows EB*(t%32), cols
+// EB*(t/32): four waves per SIMD hide the per-column LDS and barrier latency.
 constexpr int LEAF_THREADS = 1024;
 constexpr int EB = 4;
 constexpr int BPR = NB / EB;
@@ -41,6 +45,7 @@ __global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_
   __shared__ double col_s[2][NB];
   __shared__ double dinv_s[NB];
   __shared__ double dk_s[NB];
+  __shared__ int bad_s;
   __shared__ double tr_s[NB][NB + 1];
   A = bws(A, bt);
   X = bws(X, bt);
@@ -50,91 +55,105 @@ __global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_
   dsrc = bws(dsrc, bt);
 
   const int t = threadIdx.x;
-  const int rb = (t / BPR) * EB;
-  const int cb = (t % BPR) * EB;
+  const int rb = (t % BPR) * EB;
+  const int cb = (t / BPR) * EB;
+  // Stage the block through LDS so that global reads stay row-contiguous.
+  for (int e = t; e < NB * NB; e += LEAF_THREADS) {
+    const int i = e / NB, j = e % NB;
+    tr_s[i][j] = dsrc ? dsrc[e] : (double)A[(o + i) * ld + o + j];
+  }
+  if (t == 0) bad_s = NB;
+  __syncthreads();
   double s[EB][EB];
 #pragma unroll
-  for (int a = 0; a < EB; ++a) {
-    const TS* row = A + (o + rb + a) * ld + o + cb;
+  for (int a = 0; a < EB; ++a)
 #pragma unroll
-    for (int b = 0; b < EB; ++b) {
-      const int i = rb + a, j = cb + b;
-      const double v = dsrc ? dsrc[i * NB + j] : (double)row[b];
-      s[a][b] = (j <= i) ? v : 0.0;
-    }
-  }
+    for (int b = 0; b < EB; ++b) s[a][b] = (cb + b <= rb + a) ? tr_s[rb + a][cb + b] : 0.0;
+  // pivot 0 and column 0
   if (cb == 0) {
 #pragma unroll
     for (int a = 0; a < EB; ++a) col_s[0][rb + a] = s[a][0];
+    if (rb == 0) {
+      const double piv = s[0][0];
+      const double dk = sqrt(piv);
+      dk_s[0] = dk;
+      dinv_s[0] = 1.0 / dk;
+      if (!(piv > 0.0)) bad_s = 0;
+    }
   }
   __syncthreads();
 
-  int bad = -1;
   for (int k = 0; k < NB; ++k) {
-    const double* col = col_s[k & 1];
-    const double piv = col[k];
-    const double dk = sqrt(piv);
-    const double inv = 1.0 / dk;
-    // The logarithms are taken after the sweep, off the per-column critical path.
-    if (t == 0) {
-      if (bad < 0 && !(piv > 0.0)) bad = k;
-      dk_s[k] = dk;
-      dinv_s[k] = inv;
-    }
-    // Every update at step k touches column k or a column > k.
+    // Every update at step k touches column k or a column > k; the test is
+    // uniform over each half-wave (one column block).
     if (cb + EB - 1 >= k) {
-      double sc_c[EB];
+      const double* col = col_s[k & 1];
+      const double dk = dk_s[k];
+      const double inv = dinv_s[k];
+      double sci[EB], scc[EB];
+#pragma unroll
+      for (int a = 0; a < EB; ++a) {
+        const double v = col[rb + a] * inv;
+        sci[a] = (rb + a == k) ? inv : v;
+      }
 #pragma unroll
       for (int b = 0; b < EB; ++b) {
-        const int j = cb + b;
-        sc_c[b] = (j == k) ? inv : col[j] * inv;
+        const double v = col[cb + b] * inv;
+        scc[b] = (cb + b == k) ? inv : v;
       }
 #pragma unroll
       for (int a = 0; a < EB; ++a) {
         const int i = rb + a;
-        const double sci = (i == k) ? inv : col[i] * inv;
 #pragma unroll
         for (int b = 0; b < EB; ++b) {
           const int j = cb + b;
-          if (j == k) {
-            // column k: L[i][k] below the diagonal, X[k][i] (stored at (i, k)) above
-            s[a][b] = (i == k) ? dk : sci;
-          } else if (j > k && (i >= j || i <= k)) {
-            // lower part: trailing update L[i][k] L[j][k];
-            // strict upper (i, j), i <= k: numerator of X[j][i] -= L[j][k] X[k][i]
-            s[a][b] -= sci * sc_c[b];
-          }
+          // column k: L[i][k] below the diagonal, X[k][i] (stored at (i, k)) above;
+          // lower part (i >= j > k): trailing update L[i][k] L[j][k];
+          // strict upper (i, j), i <= k < j: numerator of X[j][i] -= L[j][k] X[k][i]
+          const double u = s[a][b] - sci[a] * scc[b];
+          const double ck = (i == k) ? dk : sci[a];
+          const bool upd = j > k && (i >= j || i <= k);
+          s[a][b] = (j == k) ? ck : (upd ? u : s[a][b]);
         }
       }
-    }
-    if (k + 1 < NB && cb <= k + 1 && k + 1 < cb + EB) {
+      // Publish column k+1 and, from the owner of its diagonal entry, its pivot.
+      const int q = k + 1 - cb;
+      if (k + 1 < NB && q >= 0 && q < EB) {
+        double v[EB];
 #pragma unroll
-      for (int b = 0; b < EB; ++b) {
-        if (cb + b == k + 1) {
+        for (int a = 0; a < EB; ++a) {
+          v[a] = s[a][0];
 #pragma unroll
-          for (int a = 0; a < EB; ++a) col_s[(k + 1) & 1][rb + a] = s[a][b];
+          for (int b = 1; b < EB; ++b) v[a] = (q == b) ? s[a][b] : v[a];
+          col_s[(k + 1) & 1][rb + a] = v[a];
+        }
+        const int p = k + 1 - rb;
+        if (p >= 0 && p < EB) {
+          double piv = v[0];
+#pragma unroll
+          for (int a = 1; a < EB; ++a) piv = (p == a) ? v[a] : piv;
+          const double dkn = sqrt(piv);
+          dk_s[k + 1] = dkn;
+          dinv_s[k + 1] = 1.0 / dkn;
+          if (!(piv > 0.0) && k + 1 < bad_s) bad_s = k + 1;
         }
       }
     }
     __syncthreads();
   }
 
-  // L out (lower incl. diagonal), and X^T staged in LDS for a coalesced write.
+  // L (lower incl. diagonal) and X (from the upper part, transposed) staged
+  // in LDS for row-contiguous writes.
 #pragma unroll
-  for (int a = 0; a < EB; ++a) {
-    TS* row = A + (o + rb + a) * ld + o + cb;
+  for (int a = 0; a < EB; ++a)
 #pragma unroll
-    for (int b = 0; b < EB; ++b) {
-      const int i = rb + a, j = cb + b;
-      row[b] = (TS)((j <= i) ? s[a][b] : 0.0);
-      if (j > i) tr_s[j][i] = s[a][b];
-    }
-  }
+    for (int b = 0; b < EB; ++b) tr_s[rb + a][cb + b] = s[a][b];
   if (t < NB) dk_s[t] = log(dk_s[t]);
   __syncthreads();
   for (int e = t; e < NB * NB; e += LEAF_THREADS) {
     const int i = e / NB, j = e % NB;
-    const double v = (j < i) ? tr_s[i][j] : (j == i ? dinv_s[i] : 0.0);
+    A[(o + i) * ld + o + j] = (TS)((j <= i) ? tr_s[i][j] : 0.0);
+    const double v = (j < i) ? tr_s[j][i] : (j == i ? dinv_s[i] : 0.0);
     X[(o + i) * ld + o + j] = (TS)v;
     leafinv[e] = v;
   }
@@ -142,7 +161,7 @@ __global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_
     double ld_acc = 0.0;  // in column order, as the sweep produced the pivots
     for (int k = 0; k < NB; ++k) ld_acc += dk_s[k];
     *logdet = 2.0 * ld_acc;
-    *info = bad >= 0 ? (int)(o + bad + 1) : 0;
+    *info = bad_s < NB ? (int)(o + bad_s + 1) : 0;
   }
 }
 
