@@ -26,7 +26,8 @@ namespace gpk {
 namespace {
 
 template <int AR, typename TS, int DMAX>
-__global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __restrict__ X,
+__global__ __launch_bounds__(WaveLayout<AR>::NT) void adjoint_kernel(int kind,
+                                                                     const TS* __restrict__ X,
                                                       int64_t ld, int64_t np,
                                                       const double* __restrict__ x, int64_t n,
                                                       int d, const double* __restrict__ theta,
@@ -34,6 +35,8 @@ __global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __rest
                                                       const double* __restrict__ beta, int dense,
                                                       double* __restrict__ partial, Batch bt) {
   typedef typename AccT<AR>::scalar AS;
+  typedef WaveLayout<AR> WL;
+  constexpr int NS = WL::NJ * 4;  // accumulator slots per lane and 16-row block
   bmember(bt, d, x, n, theta);
   X = bws(X, bt);
   alpha = bws(alpha, bt);
@@ -42,16 +45,16 @@ __global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __rest
   constexpr size_t ML = smem_bytes<AR>();
   constexpr size_t XS = (size_t)2 * TILE * (DMAX + 1) * sizeof(double);
   constexpr size_t AB = (size_t)4 * TILE * sizeof(double);
-  constexpr size_t WB = (size_t)4 * 16 * 64 * sizeof(AS);
+  constexpr size_t WB = (size_t)WL::NW * NS * 64 * sizeof(AS);
   constexpr size_t EP = XS + AB + WB;
   __shared__ __align__(16) char smem[ML > EP ? ML : EP];
-  __shared__ double red_s[4][DMAX + 2];
+  __shared__ double red_s[WL::NW][DMAX + 2];
 
   int64_t mt, nt;
   lower_tile(blockIdx.x, mt, nt);
   const int64_t m0 = mt * TILE, n0 = nt * TILE;
   const int64_t kb = dense ? 0 : m0;
-  typename AccT<AR>::vec acc[4][4];
+  typename AccT<AR>::vec acc[4][WL::NJ];
   if constexpr (AR == AR_F64) {
     mainloop_f64<TS, TS, true, true>(X, ld, X, ld, m0, n0, kb, np, acc, smem);
   } else {
@@ -68,9 +71,9 @@ __global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __rest
   double* an = am + TILE;                              // [TILE]
   double* bm = an + TILE;                              // [TILE]
   double* bn = bm + TILE;                              // [TILE]
-  AS* wbuf = reinterpret_cast<AS*>(bn + TILE);         // [4 waves][16][64 lanes]
+  AS* wbuf = reinterpret_cast<AS*>(bn + TILE);         // [waves][NS][64 lanes]
   const int t = threadIdx.x;
-  for (int e = t; e < TILE * DMAX; e += 256) {
+  for (int e = t; e < TILE * DMAX; e += WL::NT) {
     const int r = e / DMAX, q = e % DMAX;
     if (q < d) {
       const double l = theta[nls == 1 ? 0 : q];
@@ -90,27 +93,27 @@ __global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __rest
   __syncthreads();
 
   const int lane = t & 63, w = t >> 6;
-  const int wm = w >> 1, wn = w & 1;
+  const int wm = w / WL::WN, wn = w % WL::WN;
   const double wt = (mt == nt) ? 1.0 : 2.0;
   const bool ard = nls > 1;
   double s_var = 0.0, s_tr = 0.0;
   double s_ls[DMAX];
 #pragma unroll
   for (int q = 0; q < DMAX; ++q) s_ls[q] = 0.0;
-  AS* mine = wbuf + w * 16 * 64 + lane;
+  AS* mine = wbuf + w * NS * 64 + lane;
 
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    // Only this lane reads these 16 slots back, so no barrier is needed.
+    // Only this lane reads these NS slots back, so no barrier is needed.
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < WL::NJ; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) mine[(j * 4 + r) * 64] = acc[i][j][r];
 #pragma unroll 1
-    for (int e = 0; e < 16; ++e) {
+    for (int e = 0; e < NS; ++e) {
       const int j = e >> 2, r = e & 3;
       const int li = wm * 64 + i * 16 + acc_row<AR>(lane, r);
-      const int lj = wn * 64 + j * 16 + (lane & 15);
+      const int lj = wn * WL::WC + j * 16 + (lane & 15);
       const int64_t gi = m0 + li, gj = n0 + lj;
       if (gi >= n || gj >= n) continue;
       const double wij = (beta ? am[li] * bn[lj] + bm[li] * an[lj] : am[li] * an[lj]) -
@@ -145,7 +148,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __rest
     }
   }
 
-  // Workgroup reduction: wave shuffles, then across the four waves.
+  // Workgroup reduction: wave shuffles, then across the waves in order.
   const int nv = nls + 2;
 #pragma unroll
   for (int q = 0; q < DMAX + 2; ++q) {
@@ -157,7 +160,9 @@ __global__ __launch_bounds__(256) void adjoint_kernel(int kind, const TS* __rest
   __syncthreads();
   if (t < nv) {
     const int q = t < nls ? t : (t == nls ? DMAX : DMAX + 1);
-    const double v = red_s[0][q] + red_s[1][q] + red_s[2][q] + red_s[3][q];
+    double v = red_s[0][q];
+#pragma unroll
+    for (int u = 1; u < WL::NW; ++u) v += red_s[u][q];
     partial[(int64_t)blockIdx.x * nv + t] = v;
   }
 }
@@ -169,8 +174,8 @@ int launch_ar(int kind, const TS* X, int64_t ld, int64_t np, const double* x, in
   const int64_t nt = np / TILE;
   const dim3 grid((unsigned)(nt * (nt + 1) / 2), 1, bt.count);
 #define GPK_ADJ(DM)                                                                        \
-  hipLaunchKernelGGL((adjoint_kernel<AR, TS, DM>), grid, dim3(256), 0, s, kind, X, ld, np, x, \
-                     n, d, theta, nls, alpha, beta, dense ? 1 : 0, partial, bt)
+  hipLaunchKernelGGL((adjoint_kernel<AR, TS, DM>), grid, dim3(WaveLayout<AR>::NT), 0, s, kind, \
+                     X, ld, np, x, n, d, theta, nls, alpha, beta, dense ? 1 : 0, partial, bt)
   if (d <= 1) GPK_ADJ(1);
   else if (d <= 2) GPK_ADJ(2);
   else if (d <= 4) GPK_ADJ(4);

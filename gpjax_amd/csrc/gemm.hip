@@ -25,7 +25,8 @@ struct GemmParams {
 };
 
 template <int AR, typename TA, typename TB, typename TC, bool A_KM, bool B_KN>
-__global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
+__global__ __launch_bounds__(WaveLayout<AR>::NT) void gemm_kernel(GemmParams p) {
+  typedef WaveLayout<AR> WL;
   __shared__ __align__(16) char smem[smem_bytes<AR>()];
   p.A = bws(p.A, p.bt);
   p.B = bws(p.B, p.bt);
@@ -42,7 +43,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
   int64_t kb, ke;
   k_range(p.krange, mt, nt, p.K, kb, ke);
   const int64_t m0 = mt * TILE, n0 = nt * TILE;
-  typename AccT<AR>::vec acc[4][4];
+  typename AccT<AR>::vec acc[4][WL::NJ];
   if constexpr (AR == AR_F64) {
     mainloop_f64<TA, TB, A_KM, B_KN>(static_cast<const TA*>(p.A), p.lda,
                                      static_cast<const TB*>(p.B), p.ldb, m0, n0, kb, ke, acc,
@@ -54,17 +55,17 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
   }
   TC* C = static_cast<TC*>(p.C);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int wm = w >> 1, wn = w & 1;
+  const int wm = w / WL::WN, wn = w % WL::WN;
   if (p.cdiag && p.lower_tiles && mt == nt) {
     double* cd = p.cdiag + mt * TILE * TILE;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < WL::NJ; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = wm * 64 + i * 16 + acc_row<AR>(lane, r);
-          const int colx = wn * 64 + j * 16 + (lane & 15);
+          const int colx = wn * WL::WC + j * 16 + (lane & 15);
           double* c = cd + row * TILE + colx;
           double v = p.alpha * (double)acc[i][j][r];
           if (p.beta_one) v += *c;
@@ -75,11 +76,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < WL::NJ; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t row = m0 + wm * 64 + i * 16 + acc_row<AR>(lane, r);
-        const int64_t colx = n0 + wn * 64 + j * 16 + (lane & 15);
+        const int64_t colx = n0 + wn * WL::WC + j * 16 + (lane & 15);
         TC* c = C + row * p.ldc + colx;
         double v = p.alpha * (double)acc[i][j][r];
         if (p.beta_one) v += (double)*c;
@@ -96,7 +97,8 @@ int launch_t(const GemmParams& p, int64_t M, int64_t N, hipStream_t s) {
   } else {
     grid = dim3((unsigned)(N / TILE), (unsigned)(M / TILE), p.bt.count);
   }
-  hipLaunchKernelGGL((gemm_kernel<AR, TA, TB, TC, A_KM, B_KN>), grid, dim3(256), 0, s, p);
+  hipLaunchKernelGGL((gemm_kernel<AR, TA, TB, TC, A_KM, B_KN>), grid,
+                     dim3(WaveLayout<AR>::NT), 0, s, p);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }

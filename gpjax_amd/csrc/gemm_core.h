@@ -1,11 +1,17 @@
 // Tile main loop shared by the GEMM kernels and the fused adjoint kernel.
 //
-// One 256-thread workgroup computes a 128 x 128 tile of
+// One workgroup computes a 128 x 128 tile of
 //   acc(m, n) = sum_{k in [kb, ke)} A(m, k) * B(n, k)
-// with four waves in a 2 x 2 arrangement, 64 x 64 per wave, on the CDNA4
-// matrix cores:
-//   AR_F64    : v_mfma_f64_16x16x4f64, operands staged in LDS as float64,
+// on the CDNA4 matrix cores (wave arrangement in WaveLayout):
+//   AR_F64    : 256 threads, four waves in 2 x 2, 64 x 64 per wave;
+//               v_mfma_f64_16x16x4f64, operands staged in LDS as float64,
 //               k-slabs of 16, double-buffered LDS (one barrier per slab).
+//   bf16 loops: 512 threads, eight waves in 2 x 4, 64 x 32 per wave, so two
+//               waves share each SIMD and one's bf16 split and LDS traffic
+//               overlaps the other's matrix-core work (the 64 x 64 per-wave
+//               form needs ~450 registers and leaves one wave per SIMD; the
+//               two-per-SIMD form measured 11-15 % faster on the same tiles,
+//               with identical results).
 //   AR_BF16X3 : fp32 operands split on load into bf16 hi + lo planes;
 //               products hi*hi + hi*lo + lo*hi (Henry, Tang & Heinecke,
 //               ARITH 2019) on v_mfma_f32_16x16x32_bf16.
@@ -44,15 +50,27 @@ struct AccT {
   typedef double scalar;
 };
 
+// Waves of one tile: NT threads, NW waves of which WN along n; each wave owns
+// rows wm*64 .. +64 (four 16-row blocks) and columns wn*WC .. +WC (NJ 16-column
+// blocks), wm = w / WN, wn = w % WN.
+template <int AR>
+struct WaveLayout {
+  static constexpr int NT = AR == AR_F64 ? 256 : 512;
+  static constexpr int NW = NT / 64;
+  static constexpr int WN = AR == AR_F64 ? 2 : 4;
+  static constexpr int WC = TILE / WN;
+  static constexpr int NJ = WC / 16;
+};
+
 // ---- bf16 split: a = p0 + p1 (+ p2) + remainder, each plane rounded to
 // nearest even by the hardware conversion (v_cvt_pk_bf16_f32, two values per
-// instruction); the residual a - p is exact in fp32.  Sixteen values become
-// NP planes of sixteen bf16, packed as two uint4 per plane.
+// instruction); the residual a - p is exact in fp32.  Eight values become NP
+// planes of eight bf16, one uint4 per plane.
 template <int NP>
-__device__ __forceinline__ void split_pack16(const float (&v)[16], uint4 (&out)[NP][2]) {
-  unsigned wq[NP][8];
+__device__ __forceinline__ void split_pack8(const float (&v)[8], uint4 (&out)[NP]) {
+  unsigned wq[NP][4];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {
+  for (int e = 0; e < 4; ++e) {
     f32x2 r = {v[2 * e], v[2 * e + 1]};
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
@@ -65,10 +83,7 @@ __device__ __forceinline__ void split_pack16(const float (&v)[16], uint4 (&out)[
     }
   }
 #pragma unroll
-  for (int q = 0; q < NP; ++q) {
-    out[q][0] = make_uint4(wq[q][0], wq[q][1], wq[q][2], wq[q][3]);
-    out[q][1] = make_uint4(wq[q][4], wq[q][5], wq[q][6], wq[q][7]);
-  }
+  for (int q = 0; q < NP; ++q) out[q] = make_uint4(wq[q][0], wq[q][1], wq[q][2], wq[q][3]);
 }
 
 // ---- LDS footprint of one main loop ----
@@ -76,7 +91,7 @@ constexpr int F64_BK = 16;
 constexpr int F64_LDP = TILE + 1;  // padded row length (doubles)
 constexpr size_t F64_SMEM = 2 /*A,B*/ * 2 /*buffers*/ * F64_BK * F64_LDP * sizeof(double);
 constexpr int BF_BK = 32;
-constexpr int BF_LDK = BF_BK + 8;  // padded k length (bf16)
+constexpr int BF_LDK = BF_BK + 16;  // padded k length (bf16): 96-byte rows measured fastest
 constexpr int BF_CHUNK = 4;        // slabs per fp32 chunk: 128 k
 template <int AR>
 constexpr int bf_planes() { return AR == AR_BF16X6 ? 3 : 2; }
@@ -184,7 +199,8 @@ __device__ __forceinline__ void mainloop_f64(const TA* __restrict__ A, int64_t l
 // Software pipeline over k-slabs of BF_BK with two LDS buffers and one
 // barrier per slab: while the matrix cores work on slab s (buffer s&1), the
 // registers holding slab s+1 are split into bf16 planes and stored into
-// buffer (s+1)&1, and the global loads of slab s+2 are in flight.
+// buffer (s+1)&1, and the global loads of slab s+2 are in flight.  Eight waves
+// (two per SIMD) interleave these phases in hardware; no scheduling hints.
 //
 // Accumulation: the leading product hi*hi goes into its own fp32
 // accumulators (one fp32 rounding per slab), the correction products, 2^-8
@@ -194,84 +210,59 @@ __device__ __forceinline__ void mainloop_f64(const TA* __restrict__ A, int64_t l
 // over 256 k, the rounding of the running sums dominated the error of the
 // whole mixed-precision factorisation (float64 products of the same float32
 // operands are ~1000x closer to the float64 result).
-// Instruction interleaving for one slab of the bf16 loop, as scheduling-group
-// hints (masks: 0x008 MFMA, 0x002 VALU, 0x100 DS read, 0x200 DS write,
-// 0x020 VMEM read).  With one wave per SIMD the split of the next slab can
-// only hide under this slab's MFMAs if the two are interleaved: the plane-0
-// fragment reads go first, then every MFMA is followed by either one more
-// fragment read or a few VALU instructions, with the DS writes of the next
-// slab and the global loads of the one after spread over the rest.
-template <int AR, bool A_KM, bool B_KN, bool FLUSH>
-__device__ __forceinline__ void sched_bf16_slab() {
-  constexpr int NP = bf_planes<AR>();
-  constexpr int NM = 16 * (AR == AR_BF16X6 ? 6 : 3);     // MFMAs
-  constexpr int NR = 8 * NP;                              // ds_read_b128
-  constexpr int NW = 4 * NP;                              // ds_write_b128
-  constexpr int NG = (A_KM ? 16 : 4) + (B_KN ? 16 : 4);  // global loads
-  constexpr int NV = FLUSH ? 5 : 3;                       // VALU per MFMA slot
-  constexpr int W0 = NR;                                  // first DS write slot
-  constexpr int WS = (NM - W0) / NW > 0 ? (NM - W0) / NW : 1;
-  constexpr int G0 = NM - NG > W0 ? NM - NG : W0;         // first VMEM slot
-#pragma unroll
-  for (int q = 0; q < NM; ++q) {
-    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-    __builtin_amdgcn_sched_group_barrier(0x002, NV, 0);
-  }
-}
-
 template <int AR, bool A_KM, bool B_KN>
 __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64_t lda,
                                               const float* __restrict__ B, int64_t ldb,
                                               int64_t m0, int64_t n0, int64_t kb, int64_t ke,
-                                              f64x4 (&acc64)[4][4], char* smem) {
+                                              f64x4 (&acc64)[4][WaveLayout<AR>::NJ],
+                                              char* smem) {
+  typedef WaveLayout<AR> WL;
+  constexpr int NJ = WL::NJ;
   constexpr int NP = bf_planes<AR>();
   constexpr int OPS = NP * TILE * BF_LDK;  // bf16 elements of one operand in one buffer
   unsigned short* S = reinterpret_cast<unsigned short*>(smem);  // [2][A, B][NP][TILE][LDK]
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wm = w >> 1, wn = w & 1;
+  const int wm = w / WL::WN, wn = w % WL::WN;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc64[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < NJ; ++j) acc64[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
   if (kb >= ke) return;
 
-  // Every thread stages 16 consecutive k of one row of each operand per slab.
-  // Row-contiguous operand: row = t/2, k = (t%2)*16 .. +15 (four 16-byte loads).
-  // Column-contiguous operand: row = t%128, k = (t/128)*16 .. +15 (sixteen
+  // Every thread stages 8 consecutive k of one row of each operand per slab.
+  // Row-contiguous operand: row = t/4, k = (t%4)*8 .. +7 (two 16-byte loads).
+  // Column-contiguous operand: row = t%128, k = (t/128)*8 .. +7 (eight
   // 4-byte loads; consecutive lanes read consecutive rows, 256 bytes per load).
-  float ra[16], rb[16];
+  float ra[8], rb[8];
   auto gload_one = [&](const float* __restrict__ P, int64_t ld, int64_t r0, bool km,
-                       int64_t k0, float (&r)[16]) {
+                       int64_t k0, float (&r)[8]) {
     if (!km) {
-      const float* p = P + (r0 + (t >> 1)) * ld + k0 + (t & 1) * 16;
+      const float* p = P + (r0 + (t >> 2)) * ld + k0 + (t & 3) * 8;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < 2; ++q) {
         const float4 v = *reinterpret_cast<const float4*>(p + 4 * q);
         r[4 * q] = v.x; r[4 * q + 1] = v.y; r[4 * q + 2] = v.z; r[4 * q + 3] = v.w;
       }
     } else {
-      const float* p = P + (k0 + (t >> 7) * 16) * ld + r0 + (t & 127);
+      const float* p = P + (k0 + (t >> 7) * 8) * ld + r0 + (t & 127);
 #pragma unroll
-      for (int q = 0; q < 16; ++q) r[q] = p[q * ld];
+      for (int q = 0; q < 8; ++q) r[q] = p[q * ld];
     }
   };
   auto gload = [&](int64_t k0) {
     gload_one(A, lda, m0, A_KM, k0, ra);
     gload_one(B, ldb, n0, B_KN, k0, rb);
   };
-  // Split into bf16 planes and store as two 16-byte writes per plane into the
+  // Split into bf16 planes and store one 16-byte write per plane into the
   // k-contiguous layout the matrix-core operands read.
-  auto sstore_one = [&](unsigned short* base, const float (&r)[16], bool km) {
-    const int row = km ? (t & 127) : (t >> 1);
-    const int ko = km ? (t >> 7) * 16 : (t & 1) * 16;
-    uint4 pk[NP][2];
-    split_pack16<NP>(r, pk);
+  auto sstore_one = [&](unsigned short* base, const float (&r)[8], bool km) {
+    const int row = km ? (t & 127) : (t >> 2);
+    const int ko = km ? (t >> 7) * 8 : (t & 3) * 8;
+    uint4 pk[NP];
+    split_pack8<NP>(r, pk);
 #pragma unroll
-    for (int pl = 0; pl < NP; ++pl) {
-      uint4* d = reinterpret_cast<uint4*>(base + (pl * TILE + row) * BF_LDK + ko);
-      d[0] = pk[pl][0];
-      d[1] = pk[pl][1];
-    }
+    for (int pl = 0; pl < NP; ++pl)
+      *reinterpret_cast<uint4*>(base + (pl * TILE + row) * BF_LDK + ko) = pk[pl];
   };
   auto sstore = [&](int buf) {
     sstore_one(S + buf * 2 * OPS, ra, A_KM);
@@ -284,45 +275,43 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
   gload(kb + BF_BK);
   __syncthreads();
   const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-  f32x4 accc[4][4];
+  f32x4 accc[4][NJ];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) accc[i][j] = zero;
+    for (int j = 0; j < NJ; ++j) accc[i][j] = zero;
   for (int64_t c0 = 0; c0 < nslab; c0 += BF_CHUNK) {
-    f32x4 acch[4][4];
+    f32x4 acch[4][NJ];
 #pragma unroll
     for (int u = 0; u < BF_CHUNK; ++u) {
       const unsigned short* As = S + (u & 1) * 2 * OPS;
       const unsigned short* Bs = As + OPS;
       // Fragments of planes 0 and 1; plane 2 (x6) later reuses plane 1's registers.
-      bf16x8 a0[4], b0[4], a1[4], b1[4];
+      bf16x8 a0[4], b0[NJ], a1[4], b1[NJ];
       auto frag = [&](const unsigned short* P, int pl, int base, int i) {
         return *reinterpret_cast<const bf16x8*>(
             P + (pl * TILE + base + i * 16 + (lane & 15)) * BF_LDK + 8 * (lane >> 4));
       };
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        a0[i] = frag(As, 0, wm * 64, i);
-        b0[i] = frag(Bs, 0, wn * 64, i);
-      }
+      for (int i = 0; i < 4; ++i) a0[i] = frag(As, 0, wm * 64, i);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        a1[i] = frag(As, 1, wm * 64, i);
-        b1[i] = frag(Bs, 1, wn * 64, i);
-      }
+      for (int j = 0; j < NJ; ++j) b0[j] = frag(Bs, 0, wn * WL::WC, j);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a1[i] = frag(As, 1, wm * 64, i);
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) b1[j] = frag(Bs, 1, wn * WL::WC, j);
       // Leading products first: they need only plane 0, and at the chunk's
       // last slab their float64 flush overlaps the correction products.
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < NJ; ++j)
           acch[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[i], b0[j],
                                                                u == 0 ? zero : acch[i][j], 0, 0, 0);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < NJ; ++j) {
           f32x4 c = accc[i][j];
           if (AR == AR_BF16X6) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[i], b1[j], c, 0, 0, 0);
           c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[i], b0[j], c, 0, 0, 0);
@@ -331,14 +320,13 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
         }
       if (AR == AR_BF16X6) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          a1[i] = frag(As, 2, wm * 64, i);
-          b1[i] = frag(Bs, 2, wn * 64, i);
-        }
+        for (int i = 0; i < 4; ++i) a1[i] = frag(As, 2, wm * 64, i);
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) b1[j] = frag(Bs, 2, wn * WL::WC, j);
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
+          for (int j = 0; j < NJ; ++j) {
             f32x4 c = accc[i][j];
             c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[i], b0[j], c, 0, 0, 0);
             c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[i], b1[j], c, 0, 0, 0);
@@ -349,7 +337,7 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-          for (int j = 0; j < 4; ++j)
+          for (int j = 0; j < NJ; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)acch[i][j][r];
       }
@@ -358,8 +346,6 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
       sstore((u + 1) & 1);
       const int64_t s2 = c0 + u + 2 < nslab ? c0 + u + 2 : nslab - 1;
       gload(kb + s2 * BF_BK);
-      if (u == BF_CHUNK - 1) sched_bf16_slab<AR, A_KM, B_KN, true>();
-      else sched_bf16_slab<AR, A_KM, B_KN, false>();
       __syncthreads();
     }
   }
@@ -368,7 +354,7 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < NJ; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)accc[i][j][r];
 }
