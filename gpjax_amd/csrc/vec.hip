@@ -52,7 +52,11 @@ __global__ void trmv_n_reduce(const double* __restrict__ partial, int64_t np,
   y[i] = s;
 }
 
-// y_k partial over rows [c*RC, (c+1)*RC), i >= k.  One thread per column.
+// y_k partial over rows [c*RC, (c+1)*RC), i >= k.  64 columns per workgroup;
+// its four waves take every fourth row (each wave reads 64 consecutive
+// columns of a row, 512 or 256 bytes), so four times as many row loads are in
+// flight per column, and the four row sums are added in a fixed order.
+constexpr int TRMV_TC = 64;  // columns per workgroup, transposed
 template <typename TS>
 __global__ __launch_bounds__(256) void trmv_t_kernel(const TS* __restrict__ X, int64_t ld,
                                                      int64_t np, const double* __restrict__ r,
@@ -61,18 +65,22 @@ __global__ __launch_bounds__(256) void trmv_t_kernel(const TS* __restrict__ X, i
   r = bws(r, bt);
   partial = bws(partial, bt);
   const int64_t c = blockIdx.y;
-  const int64_t col0 = (int64_t)blockIdx.x * 256;
+  const int64_t col0 = (int64_t)blockIdx.x * TRMV_TC;
   const int64_t i0 = c * TRMV_RC;
   const int64_t i1 = (i0 + TRMV_RC < np) ? i0 + TRMV_RC : np;
   if (i1 <= col0) return;
   __shared__ double rs[TRMV_RC];
+  __shared__ double red[4][TRMV_TC];
   for (int e = threadIdx.x; e < TRMV_RC; e += 256) rs[e] = (i0 + e < np) ? r[i0 + e] : 0.0;
   __syncthreads();
-  const int64_t k = col0 + threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t k = col0 + lane;
   double acc = 0.0;
   const int64_t ib = (k > i0) ? k : i0;
-  for (int64_t i = ib; i < i1; ++i) acc += (double)X[i * ld + k] * rs[i - i0];
-  partial[c * np + k] = acc;
+  for (int64_t i = ib + w; i < i1; i += 4) acc += (double)X[i * ld + k] * rs[i - i0];
+  red[w][lane] = acc;
+  __syncthreads();
+  if (w == 0) partial[c * np + k] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
 
 __global__ void trmv_t_reduce(const double* __restrict__ partial, int64_t np, int64_t nchunks,
@@ -228,7 +236,7 @@ int launch_trmv(const TS* X, int64_t ld, int64_t np, const double* r, double* y,
     hipLaunchKernelGGL(trmv_n_reduce, gvec, dim3(256), 0, s, partial, np, y, bt);
   } else {
     const int64_t nc = (np + TRMV_RC - 1) / TRMV_RC;
-    hipLaunchKernelGGL(trmv_t_kernel<TS>, dim3((unsigned)(np / 256), (unsigned)nc, bt.count),
+    hipLaunchKernelGGL(trmv_t_kernel<TS>, dim3((unsigned)(np / TRMV_TC), (unsigned)nc, bt.count),
                        dim3(256), 0, s, X, ld, np, r, partial, bt);
     GPK_LAUNCH_CHECK();
     hipLaunchKernelGGL(trmv_t_reduce, gvec, dim3(256), 0, s, partial, np, nc, y, bt);
