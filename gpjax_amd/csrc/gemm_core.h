@@ -1,17 +1,15 @@
 // Tile main loop shared by the GEMM kernels and the fused adjoint kernel.
 //
-// One workgroup computes a 128 x 128 tile of
+// One 512-thread workgroup computes a 128 x 128 tile of
 //   acc(m, n) = sum_{k in [kb, ke)} A(m, k) * B(n, k)
-// on the CDNA4 matrix cores (wave arrangement in WaveLayout):
-//   AR_F64    : 256 threads, four waves in 2 x 2, 64 x 64 per wave;
-//               v_mfma_f64_16x16x4f64, operands staged in LDS as float64,
+// on the CDNA4 matrix cores with eight waves in 2 x 4, 64 x 32 per wave
+// (WaveLayout), so two waves share each SIMD and one's staging work (loads,
+// bf16 split, LDS stores) overlaps the other's matrix-core work.  The earlier
+// 64 x 64-per-wave form needs ~450 registers (bf16) or ~310 (FP64) and leaves
+// one wave per SIMD; the two-per-SIMD form measured 11-15 % faster (bf16) and
+// 1-7 % faster (FP64) on the same tiles, with identical results.
+//   AR_F64    : v_mfma_f64_16x16x4f64, operands staged in LDS as float64,
 //               k-slabs of 16, double-buffered LDS (one barrier per slab).
-//   bf16 loops: 512 threads, eight waves in 2 x 4, 64 x 32 per wave, so two
-//               waves share each SIMD and one's bf16 split and LDS traffic
-//               overlaps the other's matrix-core work (the 64 x 64 per-wave
-//               form needs ~450 registers and leaves one wave per SIMD; the
-//               two-per-SIMD form measured 11-15 % faster on the same tiles,
-//               with identical results).
 //   AR_BF16X3 : fp32 operands split on load into bf16 hi + lo planes;
 //               products hi*hi + hi*lo + lo*hi (Henry, Tang & Heinecke,
 //               ARITH 2019) on v_mfma_f32_16x16x32_bf16.
@@ -55,9 +53,9 @@ struct AccT {
 // blocks), wm = w / WN, wn = w % WN.
 template <int AR>
 struct WaveLayout {
-  static constexpr int NT = AR == AR_F64 ? 256 : 512;
+  static constexpr int NT = 512;
   static constexpr int NW = NT / 64;
-  static constexpr int WN = AR == AR_F64 ? 2 : 4;
+  static constexpr int WN = 4;
   static constexpr int WC = TILE / WN;
   static constexpr int NJ = WC / 16;
 };
@@ -102,32 +100,24 @@ constexpr size_t smem_bytes() {
                             sizeof(unsigned short);
 }
 
-template <typename T>
-__device__ __forceinline__ double to_f64(T v) { return (double)v; }
-
-// Load 8 consecutive elements as float64.
-template <typename T>
-__device__ __forceinline__ void load8(const T* __restrict__ p, double (&v)[8]) {
-#pragma unroll
-  for (int q = 0; q < 8; ++q) v[q] = (double)p[q];
-}
-
 // One 16-deep k-slab of FP64 matrix-core products from LDS operands stored
-// k-major: a[k * F64_LDP + m], b[k * F64_LDP + n] (F64_LDP defined below).
+// k-major: a[k * ldp + m], b[k * ldp + n]; the wave owns rows wm*64 .. +64 and
+// columns wn*16*NJ .. +16*NJ.
+template <int NJ = 4>
 __device__ __forceinline__ void mfma_slab_f64(const double* a, const double* b, int ldp,
-                                              int lane, int wm, int wn, f64x4 (&acc)[4][4]) {
+                                              int lane, int wm, int wn, f64x4 (&acc)[4][NJ]) {
 #pragma unroll
   for (int kk = 0; kk < 4; ++kk) {
     const int kr = kk * 4 + (lane >> 4);
-    double av[4], bv[4];
+    double av[4], bv[NJ];
 #pragma unroll
     for (int i = 0; i < 4; ++i) av[i] = a[kr * ldp + wm * 64 + i * 16 + (lane & 15)];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) bv[j] = b[kr * ldp + wn * 64 + j * 16 + (lane & 15)];
+    for (int j = 0; j < NJ; ++j) bv[j] = b[kr * ldp + wn * 16 * NJ + j * 16 + (lane & 15)];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < NJ; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
   }
 }
@@ -139,43 +129,50 @@ template <typename TA, typename TB, bool A_KM, bool B_KN>
 __device__ __forceinline__ void mainloop_f64(const TA* __restrict__ A, int64_t lda,
                                              const TB* __restrict__ B, int64_t ldb, int64_t m0,
                                              int64_t n0, int64_t kb, int64_t ke,
-                                             f64x4 (&acc)[4][4], char* smem) {
+                                             f64x4 (&acc)[4][WaveLayout<AR_F64>::NJ],
+                                             char* smem) {
+  typedef WaveLayout<AR_F64> WL;
+  constexpr int NJ = WL::NJ;
   double* As = reinterpret_cast<double*>(smem);                     // [2][BK][LDP]
   double* Bs = As + 2 * F64_BK * F64_LDP;                           // [2][BK][LDP]
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wm = w >> 1, wn = w & 1;
+  const int wm = w / WL::WN, wn = w % WL::WN;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < NJ; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
   if (kb >= ke) return;
 
-  // Global -> register staging: 8 elements per thread per operand.
-  // Row-contiguous operand: row = t/2, k offset (t%2)*8.
-  // Column-contiguous operand: k = t/16, row offset (t%16)*8.
-  double ra[8], rb[8];
+  // Global -> register staging: 4 elements per thread per operand (512 threads).
+  // Row-contiguous operand: row = t/4, k offset (t%4)*4.
+  // Column-contiguous operand: k = t/32, row offset (t%32)*4.
+  double ra[4], rb[4];
+  auto load4 = [](const auto* __restrict__ p, double (&v)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = (double)p[q];
+  };
   auto gload = [&](int64_t k0) {
-    if (!A_KM) load8(A + (m0 + (t >> 1)) * lda + k0 + (t & 1) * 8, ra);
-    else load8(A + (k0 + (t >> 4)) * lda + m0 + (t & 15) * 8, ra);
-    if (!B_KN) load8(B + (n0 + (t >> 1)) * ldb + k0 + (t & 1) * 8, rb);
-    else load8(B + (k0 + (t >> 4)) * ldb + n0 + (t & 15) * 8, rb);
+    if (!A_KM) load4(A + (m0 + (t >> 2)) * lda + k0 + (t & 3) * 4, ra);
+    else load4(A + (k0 + (t >> 5)) * lda + m0 + (t & 31) * 4, ra);
+    if (!B_KN) load4(B + (n0 + (t >> 2)) * ldb + k0 + (t & 3) * 4, rb);
+    else load4(B + (k0 + (t >> 5)) * ldb + n0 + (t & 31) * 4, rb);
   };
   auto sstore = [&](int buf) {
     double* a = As + buf * F64_BK * F64_LDP;
     double* b = Bs + buf * F64_BK * F64_LDP;
     if (!A_KM) {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) a[((t & 1) * 8 + q) * F64_LDP + (t >> 1)] = ra[q];
+      for (int q = 0; q < 4; ++q) a[((t & 3) * 4 + q) * F64_LDP + (t >> 2)] = ra[q];
     } else {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) a[(t >> 4) * F64_LDP + (t & 15) * 8 + q] = ra[q];
+      for (int q = 0; q < 4; ++q) a[(t >> 5) * F64_LDP + (t & 31) * 4 + q] = ra[q];
     }
     if (!B_KN) {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) b[((t & 1) * 8 + q) * F64_LDP + (t >> 1)] = rb[q];
+      for (int q = 0; q < 4; ++q) b[((t & 3) * 4 + q) * F64_LDP + (t >> 2)] = rb[q];
     } else {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) b[(t >> 4) * F64_LDP + (t & 15) * 8 + q] = rb[q];
+      for (int q = 0; q < 4; ++q) b[(t >> 5) * F64_LDP + (t & 31) * 4 + q] = rb[q];
     }
   };
 
@@ -186,8 +183,8 @@ __device__ __forceinline__ void mainloop_f64(const TA* __restrict__ A, int64_t l
   for (int64_t sl = 0; sl < nslab; ++sl) {
     const int buf = (int)(sl & 1);
     if (sl + 1 < nslab) gload(kb + (sl + 1) * F64_BK);
-    mfma_slab_f64(As + buf * F64_BK * F64_LDP, Bs + buf * F64_BK * F64_LDP, F64_LDP, lane, wm,
-                  wn, acc);
+    mfma_slab_f64<NJ>(As + buf * F64_BK * F64_LDP, Bs + buf * F64_BK * F64_LDP, F64_LDP, lane,
+                      wm, wn, acc);
     if (sl + 1 < nslab) sstore(buf ^ 1);
     __syncthreads();
   }

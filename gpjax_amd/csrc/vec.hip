@@ -75,10 +75,19 @@ __global__ __launch_bounds__(256) void trmv_t_kernel(const TS* __restrict__ X, i
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t k = col0 + lane;
-  double acc = 0.0;
   const int64_t ib = (k > i0) ? k : i0;
-  for (int64_t i = ib + w; i < i1; i += 4) acc += (double)X[i * ld + k] * rs[i - i0];
-  red[w][lane] = acc;
+  // Four independent sums per thread (rows i, i+4, i+8, i+12) keep four loads
+  // in flight; they are combined in a fixed order.
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int64_t i = ib + w;
+  for (; i + 12 < i1; i += 16) {
+    a0 += (double)X[i * ld + k] * rs[i - i0];
+    a1 += (double)X[(i + 4) * ld + k] * rs[i + 4 - i0];
+    a2 += (double)X[(i + 8) * ld + k] * rs[i + 8 - i0];
+    a3 += (double)X[(i + 12) * ld + k] * rs[i + 12 - i0];
+  }
+  for (; i < i1; i += 4) a0 += (double)X[i * ld + k] * rs[i - i0];
+  red[w][lane] = (a0 + a1) + (a2 + a3);
   __syncthreads();
   if (w == 0) partial[c * np + k] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
