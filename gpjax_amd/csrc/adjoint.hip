@@ -33,6 +33,7 @@ __global__ __launch_bounds__(WaveLayout<AR>::NT) void adjoint_kernel(int kind,
                                                       int d, const double* __restrict__ theta,
                                                       int nls, const double* __restrict__ alpha,
                                                       const double* __restrict__ beta, int dense,
+                                                      const float* __restrict__ fscale,
                                                       double* __restrict__ partial, Batch bt) {
   typedef typename AccT<AR>::scalar AS;
   typedef WaveLayout<AR> WL;
@@ -58,7 +59,9 @@ __global__ __launch_bounds__(WaveLayout<AR>::NT) void adjoint_kernel(int kind,
   if constexpr (AR == AR_F64) {
     mainloop_f64<TS, TS, true, true>(X, ld, X, ld, m0, n0, kb, np, acc, smem);
   } else {
-    mainloop_bf16<AR, true, true>(X, ld, X, ld, m0, n0, kb, np, acc, smem);
+    float sx = 1.f;
+    if constexpr (AR == AR_FP16X3) sx = fscale[0];
+    mainloop_bf16<AR, true, true, true>(X, ld, X, ld, m0, n0, kb, np, acc, smem, sx, sx);
   }
   __syncthreads();
 
@@ -170,12 +173,14 @@ __global__ __launch_bounds__(WaveLayout<AR>::NT) void adjoint_kernel(int kind,
 template <int AR, typename TS>
 int launch_ar(int kind, const TS* X, int64_t ld, int64_t np, const double* x, int64_t n, int d,
               const double* theta, int nls, const double* alpha, double* partial,
-              hipStream_t s, const Batch& bt, const double* beta, bool dense) {
+              hipStream_t s, const Batch& bt, const double* beta, bool dense,
+              const float* fscale) {
   const int64_t nt = np / TILE;
   const dim3 grid((unsigned)(nt * (nt + 1) / 2), 1, bt.count);
 #define GPK_ADJ(DM)                                                                        \
   hipLaunchKernelGGL((adjoint_kernel<AR, TS, DM>), grid, dim3(WaveLayout<AR>::NT), 0, s, kind, \
-                     X, ld, np, x, n, d, theta, nls, alpha, beta, dense ? 1 : 0, partial, bt)
+                     X, ld, np, x, n, d, theta, nls, alpha, beta, dense ? 1 : 0, fscale, partial, \
+                     bt)
   if (d <= 1) GPK_ADJ(1);
   else if (d <= 2) GPK_ADJ(2);
   else if (d <= 4) GPK_ADJ(4);
@@ -193,7 +198,7 @@ template <typename TS>
 int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, const double* x,
                    int64_t n, int d, const double* theta, int nls, const double* alpha,
                    double* partial, hipStream_t s, const Batch& bt, const double* beta,
-                   bool dense) {
+                   bool dense, const float* fscale) {
   if (d < 1 || d > MAX_D) {
     set_error("adjoint: input dimension %d outside [1, %d]", d, MAX_D);
     return GPK_ERR_BAD_ARG;
@@ -204,14 +209,22 @@ int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, con
   }
   if (arith == AR_F64)
     return launch_ar<AR_F64, TS>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s, bt,
-                                 beta, dense);
+                                 beta, dense, fscale);
   if constexpr (std::is_same<TS, float>::value) {
     if (arith == AR_BF16X3)
       return launch_ar<AR_BF16X3, float>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s,
-                                         bt, beta, dense);
+                                         bt, beta, dense, fscale);
     if (arith == AR_BF16X6)
       return launch_ar<AR_BF16X6, float>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s,
-                                         bt, beta, dense);
+                                         bt, beta, dense, fscale);
+    if (arith == AR_FP16X3) {
+      if (!fscale) {
+        set_error("adjoint: fp16 products need the operand scale");
+        return GPK_ERR_BAD_ARG;
+      }
+      return launch_ar<AR_FP16X3, float>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s,
+                                         bt, beta, dense, fscale);
+    }
   }
   set_error("adjoint: unsupported arithmetic %d", arith);
   return GPK_ERR_BAD_ARG;
@@ -219,9 +232,11 @@ int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, con
 
 template int launch_adjoint<double>(int, int, const double*, int64_t, int64_t, const double*,
                                     int64_t, int, const double*, int, const double*, double*,
-                                    hipStream_t, const Batch&, const double*, bool);
+                                    hipStream_t, const Batch&, const double*, bool,
+                                    const float*);
 template int launch_adjoint<float>(int, int, const float*, int64_t, int64_t, const double*,
                                    int64_t, int, const double*, int, const double*, double*,
-                                   hipStream_t, const Batch&, const double*, bool);
+                                   hipStream_t, const Batch&, const double*, bool,
+                                   const float*);
 
 }  // namespace gpk

@@ -238,6 +238,26 @@ __global__ __launch_bounds__(256) void finalize_bwd_kernel(
 
 inline unsigned nblk(int64_t np) { return (unsigned)((np + 255) / 256); }
 
+// Power-of-two fp16 operand scales [sL, sL, sX, sT] (AR_FP16X3): each maps its
+// bound b to at most 2^14.  |L_ij| <= sqrt(Sigma_ii) = sqrt(var + sd^2 + jitter)
+// (also bounds every trailing Schur complement); |X_ij| <= ||Sigma^-1||^(1/2) <=
+// 1 / sqrt(sd^2 + jitter); |(L21 X11)_ij| <= bL bX.  Bounds are at least 1 (the
+// identity pad) and at most 2^14.
+__global__ void fp16_scale_kernel(const double* __restrict__ theta, int nls, double jitter,
+                                  float* __restrict__ fs) {
+  if (threadIdx.x != 0) return;
+  const double var = theta[nls], sd = theta[nls + 1];
+  auto clampb = [](double b) { return fmin(fmax(b, 1.0), 16384.0); };
+  const double bl = clampb(sqrt(var + sd * sd + jitter));
+  const double bx = clampb(1.0 / sqrt(sd * sd + jitter));
+  const double bt = clampb(bl * bx);
+  auto sc = [](double b) { return (float)ldexp(1.0, (int)floor(log2(16384.0 / b))); };
+  fs[0] = sc(bl);
+  fs[1] = sc(bl);
+  fs[2] = sc(bx);
+  fs[3] = sc(bt);
+}
+
 int check_common(int kind, int prec, const void* x, int64_t n, int64_t d, const void* theta,
                  int nls) {
   if (kind < GPK_KERNEL_RBF || kind > GPK_KERNEL_MATERN52) {
@@ -295,6 +315,12 @@ int mll_forward_t(int kind, int prec, const double* x, const double* y, int64_t 
   double* dblk = f64 ? nullptr : at<double>(ws, Lo.off_dblk);
   F.dblk = dblk;
   F.bt = bt;
+  if (!f64) {
+    float* fs = at<float>(ws, Lo.off_scal);
+    hipLaunchKernelGGL(fp16_scale_kernel, dim3(1), dim3(64), 0, s, theta, nls, jitter, fs);
+    GPK_LAUNCH_CHECK();
+    F.fscale = fs;
+  }
 
   hipLaunchKernelGGL(prep_r_kernel, gv, dim3(256), 0, s, y, theta, nls, n, np, r, bt);
   GPK_LAUNCH_CHECK();
@@ -343,7 +369,8 @@ int mll_backward_t(int kind, const double* x, int64_t n, int d, const double* th
   double* gpart = at<double>(ws, Lo.off_gpart);
   PhaseScope ps(GPK_PHASE_ADJOINT, s);
   GPK_TRY(launch_adjoint<TS>(kind, f64 ? (int)AR_F64 : g_ar_lauum, at<TS>(ws, Lo.off_X), np, np,
-                             x, n, d, theta, nls, at<double>(ws, Lo.off_alpha), gpart, s, bt));
+                             x, n, d, theta, nls, at<double>(ws, Lo.off_alpha), gpart, s, bt,
+                             nullptr, false, f64 ? nullptr : at<float>(ws, Lo.off_scal) + 2));
   const int64_t nleaf = np / NB;
   hipLaunchKernelGGL(finalize_bwd_kernel, dim3((unsigned)(nls + 3), 1, bt.count), dim3(256), 0, s,
                      gpart, nleaf * (nleaf + 1) / 2, nls, theta, at<double>(ws, Lo.off_alpha), n,
@@ -553,7 +580,8 @@ int loocv_backward_t(int kind, const double* x, int64_t n, int d, const double* 
                      hipStream_t s) {
   const int64_t np = Lo.np;
   const bool f64 = std::is_same<TS, double>::value;
-  const int ar = f64 ? (int)AR_F64 : g_ar_lauum;
+  // Y = diag(c) Sigma^-1 has no a-priori bound for an fp16 scale: BF16x6 there.
+  const int ar = f64 ? (int)AR_F64 : (g_ar_lauum == AR_FP16X3 ? (int)AR_BF16X6 : g_ar_lauum);
   const TS* X = at<TS>(ws, Lo.off_X);
   TS* Y = at<TS>(ws, Lo.off_L);
   double* part = at<double>(ws, Lo.off_part);
@@ -714,9 +742,11 @@ size_t gpk_mll_workspace_bytes(int64_t n, int64_t d, int prec) {
 }
 
 int gpk_set_mixed_arith(int potrf_products, int trtri_products, int lauum_products) {
-  auto ok = [](int v) { return v == AR_F64 || v == AR_BF16X3 || v == AR_BF16X6; };
+  auto ok = [](int v) {
+    return v == AR_F64 || v == AR_BF16X3 || v == AR_FP16X3 || v == AR_BF16X6;
+  };
   if (!ok(potrf_products) || !ok(trtri_products) || !ok(lauum_products)) {
-    set_error("mixed arithmetic: products must be 0 (float64), 3 or 6");
+    set_error("mixed arithmetic: products must be 0 (float64), 3, 4 (fp16) or 6");
     return GPK_ERR_BAD_ARG;
   }
   g_ar_potrf = potrf_products;

@@ -46,6 +46,10 @@ struct Factor {
   // Optional batch: all pointers above are member 0's; member z's are
   // bt.ws_stride bytes further per z (same padded size for every member).
   Batch bt = Batch();
+  // fp16 operand scales (device, AR_FP16X3 only): [sL, sL, sX, sT] with sL for
+  // entries of L and the trailing matrix, sX for X = L^-1, sT for L21 X11; the
+  // GEMMs read the (A, B) pair at fscale + 0 (POTRF), + 1 and + 2 (TRTRI).
+  const float* fscale = nullptr;
 
   double* dblk_at(int64_t o) const { return dblk ? dblk + (o / NB) * NB * NB : nullptr; }
   int leaf(int64_t o) {
@@ -64,7 +68,8 @@ struct Factor {
     GPK_TRY(trsm(row0, m, o, h));
     GPK_TRY(launch_gemm<TS, TS, TS>(ar_potrf, false, false, L + row0 * ld + o, ld,
                                     L + (o + h) * ld + o, ld, L + row0 * ld + o + h, ld, m,
-                                    n - h, h, -1.0, true, false, KR_FULL, s, nullptr, bt));
+                                    n - h, h, -1.0, true, false, KR_FULL, s, nullptr, bt,
+                                    fscale));
     return trsm(row0, m, o + h, n - h);
   }
   int potrf(int64_t o, int64_t n) {
@@ -75,7 +80,7 @@ struct Factor {
     GPK_TRY(launch_gemm<TS, TS, TS>(ar_potrf, false, false, L + (o + h) * ld + o, ld,
                                     L + (o + h) * ld + o, ld, L + (o + h) * ld + o + h, ld,
                                     n - h, n - h, h, -1.0, true, true, KR_FULL, s,
-                                    dblk_at(o + h), bt));
+                                    dblk_at(o + h), bt, fscale));
     return potrf(o + h, n - h);
   }
   // X = L^-1 below the leaves, whose diagonal blocks the leaves already wrote.
@@ -87,11 +92,12 @@ struct Factor {
     // T = L21 X11   ((n-h) x h; X11 lower, so k >= column tile)
     GPK_TRY(launch_gemm<TS, TS, TS>(ar_trtri, false, true, L + (o + h) * ld + o, ld,
                                     X + o * ld + o, ld, T, h, n - h, h, h, 1.0, false, false,
-                                    KR_GE_N, s, nullptr, bt));
+                                    KR_GE_N, s, nullptr, bt, fscale ? fscale + 1 : nullptr));
     // X21 = -X22 T  (X22 lower, so k <= row tile)
     return launch_gemm<TS, TS, TS>(ar_trtri, false, true, X + (o + h) * ld + o + h, ld, T, h,
                                    X + (o + h) * ld + o, ld, n - h, h, n - h, -1.0, false,
-                                   false, KR_LE_M, s, nullptr, bt);
+                                   false, KR_LE_M, s, nullptr, bt,
+                                   fscale ? fscale + 2 : nullptr);
   }
 };
 

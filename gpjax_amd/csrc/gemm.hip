@@ -22,6 +22,7 @@ struct GemmParams {
   int krange;
   double* cdiag;  // lower_tiles: diagonal tiles update this float64 [tile][NB][NB] instead of C
   Batch bt;       // batch of same-shape problems, member blockIdx.z (operands in workspaces)
+  const float* fscale;  // AR_FP16X3: power-of-two scales of A and B (device)
 };
 
 template <int AR, typename TA, typename TB, typename TC, bool A_KM, bool B_KN>
@@ -49,9 +50,14 @@ __global__ __launch_bounds__(WaveLayout<AR>::NT) void gemm_kernel(GemmParams p) 
                                      static_cast<const TB*>(p.B), p.ldb, m0, n0, kb, ke, acc,
                                      smem);
   } else {
+    float sa = 1.f, sb = 1.f;
+    if constexpr (AR == AR_FP16X3) {
+      sa = p.fscale[0];
+      sb = p.fscale[1];
+    }
     mainloop_bf16<AR, A_KM, B_KN>(static_cast<const float*>(p.A), p.lda,
                                   static_cast<const float*>(p.B), p.ldb, m0, n0, kb, ke, acc,
-                                  smem);
+                                  smem, sa, sb);
   }
   TC* C = static_cast<TC*>(p.C);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -118,7 +124,7 @@ template <typename TA, typename TB, typename TC>
 int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const TB* B,
                 int64_t ldb, TC* C, int64_t ldc, int64_t M, int64_t N, int64_t K, double alpha,
                 bool beta_one, bool lower_tiles, int krange, hipStream_t s, double* cdiag,
-                const Batch& bt) {
+                const Batch& bt, const float* fscale) {
   if (M <= 0 || N <= 0) return GPK_OK;
   if (M % TILE || N % TILE || K % TILE || K < 0) {
     set_error("gemm: extents %lld x %lld x %lld not multiples of %d", (long long)M,
@@ -134,7 +140,7 @@ int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const
     return GPK_ERR_BAD_ARG;
   }
   GemmParams p{A, lda, B, ldb, C, ldc, K, alpha, beta_one ? 1 : 0, lower_tiles ? 1 : 0, krange,
-               cdiag, bt};
+               cdiag, bt, fscale};
   if (arith == AR_F64) return dispatch_layout<AR_F64, TA, TB, TC>(a_km, b_kn, p, M, N, s);
   if constexpr (std::is_same<TA, float>::value && std::is_same<TB, float>::value &&
                 std::is_same<TC, float>::value) {
@@ -142,6 +148,13 @@ int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const
       return dispatch_layout<AR_BF16X3, float, float, float>(a_km, b_kn, p, M, N, s);
     if (arith == AR_BF16X6)
       return dispatch_layout<AR_BF16X6, float, float, float>(a_km, b_kn, p, M, N, s);
+    if (arith == AR_FP16X3) {
+      if (!fscale) {
+        set_error("gemm: fp16 products need the operand scales");
+        return GPK_ERR_BAD_ARG;
+      }
+      return dispatch_layout<AR_FP16X3, float, float, float>(a_km, b_kn, p, M, N, s);
+    }
   }
   set_error("gemm: unsupported arithmetic %d for these operand types", arith);
   return GPK_ERR_BAD_ARG;
@@ -150,7 +163,8 @@ int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const
 #define GPK_GEMM_INST(TA, TB, TC)                                                            \
   template int launch_gemm<TA, TB, TC>(int, bool, bool, const TA*, int64_t, const TB*,       \
                                        int64_t, TC*, int64_t, int64_t, int64_t, int64_t, double, \
-                                       bool, bool, int, hipStream_t, double*, const Batch&);
+                                       bool, bool, int, hipStream_t, double*, const Batch&, \
+                                       const float*);
 GPK_GEMM_INST(double, double, double)
 GPK_GEMM_INST(float, float, float)
 GPK_GEMM_INST(float, double, float)

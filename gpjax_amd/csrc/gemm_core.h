@@ -15,6 +15,10 @@
 //               ARITH 2019) on v_mfma_f32_16x16x32_bf16.
 //   AR_BF16X6 : three planes hi + mid + lo, six products
 //               (hh, hm, mh, mm, hl, lh): about fp32-product accuracy.
+//   AR_FP16X3 : operands scaled by a power of two, two fp16 planes (11 + 11
+//               significant bits), products hh + hl + lh on
+//               v_mfma_f32_16x16x32_f16: 2^-22 per product, half the matrix-core
+//               work and two thirds of the LDS traffic of BF16x6.
 // Operand layouts: A(m, k) = A[m*lda + k] (row-contiguous) or A[k*lda + m]
 // (column-contiguous); likewise B(n, k).  All extents are multiples of 128.
 //
@@ -36,6 +40,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 
 constexpr int TILE = 128;
 
@@ -84,6 +90,28 @@ __device__ __forceinline__ void split_pack8(const float (&v)[8], uint4 (&out)[NP
   for (int q = 0; q < NP; ++q) out[q] = make_uint4(wq[q][0], wq[q][1], wq[q][2], wq[q][3]);
 }
 
+// ---- fp16 split: s*a = h + l + remainder with h = fp16(s*a), l = fp16(s*a - h)
+// (both round to nearest even; the residual s*a - h is exact in fp32).  The
+// power-of-two scale s keeps |s*a| <= 2^14, so h never overflows, and l is a
+// normal fp16 number for every entry within 2^-3 of that bound (smaller entries
+// keep an absolute error below 2^-25 / s).
+__device__ __forceinline__ void split_pack8_f16(const float (&v)[8], float s, uint4 (&out)[2]) {
+  unsigned wq[2][4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    f32x2 r = {v[2 * e] * s, v[2 * e + 1] * s};
+    const f16x2 h = __builtin_convertvector(r, f16x2);
+    const f32x2 hf = __builtin_convertvector(h, f32x2);
+    r.x -= hf.x;
+    r.y -= hf.y;
+    const f16x2 l = __builtin_convertvector(r, f16x2);
+    wq[0][e] = __builtin_bit_cast(unsigned, h);
+    wq[1][e] = __builtin_bit_cast(unsigned, l);
+  }
+#pragma unroll
+  for (int q = 0; q < 2; ++q) out[q] = make_uint4(wq[q][0], wq[q][1], wq[q][2], wq[q][3]);
+}
+
 // ---- LDS footprint of one main loop ----
 constexpr int F64_BK = 16;
 constexpr int F64_LDP = TILE + 1;  // padded row length (doubles)
@@ -93,6 +121,18 @@ constexpr int BF_LDK = BF_BK + 16;  // padded k length (bf16): 96-byte rows meas
 constexpr int BF_CHUNK = 4;        // slabs per fp32 chunk: 128 k
 template <int AR>
 constexpr int bf_planes() { return AR == AR_BF16X6 ? 3 : 2; }
+
+// One 16 x 16 x 32 matrix-core product of two 8-element operand fragments held
+// as 16 bytes: bf16 for the BF16 arithmetics, fp16 for AR_FP16X3.
+template <int AR>
+__device__ __forceinline__ f32x4 mfma32(uint4 a, uint4 b, f32x4 c) {
+  if constexpr (AR == AR_FP16X3)
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a),
+                                                  __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
 template <int AR>
 constexpr size_t smem_bytes() {
   return AR == AR_F64 ? F64_SMEM
@@ -201,18 +241,22 @@ __device__ __forceinline__ void mainloop_f64(const TA* __restrict__ A, int64_t l
 //
 // Accumulation: the leading product hi*hi goes into its own fp32
 // accumulators (one fp32 rounding per slab), the correction products, 2^-8
-// and less of it, into a second set.  Both restart from zero every BF_CHUNK
-// slabs (128 k; every k-range is a multiple of 128) and are then added into
-// the float64 accumulators.  With all six products in one fp32 accumulator
+// and less of it, into a second set.  The leading set restarts from zero every
+// BF_CHUNK slabs (128 k; every k-range is a multiple of 128) and is then added
+// into the float64 accumulators; the correction set runs over the whole
+// k-range, or (CFLUSH) is flushed with it.  In the fused Sigma^-1 pass CFLUSH
+// costs 1.6 % of the kernel's time and takes the gradient's relative error at
+// N = 65,536 from 9.0e-5 to 7.2e-5; the Cholesky and inverse updates keep the
+// single chain (the MLL error they leave, 1.5e-7, is well inside 1e-6).  With all six products in one fp32 accumulator
 // over 256 k, the rounding of the running sums dominated the error of the
 // whole mixed-precision factorisation (float64 products of the same float32
 // operands are ~1000x closer to the float64 result).
-template <int AR, bool A_KM, bool B_KN>
+template <int AR, bool A_KM, bool B_KN, bool CFLUSH = false>
 __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64_t lda,
                                               const float* __restrict__ B, int64_t ldb,
                                               int64_t m0, int64_t n0, int64_t kb, int64_t ke,
                                               f64x4 (&acc64)[4][WaveLayout<AR>::NJ],
-                                              char* smem) {
+                                              char* smem, float sa = 1.f, float sb = 1.f) {
   typedef WaveLayout<AR> WL;
   constexpr int NJ = WL::NJ;
   constexpr int NP = bf_planes<AR>();
@@ -252,18 +296,19 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
   };
   // Split into bf16 planes and store one 16-byte write per plane into the
   // k-contiguous layout the matrix-core operands read.
-  auto sstore_one = [&](unsigned short* base, const float (&r)[8], bool km) {
+  auto sstore_one = [&](unsigned short* base, const float (&r)[8], bool km, float sc) {
     const int row = km ? (t & 127) : (t >> 2);
     const int ko = km ? (t >> 7) * 8 : (t & 3) * 8;
     uint4 pk[NP];
-    split_pack8<NP>(r, pk);
+    if constexpr (AR == AR_FP16X3) split_pack8_f16(r, sc, pk);
+    else split_pack8<NP>(r, pk);
 #pragma unroll
     for (int pl = 0; pl < NP; ++pl)
       *reinterpret_cast<uint4*>(base + (pl * TILE + row) * BF_LDK + ko) = pk[pl];
   };
   auto sstore = [&](int buf) {
-    sstore_one(S + buf * 2 * OPS, ra, A_KM);
-    sstore_one(S + buf * 2 * OPS + OPS, rb, B_KN);
+    sstore_one(S + buf * 2 * OPS, ra, A_KM, sa);
+    sstore_one(S + buf * 2 * OPS + OPS, rb, B_KN, sb);
   };
 
   const int64_t nslab = (ke - kb) / BF_BK;  // a multiple of BF_CHUNK
@@ -284,9 +329,9 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
       const unsigned short* As = S + (u & 1) * 2 * OPS;
       const unsigned short* Bs = As + OPS;
       // Fragments of planes 0 and 1; plane 2 (x6) later reuses plane 1's registers.
-      bf16x8 a0[4], b0[NJ], a1[4], b1[NJ];
+      uint4 a0[4], b0[NJ], a1[4], b1[NJ];
       auto frag = [&](const unsigned short* P, int pl, int base, int i) {
-        return *reinterpret_cast<const bf16x8*>(
+        return *reinterpret_cast<const uint4*>(
             P + (pl * TILE + base + i * 16 + (lane & 15)) * BF_LDK + 8 * (lane >> 4));
       };
 #pragma unroll
@@ -303,16 +348,15 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < NJ; ++j)
-          acch[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[i], b0[j],
-                                                               u == 0 ? zero : acch[i][j], 0, 0, 0);
+          acch[i][j] = mfma32<AR>(a0[i], b0[j], u == 0 ? zero : acch[i][j]);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
           f32x4 c = accc[i][j];
-          if (AR == AR_BF16X6) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[i], b1[j], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[i], b0[j], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[i], b1[j], c, 0, 0, 0);
+          if (AR == AR_BF16X6) c = mfma32<AR>(a1[i], b1[j], c);
+          c = mfma32<AR>(a1[i], b0[j], c);
+          c = mfma32<AR>(a0[i], b1[j], c);
           accc[i][j] = c;
         }
       if (AR == AR_BF16X6) {
@@ -325,8 +369,8 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
 #pragma unroll
           for (int j = 0; j < NJ; ++j) {
             f32x4 c = accc[i][j];
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[i], b0[j], c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[i], b1[j], c, 0, 0, 0);
+            c = mfma32<AR>(a1[i], b0[j], c);
+            c = mfma32<AR>(a0[i], b1[j], c);
             accc[i][j] = c;
           }
       }
@@ -337,6 +381,16 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
           for (int j = 0; j < NJ; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)acch[i][j][r];
+        if constexpr (CFLUSH) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+#pragma unroll
+              for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)accc[i][j][r];
+              accc[i][j] = zero;
+            }
+        }
       }
       // Stage slab s+1 (at the last slab a repeat of it, stored and never read)
       // and start loading slab s+2.
@@ -347,13 +401,23 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
     }
   }
   // The correction products are at most 2^-8 of the leading ones, so their
-  // single fp32 chain over the whole k-range adds little error.
+  // single fp32 chain over the whole k-range adds little error (zero here
+  // under CFLUSH).
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < NJ; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)accc[i][j][r];
+  if constexpr (AR == AR_FP16X3) {
+    const double inv = 1.0 / ((double)sa * (double)sb);  // a power of two: exact
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < NJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc64[i][j][r] *= inv;
+  }
 }
 
 // Row (within a 16 x 16 block) of accumulator element r held by `lane`.

@@ -163,18 +163,22 @@ int launch_leaf(TS* A, TS* X, int64_t ld, int64_t o, double* leafinv, double* lo
 
 // k-range modes of the tile GEMM.
 enum KRange { KR_FULL = 0, KR_GE_N = 1, KR_LE_M = 2, KR_GE_M = 3 };
-// Product arithmetic of the tile GEMM.
-enum Arith { AR_F64 = 0, AR_BF16X3 = 3, AR_BF16X6 = 6 };
+// Product arithmetic of the tile GEMM.  AR_FP16X3: float32 operands scaled by a
+// power of two (so that every entry sits well inside the fp16 range), split
+// into fp16 hi + lo planes, products hi*hi + hi*lo + lo*hi.
+enum Arith { AR_F64 = 0, AR_BF16X3 = 3, AR_FP16X3 = 4, AR_BF16X6 = 6 };
 
 // C[M x N] = alpha * sum_k A(m, k) B(n, k) (+ C if beta_one), M, N, K multiples
 // of NB.  A(m, k) is A[m*lda + k] (a_km = false) or A[k*lda + m] (a_km = true);
 // B(n, k) is B[n*ldb + k] (b_kn = false) or B[k*ldb + n] (b_kn = true).
 // lower_tiles: only tiles with mt >= nt.  krange restricts k per tile.
+// fscale (AR_FP16X3 only): device pointer to the power-of-two scales of A and B.
 template <typename TA, typename TB, typename TC>
 int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const TB* B,
                 int64_t ldb, TC* C, int64_t ldc, int64_t M, int64_t N, int64_t K, double alpha,
                 bool beta_one, bool lower_tiles, int krange, hipStream_t s,
-                double* cdiag = nullptr, const Batch& bt = Batch());
+                double* cdiag = nullptr, const Batch& bt = Batch(),
+                const float* fscale = nullptr);
 
 // Fused LAUUM + adjoint: for every lower NB-tile of Sinv = X^T X (X = L^-1,
 // lower, Np x Np), reduce 1/2 tr(W dSigma/dtheta) contributions with
@@ -182,11 +186,13 @@ int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const
 // partial: [num_tiles][nls + 2] float64.
 // beta (optional): W = alpha beta^T + beta alpha^T - X^T X instead; dense: X is
 // a dense Np x Np matrix (every k) rather than lower triangular (k >= row).
+// fscale (AR_FP16X3 only): device pointer to the power-of-two scale of X.
 template <typename TS>
 int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, const double* x,
                    int64_t n, int d, const double* theta, int nls, const double* alpha,
                    double* partial, hipStream_t s, const Batch& bt = Batch(),
-                   const double* beta = nullptr, bool dense = false);
+                   const double* beta = nullptr, bool dense = false,
+                   const float* fscale = nullptr);
 
 // y = X r (trans = false) or y = X^T r (trans = true), X lower triangular
 // np x np (only k <= i is read); float64 accumulation through `partial`.

@@ -241,7 +241,8 @@ int gpk_batched_mll_grad(int kind, const double* x, const double* y, const int64
 
 /*
  * Products per element used by the GPK_PREC_MIXED stages (3 = bf16 hi/lo
- * split, 6 = hi/mid/lo split, 0 = float64 products of the float32 operands):
+ * split, 6 = hi/mid/lo split, 4 = fp16 hi/lo split of power-of-two scaled
+ * operands, 0 = float64 products of the float32 operands):
  * Cholesky updates, triangular inverse, and the
  * fused Sigma^-1 adjoint.  Process-wide; defaults 6, 3, 6.  No counterpart in
  * the reference (which has a single float64 path); it exists to trade speed

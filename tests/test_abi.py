@@ -111,7 +111,8 @@ def test_elbo_predict_workspace_and_validation():
 
 def test_mixed_arith_option_validation():
     lib = _lib.lib()
-    assert lib.gpk_set_mixed_arith(4, 3, 6) == 1
+    assert lib.gpk_set_mixed_arith(5, 3, 6) == 1
+    assert lib.gpk_set_mixed_arith(4, 3, 4) == 0   # fp16 hi/lo split of scaled operands
     assert lib.gpk_set_mixed_arith(0, 3, 0) == 0   # float64 products of float32 operands
     assert lib.gpk_set_mixed_arith(6, 3, 6) == 0
 

@@ -139,6 +139,25 @@ def test_mll_grad_mixed_parity(cuda, n):
     assert resid <= 1e-10
 
 
+@pytest.mark.parametrize("arith", [(4, 3, 4), (4, 3, 6), (6, 3, 4)])
+def test_fp16_split_arithmetic(cuda, arith):
+    """The fp16 hi/lo split of power-of-two scaled operands (arithmetic 4) in the
+    Cholesky updates and/or the fused Sigma^-1 pass.  It is about four times
+    less accurate per product than BF16x6 (2^-22 against 2^-24): within 2e-6 on
+    the MLL and 5e-4 on the gradient here (at N = 65,536: 6.3e-7 and 1.3e-4, so it
+    is not the default).  Process-wide setting, restored afterwards."""
+    w = workloads.exact_large(n=2048)
+    assert _lib.lib().gpk_set_mixed_arith(*arith) == 0
+    try:
+        rel_mll, rel_g, resid = _mixed_vs_oracle(cuda, w["X"], w["y"], "rbf", w["lengthscale"],
+                                                 w["variance"], w["obs_stddev"])
+    finally:
+        assert _lib.lib().gpk_set_mixed_arith(6, 3, 6) == 0
+    assert rel_mll <= 2e-6
+    assert rel_g <= 5e-4
+    assert resid <= 1e-10
+
+
 @pytest.mark.parametrize("n,d,kind", [(1500, 8, "rbf"), (700, 2, "matern32"), (2000, 1, "rbf")])
 def test_mixed_accuracy_envelope(cuda, n, d, kind):
     """Low-dimensional, densely sampled inputs make K nearly low-rank (condition
