@@ -121,3 +121,27 @@ def test_c4_full_size_elbo_properties(cuda):
         em = _elbo(cuda, w, X, y, **{name: base - h})[0]
         fd = (ep - em) / (2 * h)
         assert abs(fd - g[idx]) <= 1e-6 * abs(g[idx]) + 1e-9 * abs(e), (name, fd, g[idx])
+
+
+def test_c4_full_size_collapsed_predict_properties(cuda):
+    """CollapsedVariationalGaussian.predict at the C4 size (N = 1e6, M = 2,048):
+    permutation invariance in the training data, the diagonal-only call equal to
+    the full call's diagonal, and 0 <= var <= k(0) + jitter (the collapsed
+    posterior never exceeds the prior)."""
+    w = workloads.sparse()
+    X, y = w["X"], w["y"]
+    T = torch.tensor(np.random.default_rng(5).uniform(0, 1, (1500, 8)), device=cuda)
+    q, D, _, _ = _sparse_model(cuda, w, X, y)
+    full = q.predict(T, D, full_covariance=True)
+    diag = q.predict(T, D, full_covariance=False)
+    perm = np.random.default_rng(1).permutation(len(X))
+    qp, Dp, _, _ = _sparse_model(cuda, w, X[perm], y[perm])
+    fp = qp.predict(T, Dp, full_covariance=False)
+    mu, var = full.mean.cpu().numpy(), np.diag(full.covariance().cpu().numpy())
+    assert np.all(np.isfinite(mu)) and np.all(np.isfinite(var))
+    dv = np.diag(diag.covariance().cpu().numpy())
+    assert np.max(np.abs(dv - var)) <= 1e-12 * np.max(np.abs(var))
+    assert np.max(np.abs(diag.mean.cpu().numpy() - mu)) == 0.0
+    assert np.max(np.abs(fp.mean.cpu().numpy() - mu)) <= 1e-9 * np.max(np.abs(mu))
+    assert np.max(np.abs(np.diag(fp.covariance().cpu().numpy()) - var)) <= 1e-9 * np.max(var)
+    assert var.min() >= 0.0 and var.max() <= w["variance"] + 1e-6 + 1e-9

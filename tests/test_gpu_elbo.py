@@ -133,3 +133,27 @@ def test_collapsed_predict_equals_exact_when_z_is_x(cuda):
     pe = post.predict(T, D, precision="fp64")
     assert torch.allclose(pc.mean, pe.mean, rtol=1e-6, atol=1e-8)
     assert torch.allclose(pc.covariance(), pe.covariance(), rtol=1e-6, atol=1e-8)
+
+
+@pytest.mark.parametrize("n,m,t,full", [(1, 1, 1, True), (5, 3, 2, True), (9, 4, 7, False)])
+def test_collapsed_predict_tiny_shapes(cuda, n, m, t, full):
+    """Edge sizes (single point, fewer inducing points than a padded tile):
+    the padded workspace must not leak into the M x T and T x T results."""
+    rng = np.random.default_rng(n * 100 + m * 10 + t)
+    X = rng.uniform(-1, 1, (n, 2))
+    y = np.cos(X[:, 0]) + 0.1 * rng.standard_normal(n)
+    Z = rng.uniform(-1, 1, (m, 2))
+    T = rng.uniform(-1.2, 1.2, (t, 2))
+    k = gpx.kernels.Matern32(lengthscale=np.array([0.7, 1.1]), variance=0.9)
+    mf = gpx.mean_functions.Constant(gpx.parameters.Real(0.2))
+    post = gpx.gps.Prior(mean_function=mf, kernel=k) * \
+        gpx.likelihoods.Gaussian(num_datapoints=n, obs_stddev=0.3)
+    q = gpx.variational_families.CollapsedVariationalGaussian(posterior=post, inducing_inputs=Z)
+    D = gpx.Dataset(X=torch.tensor(X, device=cuda), y=torch.tensor(y[:, None], device=cuda))
+    pred = q.predict(torch.tensor(T, device=cuda), D, full_covariance=full)
+    mu_ref, cov_ref = O.collapsed_predict("matern32", X, y, Z, T, np.array([0.7, 1.1]), 0.9, 0.3,
+                                          1e-6, 0.2)
+    np.testing.assert_allclose(pred.mean.cpu().numpy(), mu_ref, rtol=1e-9, atol=1e-12)
+    cov = pred.covariance().cpu().numpy()
+    ref = cov_ref if full else np.diag(np.diag(cov_ref))
+    np.testing.assert_allclose(cov, ref, rtol=1e-9, atol=1e-12)
