@@ -61,7 +61,7 @@ __global__ __launch_bounds__(WaveLayout<AR>::NT) void adjoint_kernel(int kind,
   } else {
     float sx = 1.f;
     if constexpr (AR == AR_FP16X3) sx = fscale[0];
-    mainloop_bf16<AR, true, true, true>(X, ld, X, ld, m0, n0, kb, np, acc, smem, sx, sx);
+    mainloop_bf16<AR, true, true, true, false>(X, ld, X, ld, m0, n0, kb, np, acc, smem, sx, sx);
   }
   __syncthreads();
 

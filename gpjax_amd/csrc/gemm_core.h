@@ -236,7 +236,11 @@ __device__ __forceinline__ void mainloop_f64(const TA* __restrict__ A, int64_t l
 // Software pipeline over k-slabs of BF_BK with two LDS buffers and one
 // barrier per slab: while the matrix cores work on slab s (buffer s&1), the
 // registers holding slab s+1 are split into bf16 planes and stored into
-// buffer (s+1)&1, and the global loads of slab s+2 are in flight.  Eight waves
+// buffer (s+1)&1, and the same registers are refilled with slab s+3, so each
+// global load has two slabs to land (DEEP: two register sets alternate;
+// loading one slab ahead, with one set, measured 1-4 % slower for row-major
+// operands and equal for column-major ones, where the fused Sigma^-1 pass
+// needs the registers).  Eight waves
 // (two per SIMD) interleave these phases in hardware; no scheduling hints.
 //
 // Accumulation: the leading product hi*hi goes into its own fp32
@@ -251,7 +255,7 @@ __device__ __forceinline__ void mainloop_f64(const TA* __restrict__ A, int64_t l
 // over 256 k, the rounding of the running sums dominated the error of the
 // whole mixed-precision factorisation (float64 products of the same float32
 // operands are ~1000x closer to the float64 result).
-template <int AR, bool A_KM, bool B_KN, bool CFLUSH = false>
+template <int AR, bool A_KM, bool B_KN, bool CFLUSH = false, bool DEEP = true>
 __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64_t lda,
                                               const float* __restrict__ B, int64_t ldb,
                                               int64_t m0, int64_t n0, int64_t kb, int64_t ke,
@@ -274,7 +278,8 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
   // Row-contiguous operand: row = t/4, k = (t%4)*8 .. +7 (two 16-byte loads).
   // Column-contiguous operand: row = t%128, k = (t/128)*8 .. +7 (eight
   // 4-byte loads; consecutive lanes read consecutive rows, 256 bytes per load).
-  float ra[8], rb[8];
+  constexpr int NSET = DEEP ? 2 : 1;
+  float ra[NSET][8], rb[NSET][8];
   auto gload_one = [&](const float* __restrict__ P, int64_t ld, int64_t r0, bool km,
                        int64_t k0, float (&r)[8]) {
     if (!km) {
@@ -290,9 +295,9 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
       for (int q = 0; q < 8; ++q) r[q] = p[q * ld];
     }
   };
-  auto gload = [&](int64_t k0) {
-    gload_one(A, lda, m0, A_KM, k0, ra);
-    gload_one(B, ldb, n0, B_KN, k0, rb);
+  auto gload = [&](int64_t k0, int set) {
+    gload_one(A, lda, m0, A_KM, k0, ra[DEEP ? set : 0]);
+    gload_one(B, ldb, n0, B_KN, k0, rb[DEEP ? set : 0]);
   };
   // Split into bf16 planes and store one 16-byte write per plane into the
   // k-contiguous layout the matrix-core operands read.
@@ -306,15 +311,17 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
     for (int pl = 0; pl < NP; ++pl)
       *reinterpret_cast<uint4*>(base + (pl * TILE + row) * BF_LDK + ko) = pk[pl];
   };
+  // Slab s sits in register set s & 1 (DEEP) and is stored into LDS buffer s & 1.
   auto sstore = [&](int buf) {
-    sstore_one(S + buf * 2 * OPS, ra, A_KM, sa);
-    sstore_one(S + buf * 2 * OPS + OPS, rb, B_KN, sb);
+    sstore_one(S + buf * 2 * OPS, ra[DEEP ? buf : 0], A_KM, sa);
+    sstore_one(S + buf * 2 * OPS + OPS, rb[DEEP ? buf : 0], B_KN, sb);
   };
 
   const int64_t nslab = (ke - kb) / BF_BK;  // a multiple of BF_CHUNK
-  gload(kb);
+  gload(kb, 0);
   sstore(0);
-  gload(kb + BF_BK);
+  gload(kb + BF_BK, 1);
+  if (DEEP) gload(kb + 2 * BF_BK, 0);  // every k-range holds at least 4 slabs
   __syncthreads();
   const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
   f32x4 accc[4][NJ];
@@ -392,11 +399,11 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
             }
         }
       }
-      // Stage slab s+1 (at the last slab a repeat of it, stored and never read)
-      // and start loading slab s+2.
+      // Stage slab s+1 (at the last slab a repeat of the final slab, stored and
+      // never read) and refill its register set with slab s+3.
       sstore((u + 1) & 1);
-      const int64_t s2 = c0 + u + 2 < nslab ? c0 + u + 2 : nslab - 1;
-      gload(kb + s2 * BF_BK);
+      const int64_t ahead = c0 + u + (DEEP ? 3 : 2);
+      gload(kb + (ahead < nslab ? ahead : nslab - 1) * BF_BK, (u + 1) & 1);
       __syncthreads();
     }
   }
