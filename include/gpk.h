@@ -105,7 +105,9 @@ int gpk_mll_forward(int kind, int prec, const double* x, const double* y, int64_
  * Sigma^-1) dSigma/dtheta), layout of theta.  Must follow gpk_mll_forward on the
  * same workspace and stream.  Sigma^-1 is never stored: each tile of
  * L^-T L^-1 is formed on the matrix cores and reduced against the kernel
- * derivative tile, recomputed from x, in the same pass.
+ * derivative tile, recomputed from x, in the same pass.  The factor L in the
+ * workspace is replaced by L^-T (read along rows by that pass); L^-1 and alpha
+ * stay, so the backward may be repeated.
  */
 int gpk_mll_backward(int kind, int prec, const double* x, int64_t n, int64_t d,
                      const double* theta, int nls, double gbar_host, void* ws, size_t ws_bytes,
