@@ -14,12 +14,14 @@ something that does not share its code path:
   * collapsed ELBO: the reference's invariant Z = X => ELBO == MLL within 1e-6
     relative (``tests/test_objectives.py:193-199``) and a dense evaluation
     log N(y; m, s^2 I + Q) - tr(K - Q) / (2 s^2);
-  * predict: dense solve-based formulas, and exact interpolation as noise -> 0;
+  * predict: dense solve-based formulas, exact interpolation as noise -> 0, and
+    likelihood-weighted Monte Carlo over joint prior draws (no solve at all);
   * collapsed predict: Z = X gives the exact predictive, Titsias' dense form;
   * LOOCV: brute-force leave-one-out conditioning (N - 1 point solves per
     held-out point), the N = 1 closed form; its gradient against torch
     autograd of that brute force and central differences.
-A planted-mistake test shows the gradient pins reject a dropped factor.
+A planted-mistake test shows the gradient pins reject a dropped factor.  The
+committed fixtures in tests/golden/ must still be what the oracle computes.
 """
 
 import math
@@ -327,6 +329,41 @@ def test_predict_interpolates_without_noise():
     assert np.max(np.abs(np.diag(cov))) < 1e-6
 
 
+@pytest.mark.parametrize("kind", ["rbf", "matern32"])
+def test_predict_monte_carlo_importance_sampling(kind):
+    """Posterior moments without any solve: draw (f, f*) jointly from the prior
+    (latent covariance with the posterior's jitter on the training block and the
+    prior's jitter on the test block), weight each draw by the likelihood
+    N(y; f, obs_sd^2 I), and compare the weighted mean and variance of f* with
+    predict (the statistical check of the reference's tests/test_gps.py:283-342,
+    restated without pathwise sampling).  Tolerance: 6 Monte-Carlo standard
+    errors from the effective sample size (about 1.1e5 of the 1e6 draws);
+    dropping the noise from Sigma or the -Ktx Sigma^-1 Kxt term moves some test
+    point by 50 or more of them."""
+    rng = np.random.default_rng(11)
+    X = np.array([[-1.0], [0.2], [1.1]])
+    T = np.array([[-0.4], [0.7], [2.0]])
+    y = np.array([0.3, -0.5, 0.8])
+    ls, var, sd, jp, jq, mean = 0.9, 1.3, 0.5, 1e-6, 1e-6, 0.1
+    mu, cov = O.predict(kind, X, y, T, ls, var, sd, jp, jq, mean)
+    A = np.vstack([X, T])
+    K = O.gram(kind, A, ls, var)
+    K[:3, :3] += jp * np.eye(3)
+    K[3:, 3:] += jq * np.eye(3)
+    Lk = np.linalg.cholesky(K)
+    f = mean + rng.standard_normal((1_000_000, 6)) @ Lk.T
+    logw = -0.5 * np.sum((y - f[:, :3]) ** 2, axis=1) / sd ** 2
+    w = np.exp(logw - logw.max())
+    w /= w.sum()
+    ess = 1.0 / np.sum(w ** 2)
+    m_mc = w @ f[:, 3:]
+    v_mc = w @ (f[:, 3:] - m_mc) ** 2
+    v = np.diag(cov)
+    assert ess > 1e5
+    np.testing.assert_array_less(np.abs(m_mc - mu), 6 * np.sqrt(v / ess))
+    np.testing.assert_array_less(np.abs(v_mc - v), 6 * v * np.sqrt(2.0 / ess))
+
+
 @pytest.mark.parametrize("kind", KINDS)
 def test_collapsed_predict_equals_exact_when_z_is_x(kind):
     """With Z = X the collapsed posterior is the exact one: Q = K, so mean and
@@ -460,3 +497,25 @@ def test_loocv_grad_vs_central_differences(kind):
     fd.append((f(ls, var, sd, c + h) - f(ls, var, sd, c - h)) / (2 * h))
     np.testing.assert_allclose(_grad_vec(g, True), np.array(fd), rtol=2e-6, atol=1e-7)
     assert g["loocv"] == pytest.approx(f(ls, var, sd, c), rel=1e-12)
+
+
+def test_oracle_reproduces_golden_fixtures():
+    """tests/golden/exact_small.npz (written by tests/golden/make_fixtures.py
+    from the oracle) is what the device path is checked against without the
+    oracle; this keeps the two in step: any change to the oracle's answers on
+    those 120 problems must come with regenerated fixtures."""
+    import importlib.util
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    spec = importlib.util.spec_from_file_location(
+        "make_fixtures", os.path.join(here, "golden", "make_fixtures.py"))
+    F = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(F)
+    fresh = F.compute(with_spread=False)
+    z = np.load(os.path.join(here, "golden", "exact_small.npz"))
+    assert sorted(k for k in z.files if "_elbospread_" not in k) == sorted(fresh)
+    for key in fresh:
+        if key == "meta":
+            assert str(z[key]) == str(fresh[key])
+        else:
+            np.testing.assert_allclose(z[key], fresh[key], rtol=1e-11, atol=1e-13, err_msg=key)
