@@ -223,7 +223,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed-arith", default="6,3,6", metavar="P,T,L",
                     help="bf16 products per element for the Cholesky updates, the "
-                         "triangular inverse and the fused Sigma^-1 pass (3 or 6 each)")
+                         "triangular inverse and the fused Sigma^-1 pass (0 = float64 "
+                         "products, 3, 4 = fp16 hi/lo split, or 6 each)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES bytes 5 times "
                          "(reference for rocprof FETCH_SIZE / WRITE_SIZE scaling) and exit")

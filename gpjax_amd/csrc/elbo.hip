@@ -4,8 +4,9 @@
 // The reference forms A = Lz^-1 Kzx / sigma (an M x N matrix, materialised)
 // and AA^T.  Here the data enter only through the N-summed statistics
 //   P = Kzx Kxz (M x M),  b = Kzx r,  q = Kzx 1,  r = y - c,
-// accumulated on the matrix cores while Kzx tiles are evaluated on the fly,
-// so no N x M buffer exists.  With s = sigma^2 and C = Kzz + P / s the
+// accumulated chunk by chunk: Kzx is evaluated once per chunk of 65,536
+// points into an M x 65,536 buffer and P += Kc Kc^T runs on the FP64 matrix
+// cores, so no N x M buffer exists.  With s = sigma^2 and C = Kzz + P / s the
 // determinant and inversion lemmas that the reference's comments state
 // (objectives.py:361-385) give
 //   2 ELBO = -n log(2 pi s) - log|C| + log|Kzz| - r^T r / s + b^T C^-1 b / s^2
@@ -16,8 +17,9 @@
 //   G_K = Kzz^-1 - C^-1 - c c^T / s^2 - Kzz^-1 P Kzz^-1 / s,
 //   g_s = -n/s + tr(C^-1 P)/s^2 + r^T r/s^2 + c^T P c/s^4 - 2 b^T c/s^3
 //         + n var/s^2 - tr(Kzz^-1 P)/s^2.
-// The backward recomputes Kzx tiles, forms W tile by tile on the matrix cores
-// and reduces it against dKzx / d(lengthscale, variance, Z) in the epilogue.
+// The backward evaluates Kzx once per chunk of 65,536 points, forms W for the
+// chunk with one GEMM on the matrix cores and reduces it against
+// dKzx / d(lengthscale, variance, Z) elementwise.
 // Everything here runs in float64.
 #include <cmath>
 
@@ -27,8 +29,9 @@
 namespace gpk {
 namespace {
 
-constexpr int ST = 16;      // data points per slab (statistics kernel)
 constexpr int EDMAX = 16;   // largest input dimension on the ELBO path
+constexpr int64_t BW_CHUNK = 65536;  // points per Kzx chunk (statistics, data pass)
+constexpr int64_t EW = 1024;         // points per block of the gradient data pass
 
 __device__ __forceinline__ double lsq(const double* theta, int nls, int q) {
   return theta[nls == 1 ? 0 : q];
@@ -45,111 +48,40 @@ __device__ __forceinline__ double d2_of(const double* a, const double* b) {
   return s;
 }
 
-// ---- E1: streamed statistics P (lower tiles), b, q ----
-template <int DMAX>
-__global__ __launch_bounds__(256) void elbo_stats_kernel(
-    int kind, const double* __restrict__ x, const double* __restrict__ y, int64_t n,
-    const double* __restrict__ z, int64_t m, int d, const double* __restrict__ theta, int nls,
-    int64_t chunk, int64_t ntile, int64_t mp, double* __restrict__ ppart,
-    double* __restrict__ bpart, double* __restrict__ qpart) {
-  __shared__ double zi[TILE][DMAX + 1];
-  __shared__ double zj[TILE][DMAX + 1];
-  __shared__ double xs[ST][DMAX + 1];
-  __shared__ double rs[ST];
-  __shared__ double As[ST * F64_LDP];
-  __shared__ double Bs[ST * F64_LDP];
-  int64_t mt, nt;
-  lower_tile(blockIdx.x, mt, nt);
-  const int64_t sidx = blockIdx.y;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w >> 1, wn = w & 1;
-  for (int e = t; e < TILE * DMAX; e += 256) {
-    const int r = e / DMAX, q = e % DMAX;
-    const int64_t gi = mt * TILE + r, gj = nt * TILE + r;
-    zi[r][q] = (q < d && gi < m) ? z[gi * d + q] / lsq(theta, nls, q) : 0.0;
-    zj[r][q] = (q < d && gj < m) ? z[gj * d + q] / lsq(theta, nls, q) : 0.0;
+// ---- E1: streamed statistics b = Kzx r and q = Kzx 1 over one chunk ----
+// Kc[i * ldk + (g - c0)] = k(z_i, x_g) for the chunk's points; one block per
+// inducing row, fixed-order reduction; the first chunk writes, later ones add.
+// (P = Kzx Kxz is accumulated from the same chunk by one FP64 GEMM.)
+__global__ __launch_bounds__(256) void chunk_rowsum_kernel(
+    const double* __restrict__ Kc, int64_t ldk, int64_t c0, int64_t ncol,
+    const double* __restrict__ y, const double* __restrict__ theta, int nls, int accumulate,
+    double* __restrict__ b, double* __restrict__ q) {
+  __shared__ double red[2][4];
+  const int64_t i = blockIdx.x;
+  const double cmean = theta[nls + 2];
+  const double* row = Kc + i * ldk;
+  double sb = 0.0, sq = 0.0;
+  for (int64_t j = threadIdx.x; j < ncol; j += 256) {
+    const double k = row[j];
+    sb += k * (y[c0 + j] - cmean);
+    sq += k;
   }
-  const double var = theta[nls], cmean = theta[nls + 2];
-  const int64_t n0 = sidx * chunk;
-  const int64_t n1 = (n0 + chunk < n) ? n0 + chunk : n;
-  f64x4 acc[4][4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
-  double bacc = 0.0, qacc = 0.0;
-  const bool diag = (mt == nt);
-  const int kk = t & 15;
-  for (int64_t k0 = n0; k0 < n1; k0 += ST) {
-    __syncthreads();
-    for (int e = t; e < ST * DMAX; e += 256) {
-      const int r = e / DMAX, q = e % DMAX;
-      const int64_t g = k0 + r;
-      xs[r][q] = (q < d && g < n1) ? x[g * d + q] / lsq(theta, nls, q) : 0.0;
-    }
-    if (t < ST) rs[t] = (k0 + t < n1) ? y[k0 + t] - cmean : 0.0;
-    __syncthreads();
-    const bool kvalid = (k0 + kk) < n1;
-#pragma unroll
-    for (int q8 = 0; q8 < 8; ++q8) {
-      const int row = (t >> 4) + 16 * q8;
-      double va = 0.0, vb = 0.0;
-      if (kvalid) {
-        if (mt * TILE + row < m) va = var * kval(kind, d2_of<DMAX>(zi[row], xs[kk]));
-        if (nt * TILE + row < m) vb = var * kval(kind, d2_of<DMAX>(zj[row], xs[kk]));
-      }
-      As[kk * F64_LDP + row] = va;
-      Bs[kk * F64_LDP + row] = vb;
-    }
-    __syncthreads();
-    mfma_slab_f64(As, Bs, F64_LDP, lane, wm, wn, acc);
-    if (diag && t < TILE) {
-#pragma unroll
-      for (int q = 0; q < ST; ++q) {
-        const double a = As[q * F64_LDP + t];
-        bacc += a * rs[q];
-        qacc += a;
-      }
-    }
+  for (int off = 32; off > 0; off >>= 1) {
+    sb += __shfl_xor(sb, off);
+    sq += __shfl_xor(sq, off);
   }
-  double* out = ppart + (sidx * ntile + (int64_t)blockIdx.x) * (TILE * TILE);
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = wm * 64 + i * 16 + acc_row<AR_F64>(lane, r);
-        const int col = wn * 64 + j * 16 + (lane & 15);
-        out[row * TILE + col] = acc[i][j][r];
-      }
-  if (diag && t < TILE) {
-    bpart[sidx * mp + mt * TILE + t] = bacc;
-    qpart[sidx * mp + mt * TILE + t] = qacc;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = sb;
+    red[1][threadIdx.x >> 6] = sq;
   }
-}
-
-// P (full, symmetric) from the per-split lower tiles.
-__global__ __launch_bounds__(256) void stats_reduce_kernel(const double* __restrict__ ppart,
-                                                           int64_t S, int64_t ntile,
-                                                           double* __restrict__ P, int64_t ld) {
-  const int64_t tile = blockIdx.x;
-  const int e = blockIdx.y * 256 + threadIdx.x;
-  double s = 0.0;
-  for (int64_t k = 0; k < S; ++k) s += ppart[(k * ntile + tile) * (TILE * TILE) + e];
-  int64_t mt, nt;
-  lower_tile(tile, mt, nt);
-  const int64_t i = mt * TILE + e / TILE, j = nt * TILE + e % TILE;
-  P[i * ld + j] = s;
-  P[j * ld + i] = s;
-}
-
-__global__ void vec_sum_kernel(const double* __restrict__ part, int64_t len, int64_t S,
-                               double* __restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= len) return;
-  double s = 0.0;
-  for (int64_t k = 0; k < S; ++k) s += part[k * len + i];
-  out[i] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double tb = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    const double tq = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    b[i] = (accumulate ? b[i] : 0.0) + tb;
+    q[i] = (accumulate ? q[i] : 0.0) + tq;
+  }
 }
 
 // sum r_i^2 and sum r_i over the data, r = y - c: per-block partials.
@@ -193,6 +125,20 @@ __global__ void add_p_kernel(double* __restrict__ C, const double* __restrict__ 
 }
 
 // A[j][i] = A[i][j] for the strictly lower NB-tiles (diagonal tiles are full).
+// P (full, symmetric) from the split-K partials Pp[i * ldp + z * kz + j]
+// (z < S) of its lower tiles; off-diagonal tiles are mirrored.
+__global__ void psum_kernel(const double* __restrict__ Pp, int64_t ldp, int64_t kz, int64_t S,
+                            double* __restrict__ P, int64_t mp) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= mp * mp) return;
+  const int64_t i = e / mp, j = e % mp;
+  if (i / TILE < j / TILE) return;
+  double v = 0.0;
+  for (int64_t z = 0; z < S; ++z) v += Pp[i * ldp + z * kz + j];
+  P[i * mp + j] = v;
+  if (i / TILE > j / TILE) P[j * mp + i] = v;
+}
+
 __global__ void mirror_kernel(double* __restrict__ A, int64_t ld, int64_t mp) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= mp * mp) return;
@@ -241,140 +187,79 @@ __global__ __launch_bounds__(256) void frob_kernel(const double* __restrict__ A,
 }
 
 // ---- E3: W = G_P Kzx + (c / s^2) r^T contracted against dKzx ----
+// One launch per chunk of points [c0, c0 + ncol).  Wt holds the chunk's
+// (G_P Kzx)^T, Wt[(g - c0) * mp + i], formed beforehand by one FP64 GEMM
+// Kc^T G_P (G_P is symmetric) from Kc = Kzx evaluated once for the chunk.
+// Block (mt, cb): inducing rows mt*TILE .. +TILE (thread row li = t % TILE; the
+// two halves of the block take alternate points) against points
+// c0 + cb*EW .. +EW.  The kernel value and its derivative factor are
+// recomputed from the inputs per element.  Partials, split = c0 / EW + cb:
+// gzpart[(split * mp + i) * DMAX + q] and
+// tpart[(split * mt_count + mt) * (DMAX + 1) + q] (lengthscales, then variance).
 template <int DMAX>
 __global__ __launch_bounds__(256) void elbo_wgrad_kernel(
-    int kind, const double* __restrict__ x, const double* __restrict__ y, int64_t n,
+    int kind, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, int64_t m, int d, const double* __restrict__ theta, int nls,
-    int64_t chunk, const double* __restrict__ GP, int64_t mp, const double* __restrict__ cvec,
-    double* __restrict__ gzpart, double* __restrict__ tpart) {
-  constexpr int DP = DMAX + 1;
-  constexpr size_t SLAB = (size_t)2 * ST * F64_LDP * sizeof(double);
-  constexpr size_t WB = (size_t)4 * 16 * 64 * sizeof(double);
-  constexpr size_t U = SLAB > WB ? SLAB : WB;
-  __shared__ __align__(16) char ubuf[U];
-  __shared__ double xc[TILE][DP];
-  __shared__ double zI[TILE][DP];
-  __shared__ double zsl[ST][DP];
+    const double* __restrict__ Wt, int64_t mp, int64_t c0, int64_t ncol,
+    const double* __restrict__ cvec, double* __restrict__ gzpart, double* __restrict__ tpart) {
+  __shared__ double xc[TILE][DMAX + 1];
   __shared__ double rc[TILE];
-  __shared__ double gbI[TILE];
-  __shared__ double gz[2][TILE][DMAX];
+  __shared__ double gzs[TILE][DMAX];
   __shared__ double red_s[4][DMAX + 1];
-  double* As = reinterpret_cast<double*>(ubuf);
-  double* Bs = As + ST * F64_LDP;
-  double* wbuf = reinterpret_cast<double*>(ubuf);
-
-  const int64_t mt = blockIdx.x, sidx = blockIdx.y;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w >> 1, wn = w & 1;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int li = t & (TILE - 1), half = t >> 7;
+  const int64_t mt = blockIdx.x, cb = blockIdx.y;
+  const int64_t gi = mt * TILE + li;
+  const bool rowok = gi < m;
   const double var = theta[nls], sd = theta[nls + 1], s = sd * sd, cmean = theta[nls + 2];
-  for (int e = t; e < TILE * DMAX; e += 256) {
-    const int r = e / DMAX, q = e % DMAX;
-    const int64_t gi = mt * TILE + r;
-    zI[r][q] = (q < d && gi < m) ? z[gi * d + q] / lsq(theta, nls, q) : 0.0;
-    gz[0][r][q] = 0.0;
-    gz[1][r][q] = 0.0;
-  }
-  if (t < TILE) {
-    const int64_t gi = mt * TILE + t;
-    gbI[t] = gi < m ? cvec[gi] / (s * s) : 0.0;
-  }
-  double s_var = 0.0;
-  double s_ls[DMAX];
+  double zi[DMAX], s_ls[DMAX], gzr[DMAX];
 #pragma unroll
-  for (int q = 0; q < DMAX; ++q) s_ls[q] = 0.0;
-
-  const int64_t n0 = sidx * chunk;
-  const int64_t n1 = (n0 + chunk < n) ? n0 + chunk : n;
-  for (int64_t c0 = n0; c0 < n1; c0 += TILE) {
+  for (int q = 0; q < DMAX; ++q) {
+    zi[q] = (rowok && q < d) ? z[gi * d + q] / lsq(theta, nls, q) : 0.0;
+    s_ls[q] = 0.0;
+    gzr[q] = 0.0;
+  }
+  const double gb = rowok ? cvec[gi] / (s * s) : 0.0;
+  double s_var = 0.0;
+  const int64_t p0 = c0 + cb * EW;
+  const int64_t p1 = (p0 + EW < c0 + ncol) ? p0 + EW : c0 + ncol;
+  for (int64_t b0 = p0; b0 < p1; b0 += TILE) {
     __syncthreads();
     for (int e = t; e < TILE * DMAX; e += 256) {
       const int r = e / DMAX, q = e % DMAX;
-      const int64_t g = c0 + r;
-      xc[r][q] = (q < d && g < n1) ? x[g * d + q] / lsq(theta, nls, q) : 0.0;
+      const int64_t g = b0 + r;
+      xc[r][q] = (q < d && g < p1) ? x[g * d + q] / lsq(theta, nls, q) : 0.0;
     }
-    if (t < TILE) rc[t] = (c0 + t < n1) ? y[c0 + t] - cmean : 0.0;
-    f64x4 acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
-    for (int64_t k0 = 0; k0 < mp; k0 += ST) {
-      __syncthreads();
-      {
-        // G_P rows of this tile, k-slab k0..k0+15 (row-contiguous), stored k-major.
-        const int row = t >> 1, ko = (t & 1) * 8;
-        const double* src = GP + (mt * TILE + row) * mp + k0 + ko;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) As[(ko + q) * F64_LDP + row] = src[q];
-      }
-      for (int e = t; e < ST * DMAX; e += 256) {
-        const int r = e / DMAX, q = e % DMAX;
-        const int64_t g = k0 + r;
-        zsl[r][q] = (q < d && g < m) ? z[g * d + q] / lsq(theta, nls, q) : 0.0;
-      }
-      __syncthreads();
-      {
-        const int kk = t & 15;
-        const bool kvalid = (k0 + kk) < m;
-#pragma unroll
-        for (int q8 = 0; q8 < 8; ++q8) {
-          const int col = (t >> 4) + 16 * q8;
-          double v = 0.0;
-          if (kvalid && c0 + col < n1) v = var * kval(kind, d2_of<DMAX>(zsl[kk], xc[col]));
-          Bs[kk * F64_LDP + col] = v;
-        }
-      }
-      __syncthreads();
-      mfma_slab_f64(As, Bs, F64_LDP, lane, wm, wn, acc);
-    }
-    __syncthreads();  // As/Bs become the per-lane strips
-    double* mine = wbuf + w * 16 * 64 + lane;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) mine[(j * 4 + r) * 64] = acc[i][j][r];
-#pragma unroll 1
-      for (int r = 0; r < 4; ++r) {
-        const int li = wm * 64 + i * 16 + acc_row<AR_F64>(lane, r);
-        const int64_t gi = mt * TILE + li;
-        double gzr[DMAX];
-#pragma unroll
-        for (int q = 0; q < DMAX; ++q) gzr[q] = 0.0;
-#pragma unroll 1
-        for (int j = 0; j < 4; ++j) {
-          const int lj = wn * 64 + j * 16 + (lane & 15);
-          if (gi < m && c0 + lj < n1) {
-            const double W = mine[(j * 4 + r) * 64] + gbI[li] * rc[lj];
-            const double d2 = d2_of<DMAX>(zI[li], xc[lj]);
-            double kv, fv;
-            kval_fac(kind, d2, kv, fv);
-            s_var += W * kv;
-            const double wf = W * fv;
-#pragma unroll
-            for (int q = 0; q < DMAX; ++q) {
-              const double df = zI[li][q] - xc[lj][q];
-              s_ls[q] += wf * df * df;
-              gzr[q] += wf * df;
-            }
-          }
-        }
+    if (t < TILE) rc[t] = (b0 + t < p1) ? y[b0 + t] - cmean : 0.0;
+    __syncthreads();
+    const int jend = (b0 + TILE < p1) ? TILE : (int)(p1 - b0);
+    if (rowok) {
+      const double* wcol = Wt + (b0 - c0) * mp + gi;
+      for (int jj = half; jj < jend; jj += 2) {
+        const double W = wcol[(int64_t)jj * mp] + gb * rc[jj];
+        const double d2 = d2_of<DMAX>(zi, xc[jj]);
+        double kv, fv;
+        kval_fac(kind, d2, kv, fv);
+        s_var += W * kv;
+        const double wf = W * fv;
 #pragma unroll
         for (int q = 0; q < DMAX; ++q) {
-          double v = gzr[q];
-          v += __shfl_xor(v, 1);
-          v += __shfl_xor(v, 2);
-          v += __shfl_xor(v, 4);
-          v += __shfl_xor(v, 8);
-          if ((lane & 15) == 0) gz[wn][li][q] += v;
+          const double df = zi[q] - xc[jj][q];
+          s_ls[q] += wf * df * df;
+          gzr[q] += wf * df;
         }
       }
     }
   }
+  if (half == 1) {
+#pragma unroll
+    for (int q = 0; q < DMAX; ++q) gzs[li][q] = gzr[q];
+  }
   __syncthreads();
-  for (int e = t; e < TILE * DMAX; e += 256) {
-    const int r = e / DMAX, q = e % DMAX;
-    gzpart[(sidx * mp + mt * TILE + r) * DMAX + q] = gz[0][r][q] + gz[1][r][q];
+  const int64_t split = c0 / EW + cb;
+  if (half == 0) {
+#pragma unroll
+    for (int q = 0; q < DMAX; ++q) gzpart[(split * mp + gi) * DMAX + q] = gzr[q] + gzs[li][q];
   }
 #pragma unroll
   for (int q = 0; q < DMAX + 1; ++q) {
@@ -384,10 +269,9 @@ __global__ __launch_bounds__(256) void elbo_wgrad_kernel(
     if (lane == 0) red_s[w][q] = v;
   }
   __syncthreads();
-  if (t < DMAX + 1) {
-    const int64_t blk = sidx * gridDim.x + blockIdx.x;
-    tpart[blk * (DMAX + 1) + t] = red_s[0][t] + red_s[1][t] + red_s[2][t] + red_s[3][t];
-  }
+  if (t < DMAX + 1)
+    tpart[(split * gridDim.x + mt) * (DMAX + 1) + t] =
+        red_s[0][t] + red_s[1][t] + red_s[2][t] + red_s[3][t];
 }
 
 // ---- E4: tr(G_K dKzz) terms; one thread per inducing point (row), G_K symmetric ----
@@ -592,14 +476,17 @@ __global__ __launch_bounds__(256) void elbo_grad_finalize(
 
 // ---- workspace layout ----
 struct ElboLayout {
-  int64_t mp = 0, mt = 0, ntile = 0, S1 = 1, chunk1 = ST, S2 = 1, chunk2 = TILE;
+  int64_t mp = 0, mt = 0, ntile = 0, S2 = 1, chunk2 = EW, nbw = TILE;
   int64_t nr = 256, nf = 1024;
   size_t off_K = 0, off_XK = 0, off_C = 0, off_XC = 0, off_P = 0, off_Kinv = 0, off_Cinv = 0;
   size_t off_G = 0, off_T1 = 0, off_T2 = 0, off_T = 0;
   size_t off_liK = 0, off_ldK = 0, off_ifK = 0, off_liC = 0, off_ldC = 0, off_ifC = 0;
   size_t off_b = 0, off_q = 0, off_c = 0, off_u = 0, off_Pc = 0;
-  size_t off_ppart = 0, off_bpart = 0, off_qpart = 0, off_gzpart = 0, off_tpart = 0;
+  size_t off_gzpart = 0, off_tpart = 0;
   size_t off_tpart2 = 0, off_gzkz = 0, off_rpart = 0, off_fpart = 0, off_vpart = 0, off_scal = 0;
+  // one chunk of Kzx [mp][nbw]; (G_P Kzx)^T [nbw][mp] in the backward, the
+  // split-K partials of P in the forward
+  size_t off_kc = 0, off_wt = 0;
   size_t total = 0;
 };
 
@@ -610,12 +497,12 @@ ElboLayout make_elbo_layout(int64_t n, int64_t m) {
   L.mp = round_up(m, 256);
   L.mt = L.mp / TILE;
   L.ntile = L.mt * (L.mt + 1) / 2;
-  const int64_t want1 = (2048 + L.ntile - 1) / L.ntile;
-  L.chunk1 = round_up((n + want1 - 1) / want1, ST);
-  L.S1 = (n + L.chunk1 - 1) / L.chunk1;
-  const int64_t want2 = (1024 + L.mt - 1) / L.mt;
-  L.chunk2 = round_up((n + want2 - 1) / want2, TILE);
-  L.S2 = (n + L.chunk2 - 1) / L.chunk2;
+  // chunk width in columns: the chunk's points rounded up, at least mp (the
+  // forward's split-K partials of P are mp columns wide within a row of nbw)
+  L.nbw = round_up(n, TILE) < BW_CHUNK ? round_up(n, TILE) : BW_CHUNK;
+  if (L.nbw < L.mp) L.nbw = L.mp;
+  L.chunk2 = EW;
+  L.S2 = (n + EW - 1) / EW;
   const int64_t mp = L.mp;
   const size_t M2 = (size_t)mp * mp * sizeof(double);
   const size_t V = (size_t)mp * sizeof(double);
@@ -643,9 +530,6 @@ ElboLayout make_elbo_layout(int64_t n, int64_t m) {
   L.off_c = o; o += al(V);
   L.off_u = o; o += al(V);
   L.off_Pc = o; o += al(V);
-  L.off_ppart = o; o += al((size_t)L.S1 * L.ntile * TILE * TILE * sizeof(double));
-  L.off_bpart = o; o += al((size_t)L.S1 * mp * sizeof(double));
-  L.off_qpart = o; o += al((size_t)L.S1 * mp * sizeof(double));
   L.off_gzpart = o; o += al((size_t)L.S2 * mp * EDMAX * sizeof(double));
   L.off_tpart = o; o += al((size_t)L.S2 * L.mt * (EDMAX + 1) * sizeof(double));
   L.off_tpart2 = o; o += al((size_t)(mp / 256) * (EDMAX + 1) * sizeof(double));
@@ -654,6 +538,8 @@ ElboLayout make_elbo_layout(int64_t n, int64_t m) {
   L.off_fpart = o; o += al((size_t)L.nf * sizeof(double));
   L.off_vpart = o; o += al((size_t)((mp + 1023) / 1024) * mp * sizeof(double));
   L.off_scal = o; o += al(32 * sizeof(double));
+  L.off_kc = o; o += al((size_t)mp * L.nbw * sizeof(double));
+  L.off_wt = o; o += al((size_t)L.nbw * mp * sizeof(double));
   L.total = o;
   return L;
 }
@@ -679,24 +565,37 @@ int elbo_forward_impl(int kind, const double* x, const double* y, int64_t n, con
   double* cvec = at<double>(ws, Lo.off_c);
   double* u = at<double>(ws, Lo.off_u);
   double* vpart = at<double>(ws, Lo.off_vpart);
-  // 1. streamed statistics
-  {
-    dim3 grid((unsigned)Lo.ntile, (unsigned)Lo.S1);
-#define GPK_E1(DM)                                                                             \
-  hipLaunchKernelGGL(elbo_stats_kernel<DM>, grid, dim3(256), 0, s, kind, x, y, n, z, m, d,     \
-                     theta, nls, Lo.chunk1, Lo.ntile, mp, at<double>(ws, Lo.off_ppart),        \
-                     at<double>(ws, Lo.off_bpart), at<double>(ws, Lo.off_qpart))
-    GPK_DMAX_DISPATCH(d, GPK_E1);
-#undef GPK_E1
+  // 1. streamed statistics, chunk by chunk: Kc = Kzx for the chunk's points
+  //    (evaluated once, zero outside m x ncol), P (+)= Kc Kc^T on the FP64
+  //    matrix cores (lower tiles), b (+)= Kc r, q (+)= Kc 1.
+  //    The lower tiles alone (136 at M = 2,048) would leave most of the chip
+  //    idle, so the GEMM is split over k into S members of one launch (the
+  //    batch descriptor shifts every operand by z * kz doubles): member z reads
+  //    columns z*kz .. +kz of Kc and accumulates its partial of P at column
+  //    z*kz of a [mp][nbw] buffer (the one that holds Wt in the backward).
+  double* Kc = at<double>(ws, Lo.off_kc);
+  double* Pp = at<double>(ws, Lo.off_wt);
+  const int64_t nbw = Lo.nbw;
+  int64_t S = 1;
+  while (S < 16 && (nbw / (2 * S)) % TILE == 0 && nbw / (2 * S) >= mp) S *= 2;
+  const int64_t kz = nbw / S;
+  Batch sk;
+  sk.ws_stride = kz * (int64_t)sizeof(double);
+  sk.count = (unsigned)S;
+  for (int64_t c0 = 0; c0 < n; c0 += BW_CHUNK) {
+    const int64_t ncol = (n - c0 < BW_CHUNK) ? n - c0 : BW_CHUNK;
+    GPK_TRY(launch_kernel_matrix<double>(kind, z, m, x + c0 * d, ncol, d, theta, nls, 0.0,
+                                         KM_ZERO_PAD, Kc, nbw, mp, nbw, false, s));
+    GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, false, Kc, nbw, Kc, nbw, Pp, nbw,
+                                                mp, mp, kz, 1.0, c0 > 0, true, KR_FULL, s,
+                                                nullptr, sk));
+    hipLaunchKernelGGL(chunk_rowsum_kernel, dim3((unsigned)mp), dim3(256), 0, s, Kc, nbw, c0,
+                       ncol, y, theta, nls, c0 > 0 ? 1 : 0, b, qv);
     GPK_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(stats_reduce_kernel, dim3((unsigned)Lo.ntile, TILE * TILE / 256), dim3(256),
-                     0, s, at<double>(ws, Lo.off_ppart), Lo.S1, Lo.ntile, P, mp);
+  hipLaunchKernelGGL(psum_kernel, dim3(cdiv(mp * mp, 256)), dim3(256), 0, s, Pp, nbw, kz, S, P,
+                     mp);
   GPK_LAUNCH_CHECK();
-  hipLaunchKernelGGL(vec_sum_kernel, dim3(cdiv(mp, 256)), dim3(256), 0, s,
-                     at<double>(ws, Lo.off_bpart), mp, Lo.S1, b);
-  hipLaunchKernelGGL(vec_sum_kernel, dim3(cdiv(mp, 256)), dim3(256), 0, s,
-                     at<double>(ws, Lo.off_qpart), mp, Lo.S1, qv);
   hipLaunchKernelGGL(rstats_kernel, dim3((unsigned)Lo.nr), dim3(256), 0, s, y, n, theta, nls,
                      at<double>(ws, Lo.off_rpart));
   GPK_LAUNCH_CHECK();
@@ -771,11 +670,22 @@ int elbo_backward_impl(int kind, const double* x, const double* y, int64_t n, co
   hipLaunchKernelGGL(gp_kernel, dim3(cdiv(mp * mp, 256)), dim3(256), 0, s, G, Kinv, Cinv, cvec,
                      mp, mp, theta, nls);
   GPK_LAUNCH_CHECK();
-  {
-    dim3 grid((unsigned)Lo.mt, (unsigned)Lo.S2);
-#define GPK_E3(DM)                                                                               \
-  hipLaunchKernelGGL(elbo_wgrad_kernel<DM>, grid, dim3(256), 0, s, kind, x, y, n, z, m, d, theta, \
-                     nls, Lo.chunk2, G, mp, cvec, at<double>(ws, Lo.off_gzpart),                  \
+  // Data pass, chunk by chunk: Kc = Kzx for the chunk's points (zero outside
+  // m x ncol), Wt = Kc^T G_P on the FP64 matrix cores, then the elementwise
+  // contraction against dKzx.
+  double* Kc = at<double>(ws, Lo.off_kc);
+  double* Wt = at<double>(ws, Lo.off_wt);
+  for (int64_t c0 = 0; c0 < n; c0 += BW_CHUNK) {
+    const int64_t ncol = (n - c0 < BW_CHUNK) ? n - c0 : BW_CHUNK;
+    const int64_t ncp = round_up(ncol, TILE);
+    GPK_TRY(launch_kernel_matrix<double>(kind, z, m, x + c0 * d, ncol, d, theta, nls, 0.0,
+                                         KM_ZERO_PAD, Kc, ncp, mp, ncp, false, s));
+    GPK_TRY(launch_gemm<double, double, double>(AR_F64, true, false, Kc, ncp, G, mp, Wt, mp, ncp,
+                                                mp, mp, 1.0, false, false, KR_FULL, s));
+    dim3 grid((unsigned)Lo.mt, (unsigned)((ncol + EW - 1) / EW));
+#define GPK_E3(DM)                                                                              \
+  hipLaunchKernelGGL(elbo_wgrad_kernel<DM>, grid, dim3(256), 0, s, kind, x, y, z, m, d, theta, \
+                     nls, Wt, mp, c0, ncol, cvec, at<double>(ws, Lo.off_gzpart),                \
                      at<double>(ws, Lo.off_tpart))
     GPK_DMAX_DISPATCH(d, GPK_E3);
 #undef GPK_E3

@@ -4,9 +4,10 @@ The oracle's streamed-statistics form (``collapsed_elbo_grad``) is itself
 pinned against the reference-form ``collapsed_elbo`` and against central
 differences and torch autograd in tests/test_oracle.py.  Tolerances (float64
 path): ELBO within 1e-9 relative, gradients within 1e-7 relative norm-wise
-(theta) and 1e-7 of the largest entry (Z).  The reference's own invariant,
-Z = X gives ELBO == MLL within 1e-6 (tests/test_objectives.py:193-199), is
-checked on the device too.
+(theta) and 1e-7 of the largest entry (Z), or 10x the Z gradient's own spread
+under ulp-level perturbations of X where that is larger.  The reference's own
+invariant, Z = X gives ELBO == MLL within 1e-6 (tests/test_objectives.py:193-199),
+is checked on the device too.
 """
 
 import numpy as np
@@ -65,7 +66,24 @@ def test_elbo_and_grad_parity(cuda, kind, n, m, d, ard, tol):
     gtol = 1e-7 if tol <= 1e-9 else 1e-5
     assert np.linalg.norm(g - gr) <= gtol * np.linalg.norm(gr), (g, gr)
     gz = q.inducing_inputs.value.grad.cpu().numpy()
-    assert np.max(np.abs(gz - ref["Z"])) <= gtol * np.max(np.abs(ref["Z"]))
+    zmax = np.max(np.abs(ref["Z"]))
+    ztol = max(gtol, 10 * _z_spread(kind, X, y, Z, ls, var, sd, mean, ref["Z"]) / zmax)
+    assert np.max(np.abs(gz - ref["Z"])) <= ztol * zmax
+
+
+def _z_spread(kind, X, y, Z, ls, var, sd, mean, gz_ref, draws=2):
+    """How far the oracle's inducing-input gradient moves when X is perturbed by
+    a few units in the last place (relative 1e-15): the digits the problem
+    determines.  For the Matern-1/2 case (Kzz condition number 3e5) it is 2e-7
+    of the largest entry, so two correct float64 evaluations that round
+    differently need not agree to 1e-7."""
+    rng = np.random.default_rng(0)
+    out = 0.0
+    for _ in range(draws):
+        Xp = X * (1.0 + 1e-15 * rng.standard_normal(X.shape))
+        g = O.collapsed_elbo_grad(kind, Xp, y, Z, ls, var, sd, 1e-6, mean)["Z"]
+        out = max(out, float(np.max(np.abs(g - gz_ref))))
+    return out
 
 
 def test_elbo_equals_mll_when_z_is_x(cuda):
