@@ -121,8 +121,8 @@ size_t gpk_elbo_workspace_bytes(int64_t n, int64_t m, int64_t d, int prec);
  * n data points x [n, d], y [n] and m inducing inputs z [m, d], with Kzz
  * jitter `jitter` (CollapsedVariationalGaussian, variational_families.py:767-784).
  * The data enter through P = Kzx Kxz, b = Kzx (y - c) and q = Kzx 1, which are
- * accumulated on the matrix cores from Kzx tiles evaluated on the fly: no
- * n x m matrix is stored.  result slots: [GPK_RES_MLL] = ELBO, [1] = log|B|,
+ * accumulated on the matrix cores chunk by chunk (Kzx evaluated once per chunk
+ * of 65,536 points into the workspace): no n x m matrix is stored.  result slots: [GPK_RES_MLL] = ELBO, [1] = log|B|,
  * [2] = the quadratic term, [3] = the trace term, [GPK_RES_INFO] = pivot status.
  * d <= 16; prec must be GPK_PREC_FP64.
  */
@@ -133,7 +133,8 @@ int gpk_elbo_forward(int kind, int prec, const double* x, const double* y, int64
 /*
  * Gradient of the ELBO (scaled by gbar_host) w.r.t. theta (layout above) into
  * grad [nls + 3] and w.r.t. the inducing inputs into grad_z [m, d].  Must follow
- * gpk_elbo_forward on the same workspace and stream; Kzx tiles are recomputed.
+ * gpk_elbo_forward on the same workspace and stream; Kzx is evaluated again,
+ * chunk by chunk.
  */
 int gpk_elbo_backward(int kind, int prec, const double* x, const double* y, int64_t n,
                       const double* z, int64_t m, int64_t d, const double* theta, int nls,
