@@ -17,7 +17,7 @@
 // k+1 is published to a double-buffered LDS vector by its owners right after
 // their update, together with sqrt and reciprocal of the new pivot (computed
 // once, by the owner of the diagonal entry), so each step costs one barrier.
-// The update is written with selects only (no per-entry branches).
+// The update is one FMA per entry, the masks folded into its factors.
 //
 // Outputs: L (lower, zero strict upper) into A, X = L^-1 (lower, zero strict
 // upper) into X, X in float64 into leafinv, 2*sum(log L_kk) into *logdet and
@@ -90,31 +90,38 @@ __global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_
       const double* col = col_s[k & 1];
       const double dk = dk_s[k];
       const double inv = dinv_s[k];
-      double sci[EB], scc[EB];
+      // Entry (i, j) changes at step k only if j > k and either i >= j (lower
+      // trailing update L[i][k] L[j][k]) or i <= k (strict upper: numerator of
+      // X[j][i] -= L[j][k] X[k][i]).  The j > k mask goes into the column
+      // factors, the i <= k mask into the row factors, and whether i >= j is
+      // fixed per thread: always (rb > cb), never (rb < cb), or a >= b on the
+      // diagonal block, which is known at compile time.  So every entry takes
+      // one FMA, with a zero factor where it must not change.
+      double sci[EB], alo[EB], ahi[EB], scc[EB];
 #pragma unroll
       for (int a = 0; a < EB; ++a) {
         const double v = col[rb + a] * inv;
         sci[a] = (rb + a == k) ? inv : v;
+        const double sx = (rb + a <= k) ? sci[a] : 0.0;
+        alo[a] = (rb >= cb) ? sci[a] : sx;  // entries with a >= b
+        ahi[a] = (rb > cb) ? sci[a] : sx;   // entries with a < b
       }
 #pragma unroll
-      for (int b = 0; b < EB; ++b) {
-        const double v = col[cb + b] * inv;
-        scc[b] = (cb + b == k) ? inv : v;
-      }
+      for (int b = 0; b < EB; ++b) scc[b] = (cb + b > k) ? col[cb + b] * inv : 0.0;
 #pragma unroll
-      for (int a = 0; a < EB; ++a) {
-        const int i = rb + a;
+      for (int a = 0; a < EB; ++a)
 #pragma unroll
-        for (int b = 0; b < EB; ++b) {
-          const int j = cb + b;
-          // column k: L[i][k] below the diagonal, X[k][i] (stored at (i, k)) above;
-          // lower part (i >= j > k): trailing update L[i][k] L[j][k];
-          // strict upper (i, j), i <= k < j: numerator of X[j][i] -= L[j][k] X[k][i]
-          const double u = s[a][b] - sci[a] * scc[b];
-          const double ck = (i == k) ? dk : sci[a];
-          const bool upd = j > k && (i >= j || i <= k);
-          s[a][b] = (j == k) ? ck : (upd ? u : s[a][b]);
-        }
+        for (int b = 0; b < EB; ++b)
+          s[a][b] = fma(-(a >= b ? alo[a] : ahi[a]), scc[b], s[a][b]);
+      // Column k itself: L[i][k] below the diagonal, the pivot on it, X[k][i]
+      // (stored at (i, k)) above; only the half-wave holding column k.
+      if (k >= cb) {
+#pragma unroll
+        for (int b = 0; b < EB; ++b)
+          if (cb + b == k) {
+#pragma unroll
+            for (int a = 0; a < EB; ++a) s[a][b] = (rb + a == k) ? dk : sci[a];
+          }
       }
       // Publish column k+1 and, from the owner of its diagonal entry, its pivot.
       const int q = k + 1 - cb;
