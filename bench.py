@@ -114,7 +114,10 @@ def other_workload(args) -> dict:
     batched -- 4,096 independent exact GPs (N_i in 32..1024, D = 4, RBF ARD), MLL +
                gradient of all of them per step;
     sparse  -- collapsed ELBO + gradient (theta and the 2,048 x 8 inducing inputs),
-               N = 1e6, D = 8, Matern-3/2 ARD, FP64."""
+               N = 1e6, D = 8, Matern-3/2 ARD, FP64;
+    predict -- ConjugatePosterior.predict with the diagonal variance at large T:
+               the matern workload's N = 16,384 training points, T = 1,000,000 test
+               points ~ N(0, I_8), FP64 (factorisation included in every step)."""
     import torch
 
     import gpjax_amd as gpx
@@ -172,6 +175,31 @@ def other_workload(args) -> dict:
                                         "(gpjax_amd.workloads.sparse)", "n": len(w["X"]),
                             "m": 2048, "d": 8}, elbo=float(e.detach()))
         cpu_n = 16384
+    elif args.workload == "predict":
+        import numpy as np
+        w = workloads.exact_matern()
+        n, t = len(w["X"]), 1_000_000
+        k = gpx.kernels.Matern52(lengthscale=w["lengthscale"], variance=w["variance"])
+        post = gpx.gps.Prior(mean_function=gpx.mean_functions.Zero(), kernel=k,
+                             jitter=w["jitter"]) * \
+            gpx.likelihoods.Gaussian(num_datapoints=n, obs_stddev=w["obs_stddev"])
+        D = gpx.Dataset(X=torch.tensor(w["X"], device=dev),
+                        y=torch.tensor(w["y"][:, None], device=dev))
+        T = torch.tensor(np.random.default_rng(7).standard_normal((t, 8)), device=dev)
+
+        def step():
+            return post.predict(T, D, full_covariance=False, precision="fp64")
+
+        tm, dist = timed(step, args.steps, args.warmup)
+        line.update(metric="test points/s, diagonal predictive, N=16,384, T=1e6 (FP64)",
+                    value=round(t / tm, 1), unit="test points/s", ms_per_step=round(tm * 1e3, 2),
+                    dtype="f64",
+                    config={"workload": "predict: matern training data, T=1e6 test points "
+                                        "(gpjax_amd.workloads.exact_matern)", "n": n, "t": t,
+                            "d": 8},
+                    all_finite=bool(torch.isfinite(dist.mean).all()
+                                    and torch.isfinite(dist.variance).all()))
+        cpu_n = None
     else:  # matern
         w = workloads.exact_matern()
         n = len(w["X"])
@@ -206,7 +234,8 @@ def other_workload(args) -> dict:
                     accuracy={"rel_mll_vs_fp64": abs(mll_m - mll_d) / abs(mll_d),
                               "rel_grad_vs_fp64": rel_g})
         cpu_n = 4096
-    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(cpu_n, args.workload)
+    line["cpu_baseline"] = None if (args.no_cpu or cpu_n is None) else \
+        cpu_baseline(cpu_n, args.workload)
     return line
 
 
@@ -229,7 +258,7 @@ def main() -> int:
                     help="only run a plain device-to-device copy of BYTES bytes 5 times "
                          "(reference for rocprof FETCH_SIZE / WRITE_SIZE scaling) and exit")
     ap.add_argument("--workload", default="exact_large",
-                    choices=["exact_large", "matern", "batched", "sparse"],
+                    choices=["exact_large", "matern", "batched", "sparse", "predict"],
                     help="exact_large (default) is the headline; the others print their own line")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -393,8 +393,14 @@ int mll_backward_t(int kind, const double* x, int64_t n, int d, const double* th
 }
 
 // ---- predict: ConjugatePosterior.predict (gpjax/gps.py:494-526) ----
+// tc: test points per pass.  The full covariance needs all of them at once (its
+// T x T block is the output anyway); the diagonal mode streams them in passes
+// whose two np x tc buffers (Kxt and A) hold at most PRED_ELEMS doubles each,
+// so T is bounded only by the caller's mean and variance vectors.
+constexpr int64_t PRED_ELEMS = (int64_t)1 << 26;  // 512 MB per buffer
+
 struct PredLayout {
-  int64_t tp = 0;
+  int64_t tp = 0, tc = 0;
   size_t off_kxt = 0, off_a = 0, off_ktt = 0, off_part = 0, off_vec = 0, off_res = 0, total = 0;
 };
 
@@ -402,12 +408,19 @@ PredLayout make_pred_layout(const Layout& Lo, int64_t t, int full_cov) {
   PredLayout P;
   P.tp = round_up(t, NB);
   const int64_t np = Lo.np, tp = P.tp;
+  int64_t tc = tp;
+  if (!full_cov) {
+    tc = (PRED_ELEMS / np) / NB * NB;
+    if (tc < NB) tc = NB;
+    if (tc > tp) tc = tp;
+  }
+  P.tc = tc;
   size_t o = Lo.total;
-  P.off_kxt = o; o += al((size_t)np * tp * sizeof(double));
-  P.off_a = o; o += al((size_t)np * tp * sizeof(double));
+  P.off_kxt = o; o += al((size_t)np * tc * sizeof(double));
+  P.off_a = o; o += al((size_t)np * tc * sizeof(double));
   P.off_ktt = o; o += al((size_t)(full_cov ? tp * tp : 1) * sizeof(double));
-  P.off_part = o; o += al((size_t)((np + 1023) / 1024) * tp * sizeof(double));
-  P.off_vec = o; o += al((size_t)tp * sizeof(double));
+  P.off_part = o; o += al((size_t)((np + 1023) / 1024) * tc * sizeof(double));
+  P.off_vec = o; o += al((size_t)tc * sizeof(double));
   P.off_res = o; o += al((size_t)GPK_RESULT_LEN * sizeof(double));
   P.total = o;
   return P;
@@ -440,17 +453,40 @@ int predict_t(int kind, int prec, const double* x, const double* y, int64_t n, i
   double* A = at<double>(ws, P.off_a);
   double* part = at<double>(ws, P.off_part);
   double* vec = at<double>(ws, P.off_vec);
+  const double* alpha = at<double>(ws, Lo.off_alpha);
+  const TS* X = at<TS>(ws, Lo.off_X);
+  if (!full_cov) {
+    // Diagonal only, the test points in passes of tc:
+    //   mean_j = m(t) + Kxt^T alpha,  var_j = k(0) + jitter_prior - sum_i A_ij^2.
+    for (int64_t t0 = 0; t0 < t; t0 += P.tc) {
+      const int64_t tn = (t - t0 < P.tc) ? t - t0 : P.tc;
+      const int64_t tnp = round_up(tn, NB);
+      GPK_TRY(launch_kernel_matrix<double>(kind, x, n, xt + t0 * d, tn, d, theta, nls, 0.0,
+                                           KM_ZERO_PAD, kxt, tnp, np, tnp, false, s));
+      GPK_TRY(launch_colreduce(kxt, tnp, np, tnp, alpha, vec, part, s));
+      hipLaunchKernelGGL(pred_mean_kernel, dim3(nblk(tn)), dim3(256), 0, s, vec, theta, nls, tn,
+                         mean + t0);
+      GPK_LAUNCH_CHECK();
+      GPK_TRY(launch_gemm<TS, double, double>(AR_F64, false, true, X, np, kxt, tnp, A, tnp, np,
+                                              tnp, np, 1.0, false, false, KR_LE_M, s));
+      GPK_TRY(launch_colreduce(A, tnp, np, tnp, nullptr, vec, part, s));
+      hipLaunchKernelGGL(pred_var_kernel, dim3(nblk(tn)), dim3(256), 0, s, vec, theta, nls,
+                         jitter_prior, tn, cov + t0);
+      GPK_LAUNCH_CHECK();
+    }
+    return GPK_OK;
+  }
   // Kxt (gps.py:515), zero outside n x t
   GPK_TRY(launch_kernel_matrix<double>(kind, x, n, xt, t, d, theta, nls, 0.0, KM_ZERO_PAD, kxt,
                                        tp, np, tp, false, s));
   // mean = m(t) + Kxt^T alpha  (= m(t) + (L^-1 Kxt)^T L^-1 (y - m), gps.py:517-520)
-  GPK_TRY(launch_colreduce(kxt, tp, np, tp, at<double>(ws, Lo.off_alpha), vec, part, s));
+  GPK_TRY(launch_colreduce(kxt, tp, np, tp, alpha, vec, part, s));
   hipLaunchKernelGGL(pred_mean_kernel, dim3(nblk(t)), dim3(256), 0, s, vec, theta, nls, t, mean);
   GPK_LAUNCH_CHECK();
   // A = L^-1 Kxt  (X lower: k <= row tile)
-  GPK_TRY(launch_gemm<TS, double, double>(AR_F64, false, true, at<TS>(ws, Lo.off_X), np, kxt, tp,
-                                          A, tp, np, tp, np, 1.0, false, false, KR_LE_M, s));
-  if (full_cov) {
+  GPK_TRY(launch_gemm<TS, double, double>(AR_F64, false, true, X, np, kxt, tp, A, tp, np, tp, np,
+                                          1.0, false, false, KR_LE_M, s));
+  {
     // cov = Ktt + jitter_prior I - A^T A  (gps.py:514,522-523)
     double* ktt = at<double>(ws, P.off_ktt);
     GPK_TRY(launch_kernel_matrix<double>(kind, xt, t, xt, t, d, theta, nls, jitter_prior,
@@ -459,12 +495,6 @@ int predict_t(int kind, int prec, const double* x, const double* y, int64_t n, i
                                                 np, -1.0, true, false, KR_FULL, s));
     GPK_HIP_TRY(hipMemcpy2DAsync(cov, (size_t)t * sizeof(double), ktt, (size_t)tp * sizeof(double),
                                  (size_t)t * sizeof(double), (size_t)t, hipMemcpyDeviceToDevice, s));
-  } else {
-    // diagonal only: var_j = k(0) + jitter_prior - sum_i A_ij^2
-    GPK_TRY(launch_colreduce(A, tp, np, tp, nullptr, vec, part, s));
-    hipLaunchKernelGGL(pred_var_kernel, dim3(nblk(t)), dim3(256), 0, s, vec, theta, nls,
-                       jitter_prior, t, cov);
-    GPK_LAUNCH_CHECK();
   }
   return GPK_OK;
 }

@@ -756,8 +756,13 @@ int check_elbo(int kind, int prec, const double* x, const double* y, int64_t n, 
 // where |M|^2 = M^T M and L_C^-1 = L^-1 Lz^-1 (L = chol(I + A A^T)).  The
 // forward leaves X_K = Lz^-1, X_C = L_C^-1 and C^-1 b (b = Kzx (y - c)) in the
 // workspace; Kzt is formed once, zero-padded to mp x tp.
+// tc: test points per pass (all of them for the full covariance; in the
+// diagonal mode passes whose three mp x tc buffers hold at most 2^26 doubles
+// each, so T is bounded only by the caller's output vectors).
+constexpr int64_t EPRED_ELEMS = (int64_t)1 << 26;
+
 struct ElboPredLayout {
-  int64_t tp = 0;
+  int64_t tp = 0, tc = 0;
   size_t off_kzt = 0, off_a1 = 0, off_a2 = 0, off_ktt = 0, off_part = 0, off_v1 = 0, off_v2 = 0;
   size_t off_res = 0, total = 0;
 };
@@ -766,14 +771,21 @@ ElboPredLayout make_elbo_pred_layout(const ElboLayout& Lo, int64_t t, int full_c
   ElboPredLayout P;
   P.tp = round_up(t, NB);
   const int64_t mp = Lo.mp, tp = P.tp;
+  int64_t tc = tp;
+  if (!full_cov) {
+    tc = (EPRED_ELEMS / mp) / NB * NB;
+    if (tc < NB) tc = NB;
+    if (tc > tp) tc = tp;
+  }
+  P.tc = tc;
   size_t o = Lo.total;
-  P.off_kzt = o; o += al((size_t)mp * tp * sizeof(double));
-  P.off_a1 = o; o += al((size_t)mp * tp * sizeof(double));
-  P.off_a2 = o; o += al((size_t)mp * tp * sizeof(double));
+  P.off_kzt = o; o += al((size_t)mp * tc * sizeof(double));
+  P.off_a1 = o; o += al((size_t)mp * tc * sizeof(double));
+  P.off_a2 = o; o += al((size_t)mp * tc * sizeof(double));
   P.off_ktt = o; o += al((size_t)(full_cov ? tp * tp : 1) * sizeof(double));
-  P.off_part = o; o += al((size_t)((mp + 1023) / 1024) * tp * sizeof(double));
-  P.off_v1 = o; o += al((size_t)tp * sizeof(double));
-  P.off_v2 = o; o += al((size_t)tp * sizeof(double));
+  P.off_part = o; o += al((size_t)((mp + 1023) / 1024) * tc * sizeof(double));
+  P.off_v1 = o; o += al((size_t)tc * sizeof(double));
+  P.off_v2 = o; o += al((size_t)tc * sizeof(double));
   P.off_res = o; o += al(GPK_RESULT_LEN * sizeof(double));
   P.total = o;
   return P;
@@ -809,6 +821,29 @@ int elbo_predict_impl(int kind, const double* x, const double* y, int64_t n, con
   double* v2 = at<double>(ws, P.off_v2);
   const double* XK = at<double>(ws, Lo.off_XK);
   const double* XC = at<double>(ws, Lo.off_XC);
+  if (!full_cov) {
+    // Diagonal only, the test points in passes of tc.
+    for (int64_t t0 = 0; t0 < t; t0 += P.tc) {
+      const int64_t tn = (t - t0 < P.tc) ? t - t0 : P.tc;
+      const int64_t tnp = round_up(tn, NB);
+      GPK_TRY(launch_kernel_matrix<double>(kind, z, m, xt + t0 * d, tn, d, theta, nls, 0.0,
+                                           KM_ZERO_PAD, kzt, tnp, mp, tnp, false, s));
+      GPK_TRY(launch_colreduce(kzt, tnp, mp, tnp, at<double>(ws, Lo.off_c), v1, part, s));
+      hipLaunchKernelGGL(sgpr_mean_kernel, dim3(cdiv(tn, 256)), dim3(256), 0, s, v1, theta, nls,
+                         tn, mean + t0);
+      GPK_LAUNCH_CHECK();
+      GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, true, XK, mp, kzt, tnp, a1, tnp,
+                                                  mp, tnp, mp, 1.0, false, false, KR_LE_M, s));
+      GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, true, XC, mp, kzt, tnp, a2, tnp,
+                                                  mp, tnp, mp, 1.0, false, false, KR_LE_M, s));
+      GPK_TRY(launch_colreduce(a1, tnp, mp, tnp, nullptr, v1, part, s));
+      GPK_TRY(launch_colreduce(a2, tnp, mp, tnp, nullptr, v2, part, s));
+      hipLaunchKernelGGL(sgpr_var_kernel, dim3(cdiv(tn, 256)), dim3(256), 0, s, v1, v2, theta,
+                         nls, jitter, tn, cov + t0);
+      GPK_LAUNCH_CHECK();
+    }
+    return GPK_OK;
+  }
   GPK_TRY(launch_kernel_matrix<double>(kind, z, m, xt, t, d, theta, nls, 0.0, KM_ZERO_PAD, kzt,
                                        tp, mp, tp, false, s));
   // mean: c + Kzt^T (C^-1 b) / s
@@ -821,7 +856,7 @@ int elbo_predict_impl(int kind, const double* x, const double* y, int64_t n, con
                                               tp, mp, 1.0, false, false, KR_LE_M, s));
   GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, true, XC, mp, kzt, tp, a2, tp, mp,
                                               tp, mp, 1.0, false, false, KR_LE_M, s));
-  if (full_cov) {
+  {
     double* ktt = at<double>(ws, P.off_ktt);
     GPK_TRY(launch_kernel_matrix<double>(kind, xt, t, xt, t, d, theta, nls, jitter,
                                          1 | KM_ZERO_PAD, ktt, tp, tp, tp, false, s));
@@ -832,12 +867,6 @@ int elbo_predict_impl(int kind, const double* x, const double* y, int64_t n, con
     GPK_HIP_TRY(hipMemcpy2DAsync(cov, (size_t)t * sizeof(double), ktt, (size_t)tp * sizeof(double),
                                  (size_t)t * sizeof(double), (size_t)t, hipMemcpyDeviceToDevice,
                                  s));
-  } else {
-    GPK_TRY(launch_colreduce(a1, tp, mp, tp, nullptr, v1, part, s));
-    GPK_TRY(launch_colreduce(a2, tp, mp, tp, nullptr, v2, part, s));
-    hipLaunchKernelGGL(sgpr_var_kernel, dim3(cdiv(t, 256)), dim3(256), 0, s, v1, v2, theta, nls,
-                       jitter, t, cov);
-    GPK_LAUNCH_CHECK();
   }
   return GPK_OK;
 }

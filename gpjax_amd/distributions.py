@@ -1,8 +1,10 @@
 """Multivariate normal returned by ``predict`` (``gpjax/distributions.py:42-108``).
 
-Holds a mean vector and a dense covariance (torch float64); ``variance`` and
-``stddev`` read the diagonal.  ``log_prob`` of a dense covariance goes through
-the same C-ABI factorisation as ``conjugate_mll``.
+Holds a mean vector and a covariance (torch float64): either the dense [T, T]
+matrix or, for a diagonal covariance (``predict(..., full_covariance=False)``),
+the [T] vector of variances, so that a predictive at large T needs O(T) memory.
+``variance`` and ``stddev`` read the diagonal; ``covariance()`` forms the dense
+matrix (for a diagonal one, only when asked).
 """
 
 from __future__ import annotations
@@ -12,8 +14,14 @@ import torch
 
 class GaussianDistribution:
     def __init__(self, loc: torch.Tensor, scale: torch.Tensor):
+        if scale.ndim not in (1, 2):
+            raise ValueError("scale must be a [T, T] covariance or a [T] diagonal")
         self.loc = loc
         self.scale = scale
+
+    @property
+    def is_diagonal(self) -> bool:
+        return self.scale.ndim == 1
 
     @property
     def mean(self) -> torch.Tensor:
@@ -21,17 +29,17 @@ class GaussianDistribution:
 
     @property
     def variance(self) -> torch.Tensor:
-        return torch.diagonal(self.scale)
+        return self.scale if self.is_diagonal else torch.diagonal(self.scale)
 
     def stddev(self) -> torch.Tensor:
         return torch.sqrt(self.variance)
 
     def covariance(self) -> torch.Tensor:
-        return self.scale
+        return torch.diag(self.scale) if self.is_diagonal else self.scale
 
     @property
     def covariance_matrix(self) -> torch.Tensor:
-        return self.scale
+        return self.covariance()
 
     @property
     def event_shape(self):

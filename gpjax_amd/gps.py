@@ -51,9 +51,10 @@ class ConjugatePosterior:
 
         Sigma uses this posterior's jitter (``gps.py:506``) and the predictive
         covariance the prior's (``gps.py:523``).  With ``full_covariance=False``
-        only the diagonal is formed (a [T] variance vector is held as a diagonal
-        matrix), which avoids the T x T product for large T.  Values only: no
-        autograd graph is recorded through predict.
+        only the diagonal is formed and returned as a [T] variance vector (the
+        test points pass through the device in passes), so large T costs O(T)
+        memory rather than a T x T product.  Values only: no autograd graph is
+        recorded through predict.
         """
         from . import _lib, config
         from .objectives import _mean_constant, _precision_id
@@ -81,8 +82,6 @@ class ConjugatePosterior:
                       _lib.ptr(T), t, _lib.ptr(theta), nls, float(self.jitter),
                       float(self.prior.jitter), full, iters, _lib.ptr(ws), nbytes,
                       _lib.ptr(mean), _lib.ptr(cov), _lib.stream_ptr(X.device))
-        if not full:
-            cov = torch.diag(cov)
         return GaussianDistribution(mean, cov)
 
 

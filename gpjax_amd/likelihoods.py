@@ -17,8 +17,11 @@ class Gaussian:
 
     def predict(self, dist: GaussianDistribution) -> GaussianDistribution:
         """Add obs_stddev^2 to the covariance diagonal (``likelihoods.py:182-186``)."""
+        noise = self.obs_stddev.value.detach() ** 2
+        if dist.is_diagonal:
+            return GaussianDistribution(dist.mean, dist.variance + noise)
         cov = dist.covariance().clone()
-        cov.diagonal().add_(self.obs_stddev.value.detach() ** 2)
+        cov.diagonal().add_(noise)
         return GaussianDistribution(dist.mean, cov)
 
     def __call__(self, dist: GaussianDistribution) -> GaussianDistribution:

@@ -41,7 +41,8 @@ class CollapsedVariationalGaussian:
         cov = Ktt - Ktz Kzz^-1 Kzt + Ktz C^-1 Kzt + jitter I with
         C = Kzz + jitter I + Kzx Kxz / s, s = obs_stddev^2.  The training data enter
         through the same N-summed statistics as the ELBO.  ``full_covariance=False``
-        forms only the diagonal.  Values only: no autograd graph is recorded.
+        forms only the diagonal, returned as a [T] variance vector (O(T) memory).
+        Values only: no autograd graph is recorded.
         """
         from . import _lib
         from .objectives import _mean_constant
@@ -75,6 +76,4 @@ class CollapsedVariationalGaussian:
                       _lib.ptr(Z), m, d, _lib.ptr(T), t, _lib.ptr(theta), nls,
                       float(self.jitter), full, _lib.ptr(ws), nbytes, _lib.ptr(mean),
                       _lib.ptr(cov), _lib.stream_ptr(dev))
-        if not full:
-            cov = torch.diag(cov)
         return GaussianDistribution(mean, cov)

@@ -175,3 +175,32 @@ def test_collapsed_predict_tiny_shapes(cuda, n, m, t, full):
     cov = pred.covariance().cpu().numpy()
     ref = cov_ref if full else np.diag(np.diag(cov_ref))
     np.testing.assert_allclose(cov, ref, rtol=1e-9, atol=1e-12)
+
+
+def test_collapsed_predict_diagonal_streams_large_t(cuda):
+    """The collapsed diagonal predictive streams the test points in passes
+    (32,768 points per pass at M = 2,048).  A slice straddling a pass boundary,
+    predicted on its own, must match exactly, and the first points match the
+    full-covariance path's diagonal."""
+    rng = np.random.default_rng(5)
+    n, m, d, t = 20000, 2048, 2, 70000
+    X = rng.uniform(0, 1, (n, d))
+    y = np.sin(2 * np.pi * X).sum(1) + 0.2 * rng.standard_normal(n)
+    Z = X[rng.choice(n, m, replace=False)] + 0.001
+    T = rng.uniform(-0.1, 1.1, (t, d))
+    k = gpx.kernels.Matern32(lengthscale=np.array([0.3, 0.5]), variance=1.1)
+    mf = gpx.mean_functions.Constant(gpx.parameters.Real(0.1))
+    post = gpx.gps.Prior(mean_function=mf, kernel=k) * \
+        gpx.likelihoods.Gaussian(num_datapoints=n, obs_stddev=0.2)
+    q = gpx.variational_families.CollapsedVariationalGaussian(posterior=post, inducing_inputs=Z)
+    D = gpx.Dataset(X=torch.tensor(X, device=cuda), y=torch.tensor(y[:, None], device=cuda))
+    big = q.predict(torch.tensor(T, device=cuda), D, full_covariance=False)
+    mu, var = big.mean.cpu().numpy(), big.variance.cpu().numpy()
+    assert np.all(np.isfinite(mu)) and np.all(var > 0)
+    part = q.predict(torch.tensor(T[30000:36000], device=cuda), D, full_covariance=False)
+    np.testing.assert_array_equal(part.mean.cpu().numpy(), mu[30000:36000])
+    np.testing.assert_array_equal(part.variance.cpu().numpy(), var[30000:36000])
+    full = q.predict(torch.tensor(T[:200], device=cuda), D, full_covariance=True)
+    np.testing.assert_allclose(full.mean.cpu().numpy(), mu[:200], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(np.diag(full.covariance().cpu().numpy()), var[:200], rtol=0,
+                               atol=1e-11)

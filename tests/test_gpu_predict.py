@@ -73,3 +73,29 @@ def test_predict_mixed_envelope_small_1d(cuda):
     w = workloads.exact_small()
     _check(cuda, w["kind"], w["X"], w["y"], w["T"], w["lengthscale"], w["variance"],
            w["obs_stddev"], 0.0, "mixed", 1e-4, 1e-3)
+
+
+def test_predict_diagonal_streams_large_t(cuda):
+    """The diagonal predictive streams the test points in passes (512 MB per
+    buffer: 4,096 points per pass at N = 16,384).  Every test point's mean and
+    variance depend only on its own column, so a slice that straddles pass
+    boundaries, predicted in a call of its own, must match exactly; the first
+    points also match the full-covariance path's diagonal."""
+    rng = np.random.default_rng(21)
+    n, d, t = 16384, 3, 10000
+    X = rng.uniform(-3, 3, (n, d))
+    y = np.sin(X[:, 0]) * np.cos(X[:, 1]) + 0.1 * rng.standard_normal(n)
+    T = rng.uniform(-3.5, 3.5, (t, d))
+    post = _post("matern32", np.array([0.9, 1.4, 2.0]), 1.1, 0.1, n, 0.2)
+    D = gpx.Dataset(X=torch.tensor(X, device=cuda), y=torch.tensor(y[:, None], device=cuda))
+    big = post.predict(torch.tensor(T, device=cuda), D, full_covariance=False, precision="fp64")
+    mu, var = big.mean.cpu().numpy(), big.variance.cpu().numpy()
+    assert np.all(np.isfinite(mu)) and np.all(var > 0)
+    part = post.predict(torch.tensor(T[3000:9000], device=cuda), D, full_covariance=False,
+                        precision="fp64")
+    np.testing.assert_array_equal(part.mean.cpu().numpy(), mu[3000:9000])
+    np.testing.assert_array_equal(part.variance.cpu().numpy(), var[3000:9000])
+    full = post.predict(torch.tensor(T[:200], device=cuda), D, precision="fp64")
+    np.testing.assert_allclose(full.mean.cpu().numpy(), mu[:200], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(np.diag(full.covariance().cpu().numpy()), var[:200], rtol=0,
+                               atol=1e-12)
