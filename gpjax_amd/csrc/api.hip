@@ -458,6 +458,11 @@ int predict_t(int kind, int prec, const double* x, const double* y, int64_t n, i
   if (!full_cov) {
     // Diagonal only, the test points in passes of tc:
     //   mean_j = m(t) + Kxt^T alpha,  var_j = k(0) + jitter_prior - sum_i A_ij^2.
+    // With float32 storage (mixed) A = X Kxt runs as six-product bf16 splits of
+    // X and a float32 copy of Kxt, written in float64, like the Cholesky updates;
+    // the mean takes the float64 Kxt first, and the copy then reuses its buffer.
+    const bool f64 = std::is_same<TS, double>::value;
+    float* kxtf = reinterpret_cast<float*>(kxt);
     for (int64_t t0 = 0; t0 < t; t0 += P.tc) {
       const int64_t tn = (t - t0 < P.tc) ? t - t0 : P.tc;
       const int64_t tnp = round_up(tn, NB);
@@ -467,9 +472,19 @@ int predict_t(int kind, int prec, const double* x, const double* y, int64_t n, i
       hipLaunchKernelGGL(pred_mean_kernel, dim3(nblk(tn)), dim3(256), 0, s, vec, theta, nls, tn,
                          mean + t0);
       GPK_LAUNCH_CHECK();
-      GPK_TRY(launch_gemm<TS, double, double>(AR_F64, false, true, X, np, kxt, tnp, A, tnp, np,
-                                              tnp, np, 1.0, false, false, KR_LE_M, s));
-      GPK_TRY(launch_colreduce(A, tnp, np, tnp, nullptr, vec, part, s));
+      if (f64) {
+        GPK_TRY(launch_gemm<TS, double, double>(AR_F64, false, true, X, np, kxt, tnp, A, tnp, np,
+                                                tnp, np, 1.0, false, false, KR_LE_M, s));
+        GPK_TRY(launch_colreduce(A, tnp, np, tnp, nullptr, vec, part, s));
+      } else {
+        GPK_TRY(launch_kernel_matrix<float>(kind, x, n, xt + t0 * d, tn, d, theta, nls, 0.0,
+                                            KM_ZERO_PAD, kxtf, tnp, np, tnp, false, s));
+        GPK_TRY(launch_gemm<float, float, double>(AR_BF16X6, false, true,
+                                                  reinterpret_cast<const float*>(X), np, kxtf,
+                                                  tnp, A, tnp, np, tnp, np, 1.0, false, false,
+                                                  KR_LE_M, s));
+        GPK_TRY(launch_colreduce(A, tnp, np, tnp, nullptr, vec, part, s));
+      }
       hipLaunchKernelGGL(pred_var_kernel, dim3(nblk(tn)), dim3(256), 0, s, vec, theta, nls,
                          jitter_prior, tn, cov + t0);
       GPK_LAUNCH_CHECK();

@@ -142,18 +142,17 @@ int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const
   GemmParams p{A, lda, B, ldb, C, ldc, K, alpha, beta_one ? 1 : 0, lower_tiles ? 1 : 0, krange,
                cdiag, bt, fscale};
   if (arith == AR_F64) return dispatch_layout<AR_F64, TA, TB, TC>(a_km, b_kn, p, M, N, s);
-  if constexpr (std::is_same<TA, float>::value && std::is_same<TB, float>::value &&
-                std::is_same<TC, float>::value) {
+  if constexpr (std::is_same<TA, float>::value && std::is_same<TB, float>::value) {
     if (arith == AR_BF16X3)
-      return dispatch_layout<AR_BF16X3, float, float, float>(a_km, b_kn, p, M, N, s);
+      return dispatch_layout<AR_BF16X3, float, float, TC>(a_km, b_kn, p, M, N, s);
     if (arith == AR_BF16X6)
-      return dispatch_layout<AR_BF16X6, float, float, float>(a_km, b_kn, p, M, N, s);
+      return dispatch_layout<AR_BF16X6, float, float, TC>(a_km, b_kn, p, M, N, s);
     if (arith == AR_FP16X3) {
       if (!fscale) {
         set_error("gemm: fp16 products need the operand scales");
         return GPK_ERR_BAD_ARG;
       }
-      return dispatch_layout<AR_FP16X3, float, float, float>(a_km, b_kn, p, M, N, s);
+      return dispatch_layout<AR_FP16X3, float, float, TC>(a_km, b_kn, p, M, N, s);
     }
   }
   set_error("gemm: unsupported arithmetic %d for these operand types", arith);
@@ -169,6 +168,7 @@ GPK_GEMM_INST(double, double, double)
 GPK_GEMM_INST(float, float, float)
 GPK_GEMM_INST(float, double, float)
 GPK_GEMM_INST(float, double, double)
+GPK_GEMM_INST(float, float, double)
 #undef GPK_GEMM_INST
 
 }  // namespace gpk
