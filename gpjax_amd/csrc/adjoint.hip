@@ -52,7 +52,7 @@ __global__ __launch_bounds__(WaveLayout<AR>::NT) void adjoint_kernel(int kind,
   __shared__ double red_s[WL::NW][DMAX + 2];
 
   int64_t mt, nt;
-  lower_tile(blockIdx.x, mt, nt);
+  lower_tile_banded(blockIdx.x, np / TILE, mt, nt);
   const int64_t m0 = mt * TILE, n0 = nt * TILE;
   const int64_t kb = dense ? 0 : m0;
   typename AccT<AR>::vec acc[4][WL::NJ];

@@ -23,6 +23,7 @@ struct GemmParams {
   double* cdiag;  // lower_tiles: diagonal tiles update this float64 [tile][NB][NB] instead of C
   Batch bt;       // batch of same-shape problems, member blockIdx.z (operands in workspaces)
   const float* fscale;  // AR_FP16X3: power-of-two scales of A and B (device)
+  int64_t tiles_m;      // M / TILE (the lower-tile decode needs the side)
 };
 
 template <int AR, typename TA, typename TB, typename TC, bool A_KM, bool B_KN>
@@ -33,13 +34,15 @@ __global__ __launch_bounds__(WaveLayout<AR>::NT) void gemm_kernel(GemmParams p) 
   p.B = bws(p.B, p.bt);
   p.C = bws(p.C, p.bt);
   p.cdiag = bws(p.cdiag, p.bt);
+  // Banded tile orders (16 x 16 blocks of tiles resident together share their
+  // operand panels in L2).
   int64_t mt, nt;
   if (p.lower_tiles) {
-    lower_tile(blockIdx.x, mt, nt);
+    lower_tile_banded(blockIdx.x, p.tiles_m, mt, nt);
   } else {
-    nt = blockIdx.x;
+    grid_tile_banded((int64_t)blockIdx.y * gridDim.x + blockIdx.x, gridDim.y, gridDim.x, mt, nt);
     // k <= row tile: the last tile rows carry the most work, so dispatch them first.
-    mt = p.krange == KR_LE_M ? gridDim.y - 1 - blockIdx.y : blockIdx.y;
+    if (p.krange == KR_LE_M) mt = gridDim.y - 1 - mt;
   }
   int64_t kb, ke;
   k_range(p.krange, mt, nt, p.K, kb, ke);
@@ -140,7 +143,7 @@ int launch_gemm(int arith, bool a_km, bool b_kn, const TA* A, int64_t lda, const
     return GPK_ERR_BAD_ARG;
   }
   GemmParams p{A, lda, B, ldb, C, ldc, K, alpha, beta_one ? 1 : 0, lower_tiles ? 1 : 0, krange,
-               cdiag, bt, fscale};
+               cdiag, bt, fscale, M / TILE};
   if (arith == AR_F64) return dispatch_layout<AR_F64, TA, TB, TC>(a_km, b_kn, p, M, N, s);
   if constexpr (std::is_same<TA, float>::value && std::is_same<TB, float>::value) {
     if (arith == AR_BF16X3)

@@ -443,6 +443,50 @@ __device__ __forceinline__ void lower_tile(int64_t t, int64_t& mt, int64_t& nt) 
   nt = t - a * (a + 1) / 2;
 }
 
+// Tile decode for a lower-triangular tile set of T x T tiles in bands: rows
+// 16S .. 16S+15 form band S (bands from the top, so the tiles with the longest
+// k >= row ranges still come first); inside a band the tiles go 16 columns at a
+// time (row-major within each 16-column block, the diagonal block last).  About
+// 256 consecutive tiles, the ones resident at a time, then share 16 row and 16
+// column operand panels instead of one row panel and 256 column panels.
+__device__ __forceinline__ void lower_tile_banded(int64_t t, int64_t T, int64_t& mt,
+                                                  int64_t& nt) {
+  constexpr int64_t H = 16;
+  auto cum = [](int64_t S) { return 128 * S * (S - 1) + 136 * S; };  // tiles of full bands < S
+  int64_t S = (int64_t)((-8.0 + sqrt(64.0 + 512.0 * (double)t)) / 256.0);
+  while (S > 0 && cum(S) > t) --S;
+  while (cum(S + 1) <= t && H * (S + 1) < T) ++S;
+  const int64_t R0 = H * S;
+  const int64_t h = (T - R0 < H) ? T - R0 : H;
+  const int64_t u = t - cum(S);
+  if (u < S * H * h) {
+    const int64_t cb = u / (H * h), v = u % (H * h);
+    mt = R0 + v / H;
+    nt = H * cb + v % H;
+  } else {
+    int64_t a, b;
+    lower_tile(u - S * H * h, a, b);
+    mt = R0 + a;
+    nt = R0 + b;
+  }
+}
+
+// The same for a full Mt x Nt tile grid: b (row-major launch order) -> bands of
+// 16 tile rows from the top, 16-column blocks within a band, row-major within
+// a block (ragged last band and block included).
+__device__ __forceinline__ void grid_tile_banded(int64_t b, int64_t Mt, int64_t Nt, int64_t& mt,
+                                                 int64_t& nt) {
+  constexpr int64_t H = 16;
+  const int64_t S = b / (H * Nt);
+  const int64_t u = b - S * H * Nt;
+  const int64_t h = (Mt - S * H < H) ? Mt - S * H : H;
+  const int64_t cb = u / (h * H);
+  const int64_t v = u - cb * h * H;
+  const int64_t w = (Nt - H * cb < H) ? Nt - H * cb : H;
+  mt = S * H + v / w;
+  nt = H * cb + v % w;
+}
+
 __device__ __forceinline__ void k_range(int krange, int64_t mt, int64_t nt, int64_t K,
                                         int64_t& kb, int64_t& ke) {
   kb = 0;
