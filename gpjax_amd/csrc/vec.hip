@@ -12,7 +12,9 @@ constexpr int TRMV_CW = 2048;  // columns per chunk, non-transposed
 constexpr int TRMV_RC = 1024;  // rows per chunk, transposed
 
 // y_i partial over columns [c*CW, (c+1)*CW), k <= i.  One wave per row, 16 rows
-// per workgroup.
+// per workgroup; each lane reads 16 bytes per load (four float32 or two
+// float64 entries, 1 KB per wave), the row's tail up to the diagonal entry by
+// entry.
 template <typename TS>
 __global__ __launch_bounds__(256) void trmv_n_kernel(const TS* __restrict__ X, int64_t ld,
                                                      int64_t np, const double* __restrict__ r,
@@ -28,12 +30,24 @@ __global__ __launch_bounds__(256) void trmv_n_kernel(const TS* __restrict__ X, i
   for (int e = threadIdx.x; e < TRMV_CW; e += 256) rs[e] = (k0 + e < np) ? r[k0 + e] : 0.0;
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int V = 16 / sizeof(TS);  // entries per 16-byte load
+  typedef typename std::conditional<sizeof(TS) == 4, float4, double2>::type Vec;
   for (int q = 0; q < 4; ++q) {
     const int64_t i = row0 + w * 4 + q;
     const int64_t kend = (i + 1 < k0 + TRMV_CW) ? i + 1 : k0 + TRMV_CW;
     double acc = 0.0;
     const TS* xr = X + i * ld;
-    for (int64_t k = k0 + lane; k < kend; k += 64) acc += (double)xr[k] * rs[k - k0];
+    for (int64_t kb = k0 + V * lane; kb < kend; kb += 64 * V) {
+      const double* rr = rs + (kb - k0);
+      if (kb + V <= kend) {
+        const Vec v = *reinterpret_cast<const Vec*>(xr + kb);
+        const TS* e = reinterpret_cast<const TS*>(&v);
+#pragma unroll
+        for (int u = 0; u < V; ++u) acc += (double)e[u] * rr[u];
+      } else {
+        for (int u = 0; kb + u < kend; ++u) acc += (double)xr[kb + u] * rr[u];
+      }
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
     if (lane == 0) partial[c * np + i] = acc;
