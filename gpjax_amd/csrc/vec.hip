@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void sigma_mv_kernel(int kind, const double* _
                                                        int nls, double jitter,
                                                        const double* __restrict__ a,
                                                        double* __restrict__ partial, Batch bt) {
-  __shared__ double xj_s[SMV_JT][DMAX + 1];
+  __shared__ __align__(16) double xj_s[SMV_JT][DMAX];  // unpadded: 16-byte broadcast reads
   __shared__ double aj_s[SMV_JT];
   bmember(bt, d, x, n, theta);
   a = bws(a, bt);
