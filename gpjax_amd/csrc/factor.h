@@ -59,6 +59,14 @@ struct Factor {
   // Solve Y L^T = B in place, B = L[row0 : row0+m, o : o+n], triangle L[o:o+n, o:o+n].
   int trsm(int64_t row0, int64_t m, int64_t o, int64_t n) {
     if (n == NB) {
+      if (ar_potrf != AR_F64) {
+        // float32 storage: the leaf also left L11^-1 in float32 in X's diagonal
+        // block, so the product takes the Cholesky updates' split arithmetic.
+        return launch_gemm<TS, TS, TS>(ar_potrf, false, false, L + row0 * ld + o, ld,
+                                       X + o * ld + o, ld, L + row0 * ld + o, ld, m, NB, NB, 1.0,
+                                       false, false, KR_FULL, s, nullptr, bt,
+                                       fscale ? fscale + 1 : nullptr);
+      }
       const double* inv = leafinv + (o / NB) * NB * NB;
       return launch_gemm<TS, double, TS>(AR_F64, false, false, L + row0 * ld + o, ld, inv, NB,
                                          L + row0 * ld + o, ld, m, NB, NB, 1.0, false, false,
