@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void elbo_wgrad_kernel(
   const int64_t mt = blockIdx.x, cb = blockIdx.y;
   const int64_t gi = mt * TILE + li;
   const bool rowok = gi < m;
-  const double var = theta[nls], sd = theta[nls + 1], s = sd * sd, cmean = theta[nls + 2];
+  const double sd = theta[nls + 1], s = sd * sd, cmean = theta[nls + 2];
   double zi[DMAX], s_ls[DMAX], gzr[DMAX];
 #pragma unroll
   for (int q = 0; q < DMAX; ++q) {
