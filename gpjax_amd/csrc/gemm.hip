@@ -40,9 +40,16 @@ __global__ __launch_bounds__(WaveLayout<AR>::NT) void gemm_kernel(GemmParams p) 
   if (p.lower_tiles) {
     lower_tile_banded(blockIdx.x, p.tiles_m, mt, nt);
   } else {
-    grid_tile_banded((int64_t)blockIdx.y * gridDim.x + blockIdx.x, gridDim.y, gridDim.x, mt, nt);
-    // k <= row tile: the last tile rows carry the most work, so dispatch them first.
-    if (p.krange == KR_LE_M) mt = gridDim.y - 1 - mt;
+    const int64_t b = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    if (p.krange == KR_GE_N) {
+      // k >= column tile: the first tile columns carry the most work, so band
+      // by columns and dispatch them first.
+      grid_tile_banded(b, gridDim.x, gridDim.y, nt, mt);
+    } else {
+      grid_tile_banded(b, gridDim.y, gridDim.x, mt, nt);
+      // k <= row tile: the last tile rows carry the most work, so dispatch them first.
+      if (p.krange == KR_LE_M) mt = gridDim.y - 1 - mt;
+    }
   }
   int64_t kb, ke;
   k_range(p.krange, mt, nt, p.K, kb, ke);
