@@ -262,7 +262,7 @@ def main() -> int:
                     help="exact_large (default) is the headline; the others print their own line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no "
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): gpjax_amd has no "
               "multi-GPU path", file=sys.stderr)
         return 2
     if args.workload != "exact_large":
