@@ -11,8 +11,8 @@ products on the matrix cores, float64 diagonal blocks / refinement /
 log-determinant).  The plain float64 GPU path is timed beside it; its MLL and
 gradient are the accuracy reference at this size (the float64 path itself is
 checked against the float64 CPU oracle at smaller N in tests/).  The CPU
-baseline runs the float64 oracle on the host cores on a bounded sample
-(same recipe, smaller N) and is extrapolated to N=65,536 by N^3.
+baseline runs the float64 oracle on the host cores on bounded samples (same
+recipe at N = 2,048, 4,096, 8,192), fits t = c N^3 and extrapolates to N=65,536.
 
 Prints one JSON line.
 """
@@ -90,6 +90,8 @@ def cpu_baseline(n: int, workload: str = "exact_large") -> dict:
                MKL_NUM_THREADS=str(threads))
     cmd = [sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"), "--n", str(n),
            "--threads", str(threads), "--workload", workload]
+    if workload == "exact_large":
+        cmd += ["--ns", "2048,4096,8192"]
     proc = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
     if proc.returncode != 0:
         return {"value": None, "error": proc.stderr[-500:]}
@@ -286,7 +288,8 @@ def main() -> int:
 
     dev = torch.device("cuda:0")
     arith = [int(v) for v in args.mixed_arith.split(",")]
-    _lib.call("gpk_set_mixed_arith", *arith)
+    mixed = "mixed:%d,%d,%d" % tuple(arith)
+    _lib.mixed_arith(*arith)  # validates the stages
     t0 = time.time()
     w = workloads.exact_large(n=args.n)
     print(f"# workload generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
@@ -300,13 +303,13 @@ def main() -> int:
     params = [kern.lengthscale, kern.variance, lik.obs_stddev]
 
     # ---- mixed precision: the headline ----
-    run_steps(post, D, "mixed", args.warmup, params, report=True)
+    run_steps(post, D, mixed, args.warmup, params, report=True)
     torch.cuda.synchronize()
     _lib.profile_collect()
     _lib.profile_enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    mll_m, res_m = run_steps(post, D, "mixed", args.steps, params)
+    mll_m, res_m = run_steps(post, D, mixed, args.steps, params)
     torch.cuda.synchronize()
     t_mixed = time.perf_counter() - t0
     _lib.profile_enable(False)
@@ -338,19 +341,35 @@ def main() -> int:
               file=sys.stderr, flush=True)
 
     # ---- roofline of the dominant kernel (fused Sigma^-1 + adjoint, one launch per eval) ----
+    # Algorithmic work: the LAUUM X^T X over the lower tiles (n^3/3).  The bf16
+    # split issues `arith[2]` products per algorithmic product; that issue rate is
+    # reported beside the roofline fraction, not as it.
     adj_ms, adj_cnt = phases_m["adjoint"]
     adj_s = adj_ms / max(adj_cnt, 1) / 1e3
-    flops = 6.0 * lauum_flops(args.n)  # six bf16 products per fp32-class product
+    flops = lauum_flops(args.n)
     achieved = flops / adj_s / 1e12
-    roofline = {"kernel": "adjoint (fused L^-T L^-1 tile GEMM + gradient epilogue, bf16x6)",
+    issued = max(arith[2], 1) * flops
+    traffic = profiled_traffic(args.n) if tuple(arith) == (6, 3, 6) else None
+    roofline = {"kernel": "adjoint (fused L^-T L^-1 tile GEMM + gradient epilogue, "
+                          f"bf16x{arith[2]})",
                 "bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 4),
-                "traffic": profiled_traffic(args.n), "traffic_unit": "bytes per launch (HBM)",
-                "algorithmic_flops_per_launch": flops, "avg_launch_ms": round(adj_s * 1e3, 3)}
+                "traffic": traffic, "traffic_unit": "bytes per launch (HBM)",
+                "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel, scaled by "
+                                   "a 1 GiB copy's counters; read from the committed "
+                                   "profiles/adjoint_bf16x6_n65536_traffic.md, not measured "
+                                   "in this run") if traffic else None,
+                "algorithmic_flops_per_launch": flops, "avg_launch_ms": round(adj_s * 1e3, 3),
+                "issued_bf16_flops_per_launch": issued,
+                "mfma_issue_frac": round(issued / adj_s / 1e12 / BF16_PEAK_TFLOPS, 4),
+                "eval_frac_of_bf16_peak": None}
 
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
 
     value = args.steps / t_mixed
+    # whole evaluation: n^3 algorithmic flops (POTRF + TRTRI + LAUUM, n^3/3 each)
+    roofline["eval_frac_of_bf16_peak"] = round(float(args.n) ** 3 * value / 1e12
+                                               / BF16_PEAK_TFLOPS, 4)
     line = {
         "metric": METRIC, "value": round(value, 4), "unit": "evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,

@@ -13,9 +13,20 @@ from pathlib import Path
 LIB_PATH = Path(__file__).with_name("libgpk.so")
 
 GPK_OK = 0
-STATUS = {0: "OK", 1: "BAD_ARG", 2: "NOT_PD", 3: "NO_DEVICE", 4: "HIP", 5: "WORKSPACE"}
+STATUS = {0: "OK", 1: "BAD_ARG", 3: "NO_DEVICE", 4: "HIP", 5: "WORKSPACE"}
 KERNEL_IDS = {"rbf": 0, "matern12": 1, "matern32": 2, "matern52": 3}
 PRECISIONS = {"fp64": 0, "mixed": 1}
+ARITH_FLAG = 0x100
+
+
+def mixed_arith(potrf: int, trtri: int, lauum: int) -> int:
+    """The ``prec`` code of the mixed path with the products of each stage named
+    (GPK_PREC_MIXED_ARITH in include/gpk.h): 6, 3, 4 (fp16 split) or 0 (float64
+    products of the float32 operands)."""
+    for v in (potrf, trtri, lauum):
+        if v not in (0, 3, 4, 6):
+            raise ValueError(f"stage products must be 0, 3, 4 or 6, got {v}")
+    return PRECISIONS["mixed"] | ARITH_FLAG | (potrf << 12) | (trtri << 16) | (lauum << 20)
 RESULT_LEN = 8
 RES_MLL, RES_LOGDET, RES_QUAD, RES_RESID, RES_INFO = 0, 1, 2, 3, 4
 
@@ -35,7 +46,6 @@ SIGNATURES = {
     "gpk_mll_workspace_bytes": (_S, [_L, _L, _I]),
     "gpk_mll_forward": (_I, [_I, _I, _P, _P, _L, _L, _P, _I, _D, _I, _P, _S, _P, _P]),
     "gpk_mll_backward": (_I, [_I, _I, _P, _L, _L, _P, _I, _D, _P, _S, _P, _P]),
-    "gpk_set_mixed_arith": (_I, [_I, _I, _I]),
     "gpk_mfma_selftest": (_I, [_P, _P, _P]),
     "gpk_elbo_workspace_bytes": (_S, [_L, _L, _L, _I]),
     "gpk_elbo_forward": (_I, [_I, _I, _P, _P, _L, _P, _L, _L, _P, _I, _D, _P, _S, _P, _P]),
@@ -52,6 +62,7 @@ SIGNATURES = {
     "gpk_batched_workspace_bytes": (_S, [_P, _L, _L]),
     "gpk_batched_plan": (_I, [_P, _L, _L, _P, _S]),
     "gpk_batched_mll_grad": (_I, [_I, _P, _P, _P, _L, _L, _P, _I, _D, _I, _P, _S, _P, _P, _P]),
+    "gpk_adam_step": (_I, [_L, _P, _P, _P, _P, _P, _P, _D, _D, _D, _D, _P, _P, _P, _L, _P]),
     "gpk_profile_enable": (_I, [_I]),
     "gpk_profile_collect": (_I, [_P, _P]),
 }

@@ -16,7 +16,8 @@ refine_iters = None
 
 
 def set_precision(p: str) -> None:
+    """'fp64', 'mixed', or 'mixed:P,T,L' (products per stage, see objectives)."""
     global precision
-    if p not in ("fp64", "mixed"):
-        raise ValueError(f"precision must be 'fp64' or 'mixed', got {p!r}")
+    if p not in ("fp64", "mixed") and not (isinstance(p, str) and p.startswith("mixed:")):
+        raise ValueError(f"precision must be 'fp64', 'mixed' or 'mixed:P,T,L', got {p!r}")
     precision = p

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <mutex>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "factor.h"
@@ -36,6 +37,8 @@ void set_error(const char* fmt, ...) {
 namespace {
 
 constexpr size_t ALIGN = 256;
+// Rademacher probes of the mixed path's a-posteriori inverse check.
+constexpr int NPROBE = 2;
 inline size_t al(size_t b) { return (b + ALIGN - 1) / ALIGN * ALIGN; }
 
 // ---- phase profiling: HIP events recorded on the launch stream ----
@@ -69,10 +72,44 @@ struct PhaseScope {
   }
 };
 
-// Arithmetic of the three product stages in the mixed path.
-int g_ar_potrf = AR_BF16X6;
-int g_ar_trtri = AR_BF16X3;
-int g_ar_lauum = AR_BF16X6;
+// Arithmetic of the three product stages of one call: float64 products for
+// the FP64 path; for the mixed path the defaults (6, 3, 6) or the stages that
+// GPK_PREC_MIXED_ARITH(p, t, l) names.  Per call, so no process-wide state.
+struct Arith3 {
+  int potrf = AR_F64, trtri = AR_F64, lauum = AR_F64;
+};
+
+inline bool arith_ok(int v) {
+  return v == AR_F64 || v == AR_BF16X3 || v == AR_FP16X3 || v == AR_BF16X6;
+}
+
+// Splits a `prec` argument into the storage precision (GPK_PREC_FP64 or
+// GPK_PREC_MIXED) and the stage arithmetic; -1 if it is not a valid code.
+int decode_prec(int prec, Arith3* ar) {
+  const int base = prec & 0xFF;
+  Arith3 a;
+  if (base == GPK_PREC_FP64) {
+    if (prec != GPK_PREC_FP64) return -1;
+  } else if (base == GPK_PREC_MIXED) {
+    a.potrf = AR_BF16X6;
+    a.trtri = AR_BF16X3;
+    a.lauum = AR_BF16X6;
+    if (prec & GPK_PREC_ARITH_FLAG) {
+      a.potrf = (prec >> 12) & 0xF;
+      a.trtri = (prec >> 16) & 0xF;
+      a.lauum = (prec >> 20) & 0xF;
+      if ((prec & ~0xFFF1FF) != 0 || !arith_ok(a.potrf) || !arith_ok(a.trtri) ||
+          !arith_ok(a.lauum))
+        return -1;
+    } else if (prec != GPK_PREC_MIXED) {
+      return -1;
+    }
+  } else {
+    return -1;
+  }
+  if (ar) *ar = a;
+  return base;
+}
 
 struct Layout {
   int64_t np = 0;
@@ -81,7 +118,7 @@ struct Layout {
   size_t off_L = 0, off_X = 0, off_T = 0, off_leafinv = 0, off_leafld = 0, off_leafinfo = 0;
   size_t off_r = 0, off_alpha = 0, off_u = 0, off_v = 0, off_res = 0, off_delta = 0;
   size_t off_dblk = 0;
-  size_t off_part = 0, off_gpart = 0, off_scal = 0, total = 0;
+  size_t off_part = 0, off_gpart = 0, off_scal = 0, off_probe = 0, total = 0;
 };
 
 Layout make_layout(int64_t n, int64_t d, int prec) {
@@ -110,6 +147,7 @@ Layout make_layout(int64_t n, int64_t d, int prec) {
   L.off_part = o; o += al((size_t)nchunk * np * sizeof(double));
   L.off_gpart = o; o += al((size_t)ntile * (d + 2) * sizeof(double));
   L.off_scal = o; o += al(16 * sizeof(double));
+  L.off_probe = o; o += al(2 * NPROBE * sizeof(double));
   L.total = o;
   return L;
 }
@@ -202,6 +240,59 @@ __global__ __launch_bounds__(256) void finalize_fwd_kernel(
   }
 }
 
+// ---- a-posteriori check of the mixed inverse ----
+// With X ~ L^-1 from the mixed factorisation, E = I - Sigma X^T X measures how
+// far X^T X is from Sigma^-1 (Sigma recomputed in float64 from the inputs).
+// Rademacher probes z give ||E z|| (E||E z||^2 = ||E||_F^2) and z^T E z
+// (E[z^T E z] = tr E, which to first order is the error of log|Sigma| when X is
+// the inverse of the computed factor: E ~ dSigma Sigma^-1).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t v) {
+  v += 0x9E3779B97F4A7C15ull;
+  v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+  v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+  return v ^ (v >> 31);
+}
+
+__global__ void probe_fill_kernel(double* __restrict__ z, int64_t n, int64_t np, int probe) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const uint64_t h = splitmix64(((uint64_t)probe << 40) ^ (uint64_t)i);
+  z[i] = i < n ? ((h >> 63) ? -1.0 : 1.0) : 0.0;
+}
+
+// stats[2 probe] = ||z - v||^2, stats[2 probe + 1] = z . (z - v); one workgroup.
+__global__ __launch_bounds__(256) void probe_stats_kernel(const double* __restrict__ z,
+                                                          const double* __restrict__ v,
+                                                          int64_t np, int probe,
+                                                          double* __restrict__ stats) {
+  __shared__ double red[4];
+  double ee = 0.0, ze = 0.0;
+  for (int64_t i = threadIdx.x; i < np; i += 256) {
+    const double e = z[i] - v[i];
+    ee += e * e;
+    ze += z[i] * e;
+  }
+  ee = block_sum(ee, red);
+  ze = block_sum(ze, red);
+  if (threadIdx.x == 0) {
+    stats[2 * probe] = ee;
+    stats[2 * probe + 1] = ze;
+  }
+}
+
+// result[GPK_RES_PROBE] = sqrt(mean ||E z||^2 / n), result[GPK_RES_TRACE] = mean z^T E z.
+__global__ void probe_finish_kernel(const double* __restrict__ stats, int64_t n,
+                                    double* __restrict__ result) {
+  if (threadIdx.x != 0) return;
+  double ee = 0.0, ze = 0.0;
+  for (int p = 0; p < NPROBE; ++p) {
+    ee += stats[2 * p];
+    ze += stats[2 * p + 1];
+  }
+  result[GPK_RES_PROBE] = sqrt(ee / NPROBE / (double)n);
+  result[GPK_RES_TRACE] = ze / NPROBE;
+}
+
 // One workgroup per gradient slot: sum the tile partials in a fixed order.
 __global__ __launch_bounds__(256) void finalize_bwd_kernel(
     const double* __restrict__ gpart, int64_t ntile, int nls, const double* __restrict__ theta,
@@ -264,8 +355,8 @@ int check_common(int kind, int prec, const void* x, int64_t n, int64_t d, const 
     set_error("unknown kernel kind %d", kind);
     return GPK_ERR_BAD_ARG;
   }
-  if (prec != GPK_PREC_FP64 && prec != GPK_PREC_MIXED) {
-    set_error("unknown precision %d", prec);
+  if (decode_prec(prec, nullptr) < 0) {
+    set_error("unknown precision code %d", prec);
     return GPK_ERR_BAD_ARG;
   }
   if (!x || !theta) {
@@ -284,8 +375,21 @@ int check_common(int kind, int prec, const void* x, int64_t n, int64_t d, const 
   return GPK_OK;
 }
 
+// Workspace size and alignment, as every entry point that takes one checks them.
+int check_ws(const char* fn, const void* ws, size_t have, size_t need) {
+  if (have < need) {
+    set_error("%s: workspace %zu bytes < required %zu", fn, have, need);
+    return GPK_ERR_WORKSPACE;
+  }
+  if (reinterpret_cast<uintptr_t>(ws) % ALIGN) {
+    set_error("%s: workspace must be %zu-byte aligned", fn, ALIGN);
+    return GPK_ERR_BAD_ARG;
+  }
+  return GPK_OK;
+}
+
 template <typename TS>
-int mll_forward_t(int kind, int prec, const double* x, const double* y, int64_t n, int d,
+int mll_forward_t(int kind, const Arith3& ar, const double* x, const double* y, int64_t n, int d,
                   const double* theta, int nls, double jitter, int iters, void* ws,
                   const Layout& Lo, double* result, hipStream_t s,
                   const Batch& bt = Batch()) {
@@ -308,8 +412,8 @@ int mll_forward_t(int kind, int prec, const double* x, const double* y, int64_t 
                at<double>(ws, Lo.off_leafinv),
                at<double>(ws, Lo.off_leafld),
                at<int>(ws, Lo.off_leafinfo),
-               f64 ? (int)AR_F64 : g_ar_potrf,
-               f64 ? (int)AR_F64 : g_ar_trtri,
+               f64 ? (int)AR_F64 : ar.potrf,
+               f64 ? (int)AR_F64 : ar.trtri,
                s};
   // float32 storage: the diagonal NB-blocks live in float64 on the side
   double* dblk = f64 ? nullptr : at<double>(ws, Lo.off_dblk);
@@ -357,13 +461,29 @@ int mll_forward_t(int kind, int prec, const double* x, const double* y, int64_t 
                      at<double>(ws, Lo.off_leafld), at<int>(ws, Lo.off_leafinfo), np / NB, r,
                      alpha, res, n, np, result, bt);
   GPK_LAUNCH_CHECK();
+  if (!f64 && !bt.gp) {
+    // a-posteriori check of X = L^-1: probes through X, X^T and Sigma in float64
+    // (u, v, res and delta are free once the objective is final)
+    double* stats = at<double>(ws, Lo.off_probe);
+    for (int p = 0; p < NPROBE; ++p) {
+      hipLaunchKernelGGL(probe_fill_kernel, gv, dim3(256), 0, s, delta, n, np, p);
+      GPK_LAUNCH_CHECK();
+      GPK_TRY(launch_trmv<TS>(X, np, np, delta, u, false, part, s));
+      GPK_TRY(launch_trmv<TS>(X, np, np, u, v, true, part, s));
+      GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, v, res, part, s));
+      hipLaunchKernelGGL(probe_stats_kernel, dim3(1), dim3(256), 0, s, delta, res, np, p, stats);
+      GPK_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(probe_finish_kernel, dim3(1), dim3(64), 0, s, stats, n, result);
+    GPK_LAUNCH_CHECK();
+  }
   return GPK_OK;
 }
 
 template <typename TS>
-int mll_backward_t(int kind, const double* x, int64_t n, int d, const double* theta, int nls,
-                   double gbar, void* ws, const Layout& Lo, double* grad, hipStream_t s,
-                   const Batch& bt = Batch()) {
+int mll_backward_t(int kind, const Arith3& ar, const double* x, int64_t n, int d,
+                   const double* theta, int nls, double gbar, void* ws, const Layout& Lo,
+                   double* grad, hipStream_t s, const Batch& bt = Batch()) {
   const int64_t np = Lo.np;
   const bool f64 = std::is_same<TS, double>::value;
   double* gpart = at<double>(ws, Lo.off_gpart);
@@ -375,12 +495,12 @@ int mll_backward_t(int kind, const double* x, int64_t n, int d, const double* th
     TS* U = at<TS>(ws, Lo.off_L);
     GPK_TRY(launch_lower_to_upper<TS>(at<TS>(ws, Lo.off_X), U, np, np, s));
     PhaseScope ps(GPK_PHASE_ADJOINT, s);
-    GPK_TRY(launch_adjoint<TS>(kind, f64 ? (int)AR_F64 : g_ar_lauum, U, np, np, x, n, d, theta,
+    GPK_TRY(launch_adjoint<TS>(kind, f64 ? (int)AR_F64 : ar.lauum, U, np, np, x, n, d, theta,
                                nls, at<double>(ws, Lo.off_alpha), gpart, s, bt, nullptr, false,
                                fs, true));
   } else {
     PhaseScope ps(GPK_PHASE_ADJOINT, s);
-    GPK_TRY(launch_adjoint<TS>(kind, f64 ? (int)AR_F64 : g_ar_lauum, at<TS>(ws, Lo.off_X), np,
+    GPK_TRY(launch_adjoint<TS>(kind, f64 ? (int)AR_F64 : ar.lauum, at<TS>(ws, Lo.off_X), np,
                                np, x, n, d, theta, nls, at<double>(ws, Lo.off_alpha), gpart, s,
                                bt, nullptr, false, fs));
   }
@@ -440,14 +560,14 @@ __global__ void pred_var_kernel(const double* __restrict__ colsq,
 }
 
 template <typename TS>
-int predict_t(int kind, int prec, const double* x, const double* y, int64_t n, int d,
+int predict_t(int kind, const Arith3& ar, const double* x, const double* y, int64_t n, int d,
               const double* xt, int64_t t, const double* theta, int nls, double jitter_post,
               double jitter_prior, int full_cov, int iters, void* ws, const Layout& Lo,
               const PredLayout& P, double* mean, double* cov, hipStream_t s) {
   const int64_t np = Lo.np, tp = P.tp;
   double* result = at<double>(ws, P.off_res);
   // L, X = L^-1 and alpha = Sigma^-1 (y - m), Sigma with the posterior's jitter (gps.py:506-511)
-  GPK_TRY(mll_forward_t<TS>(kind, prec, x, y, n, d, theta, nls, jitter_post, iters, ws, Lo,
+  GPK_TRY(mll_forward_t<TS>(kind, ar, x, y, n, d, theta, nls, jitter_post, iters, ws, Lo,
                             result, s));
   double* kxt = at<double>(ws, P.off_kxt);
   double* A = at<double>(ws, P.off_a);
@@ -574,7 +694,7 @@ __global__ __launch_bounds__(256) void loo_fwd_kernel(const double* __restrict__
     result[GPK_RES_QUAD] = sq;
     result[GPK_RES_RESID] = mres[GPK_RES_RESID];
     result[GPK_RES_INFO] = info;
-    for (int k = 5; k < GPK_RESULT_LEN; ++k) result[k] = 0.0;
+    for (int k = 5; k < GPK_RESULT_LEN; ++k) result[k] = mres[k];
   }
 }
 
@@ -616,11 +736,11 @@ __global__ __launch_bounds__(256) void loo_scale_mirror_kernel(TS* __restrict__ 
 }
 
 template <typename TS>
-int loocv_forward_t(int kind, int prec, const double* x, const double* y, int64_t n, int d,
+int loocv_forward_t(int kind, const Arith3& ar, const double* x, const double* y, int64_t n, int d,
                     const double* theta, int nls, double jitter, int iters, void* ws,
                     const Layout& Lo, const LooLayout& P, double* result, hipStream_t s) {
   const int64_t np = Lo.np;
-  GPK_TRY(mll_forward_t<TS>(kind, prec, x, y, n, d, theta, nls, jitter, iters, ws, Lo,
+  GPK_TRY(mll_forward_t<TS>(kind, ar, x, y, n, d, theta, nls, jitter, iters, ws, Lo,
                             at<double>(ws, P.off_mres), s));
   double* dg = at<double>(ws, P.off_d);
   GPK_TRY(launch_colreduce<TS>(at<TS>(ws, Lo.off_X), np, np, np, nullptr, dg,
@@ -633,13 +753,13 @@ int loocv_forward_t(int kind, int prec, const double* x, const double* y, int64_
 }
 
 template <typename TS>
-int loocv_backward_t(int kind, const double* x, int64_t n, int d, const double* theta, int nls,
-                     double gbar, void* ws, const Layout& Lo, const LooLayout& P, double* grad,
-                     hipStream_t s) {
+int loocv_backward_t(int kind, const Arith3& ar3, const double* x, int64_t n, int d,
+                     const double* theta, int nls, double gbar, void* ws, const Layout& Lo,
+                     const LooLayout& P, double* grad, hipStream_t s) {
   const int64_t np = Lo.np;
   const bool f64 = std::is_same<TS, double>::value;
   // Y = diag(c) Sigma^-1 has no a-priori bound for an fp16 scale: BF16x6 there.
-  const int ar = f64 ? (int)AR_F64 : (g_ar_lauum == AR_FP16X3 ? (int)AR_BF16X6 : g_ar_lauum);
+  const int ar = f64 ? (int)AR_F64 : (ar3.lauum == AR_FP16X3 ? (int)AR_BF16X6 : ar3.lauum);
   const TS* X = at<TS>(ws, Lo.off_X);
   TS* Y = at<TS>(ws, Lo.off_L);
   double* part = at<double>(ws, Lo.off_part);
@@ -730,6 +850,27 @@ BatchPlan make_batch_plan(const int64_t* off, int64_t count, int64_t d) {
   return P;
 }
 
+// What gpk_batched_plan wrote into a workspace, keyed by its address, so that
+// gpk_batched_mll_grad can refuse (on the host, without a sync) a workspace
+// whose device-side problem map belongs to other offsets: the kernels index x
+// and y through that map.
+struct PlanRecord {
+  int64_t count = 0, d = 0;
+  uint64_t hash = 0;
+};
+std::mutex g_plan_mu;
+std::unordered_map<const void*, PlanRecord> g_plans;
+
+uint64_t hash_offsets(const int64_t* off, int64_t count) {
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over the offsets' bytes
+  const unsigned char* b = reinterpret_cast<const unsigned char*>(off);
+  for (size_t i = 0; i < (size_t)(count + 1) * sizeof(int64_t); ++i) {
+    h ^= b[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
 int batched_mll_grad(int kind, const double* x, const double* y, const int64_t* off,
                      int64_t count, int d, const double* theta, int nls, double jitter, int iters,
                      void* ws, double* result, double* grad, hipStream_t s) {
@@ -750,9 +891,10 @@ int batched_mll_grad(int kind, const double* x, const double* y, const int64_t* 
     bt.count = (unsigned)g.count;
     void* wsg = static_cast<char*>(ws) + g.off;
     // n is taken per member inside the kernels; np fixes the shapes.
-    GPK_TRY(mll_forward_t<double>(kind, GPK_PREC_FP64, x, y, g.np, d, theta, nls, jitter, iters,
+    GPK_TRY(mll_forward_t<double>(kind, Arith3(), x, y, g.np, d, theta, nls, jitter, iters,
                                   wsg, g.lo, result, s, bt));
-    GPK_TRY(mll_backward_t<double>(kind, x, g.np, d, theta, nls, 1.0, wsg, g.lo, grad, s, bt));
+    GPK_TRY(mll_backward_t<double>(kind, Arith3(), x, g.np, d, theta, nls, 1.0, wsg, g.lo, grad,
+                                   s, bt));
   }
   return GPK_OK;
 }
@@ -795,22 +937,9 @@ int gpk_gram(int kind, const double* x, int64_t n, int64_t d, const double* thet
 }
 
 size_t gpk_mll_workspace_bytes(int64_t n, int64_t d, int prec) {
-  if (n < 1 || d < 1) return 0;
-  return make_layout(n, d, prec).total;
-}
-
-int gpk_set_mixed_arith(int potrf_products, int trtri_products, int lauum_products) {
-  auto ok = [](int v) {
-    return v == AR_F64 || v == AR_BF16X3 || v == AR_FP16X3 || v == AR_BF16X6;
-  };
-  if (!ok(potrf_products) || !ok(trtri_products) || !ok(lauum_products)) {
-    set_error("mixed arithmetic: products must be 0 (float64), 3, 4 (fp16) or 6");
-    return GPK_ERR_BAD_ARG;
-  }
-  g_ar_potrf = potrf_products;
-  g_ar_trtri = trtri_products;
-  g_ar_lauum = lauum_products;
-  return GPK_OK;
+  const int base = decode_prec(prec, nullptr);
+  if (n < 1 || d < 1 || base < 0) return 0;
+  return make_layout(n, d, base).total;
 }
 
 int gpk_mll_forward(int kind, int prec, const double* x, const double* y, int64_t n, int64_t d,
@@ -822,21 +951,16 @@ int gpk_mll_forward(int kind, int prec, const double* x, const double* y, int64_
     set_error("mll_forward: null y/ws/result");
     return GPK_ERR_BAD_ARG;
   }
-  const Layout Lo = make_layout(n, d, prec);
-  if (ws_bytes < Lo.total) {
-    set_error("mll_forward: workspace %zu bytes < required %zu", ws_bytes, Lo.total);
-    return GPK_ERR_WORKSPACE;
-  }
-  if (reinterpret_cast<uintptr_t>(ws) % ALIGN) {
-    set_error("mll_forward: workspace must be %zu-byte aligned", ALIGN);
-    return GPK_ERR_BAD_ARG;
-  }
-  const int iters = refine_iters >= 0 ? refine_iters : (prec == GPK_PREC_FP64 ? 1 : 4);
+  Arith3 ar;
+  const int base = decode_prec(prec, &ar);
+  const Layout Lo = make_layout(n, d, base);
+  if ((rc = check_ws("mll_forward", ws, ws_bytes, Lo.total))) return rc;
+  const int iters = refine_iters >= 0 ? refine_iters : (base == GPK_PREC_FP64 ? 1 : 4);
   hipStream_t s = (hipStream_t)stream;
-  if (prec == GPK_PREC_FP64)
-    return mll_forward_t<double>(kind, prec, x, y, n, (int)d, theta, nls, jitter, iters, ws, Lo,
+  if (base == GPK_PREC_FP64)
+    return mll_forward_t<double>(kind, ar, x, y, n, (int)d, theta, nls, jitter, iters, ws, Lo,
                                  result, s);
-  return mll_forward_t<float>(kind, prec, x, y, n, (int)d, theta, nls, jitter, iters, ws, Lo,
+  return mll_forward_t<float>(kind, ar, x, y, n, (int)d, theta, nls, jitter, iters, ws, Lo,
                               result, s);
 }
 
@@ -849,20 +973,21 @@ int gpk_mll_backward(int kind, int prec, const double* x, int64_t n, int64_t d,
     set_error("mll_backward: null ws/grad");
     return GPK_ERR_BAD_ARG;
   }
-  const Layout Lo = make_layout(n, d, prec);
-  if (ws_bytes < Lo.total) {
-    set_error("mll_backward: workspace %zu bytes < required %zu", ws_bytes, Lo.total);
-    return GPK_ERR_WORKSPACE;
-  }
+  Arith3 ar;
+  const int base = decode_prec(prec, &ar);
+  const Layout Lo = make_layout(n, d, base);
+  if ((rc = check_ws("mll_backward", ws, ws_bytes, Lo.total))) return rc;
   hipStream_t s = (hipStream_t)stream;
-  if (prec == GPK_PREC_FP64)
-    return mll_backward_t<double>(kind, x, n, (int)d, theta, nls, gbar_host, ws, Lo, grad, s);
-  return mll_backward_t<float>(kind, x, n, (int)d, theta, nls, gbar_host, ws, Lo, grad, s);
+  if (base == GPK_PREC_FP64)
+    return mll_backward_t<double>(kind, ar, x, n, (int)d, theta, nls, gbar_host, ws, Lo, grad,
+                                  s);
+  return mll_backward_t<float>(kind, ar, x, n, (int)d, theta, nls, gbar_host, ws, Lo, grad, s);
 }
 
 size_t gpk_predict_workspace_bytes(int64_t n, int64_t d, int64_t t, int prec, int full_cov) {
-  if (n < 1 || d < 1 || t < 1) return 0;
-  return make_pred_layout(make_layout(n, d, prec), t, full_cov).total;
+  const int base = decode_prec(prec, nullptr);
+  if (n < 1 || d < 1 || t < 1 || base < 0) return 0;
+  return make_pred_layout(make_layout(n, d, base), t, full_cov).total;
 }
 
 int gpk_predict(int kind, int prec, const double* x, const double* y, int64_t n, int64_t d,
@@ -875,28 +1000,24 @@ int gpk_predict(int kind, int prec, const double* x, const double* y, int64_t n,
     set_error("predict: null y/xt/ws/mean/cov or t < 1");
     return GPK_ERR_BAD_ARG;
   }
-  const Layout Lo = make_layout(n, d, prec);
+  Arith3 ar;
+  const int base = decode_prec(prec, &ar);
+  const Layout Lo = make_layout(n, d, base);
   const PredLayout P = make_pred_layout(Lo, t, full_cov);
-  if (ws_bytes < P.total) {
-    set_error("predict: workspace %zu bytes < required %zu", ws_bytes, P.total);
-    return GPK_ERR_WORKSPACE;
-  }
-  if (reinterpret_cast<uintptr_t>(ws) % ALIGN) {
-    set_error("predict: workspace must be %zu-byte aligned", ALIGN);
-    return GPK_ERR_BAD_ARG;
-  }
-  const int iters = refine_iters >= 0 ? refine_iters : (prec == GPK_PREC_FP64 ? 1 : 4);
+  if ((rc = check_ws("predict", ws, ws_bytes, P.total))) return rc;
+  const int iters = refine_iters >= 0 ? refine_iters : (base == GPK_PREC_FP64 ? 1 : 4);
   hipStream_t s = (hipStream_t)stream;
-  if (prec == GPK_PREC_FP64)
-    return predict_t<double>(kind, prec, x, y, n, (int)d, xt, t, theta, nls, jitter_post,
+  if (base == GPK_PREC_FP64)
+    return predict_t<double>(kind, ar, x, y, n, (int)d, xt, t, theta, nls, jitter_post,
                              jitter_prior, full_cov, iters, ws, Lo, P, mean, cov, s);
-  return predict_t<float>(kind, prec, x, y, n, (int)d, xt, t, theta, nls, jitter_post,
+  return predict_t<float>(kind, ar, x, y, n, (int)d, xt, t, theta, nls, jitter_post,
                           jitter_prior, full_cov, iters, ws, Lo, P, mean, cov, s);
 }
 
 size_t gpk_loocv_workspace_bytes(int64_t n, int64_t d, int prec) {
-  if (n < 1 || d < 1) return 0;
-  return make_loo_layout(make_layout(n, d, prec)).total;
+  const int base = decode_prec(prec, nullptr);
+  if (n < 1 || d < 1 || base < 0) return 0;
+  return make_loo_layout(make_layout(n, d, base)).total;
 }
 
 int gpk_loocv_forward(int kind, int prec, const double* x, const double* y, int64_t n, int64_t d,
@@ -908,23 +1029,18 @@ int gpk_loocv_forward(int kind, int prec, const double* x, const double* y, int6
     set_error("loocv_forward: null y/ws/result");
     return GPK_ERR_BAD_ARG;
   }
-  const Layout Lo = make_layout(n, d, prec);
+  Arith3 ar;
+  const int base = decode_prec(prec, &ar);
+  const Layout Lo = make_layout(n, d, base);
   const LooLayout P = make_loo_layout(Lo);
-  if (ws_bytes < P.total) {
-    set_error("loocv_forward: workspace %zu bytes < required %zu", ws_bytes, P.total);
-    return GPK_ERR_WORKSPACE;
-  }
-  if (reinterpret_cast<uintptr_t>(ws) % ALIGN) {
-    set_error("loocv_forward: workspace must be %zu-byte aligned", ALIGN);
-    return GPK_ERR_BAD_ARG;
-  }
-  const int iters = refine_iters >= 0 ? refine_iters : (prec == GPK_PREC_FP64 ? 1 : 4);
+  if ((rc = check_ws("loocv_forward", ws, ws_bytes, P.total))) return rc;
+  const int iters = refine_iters >= 0 ? refine_iters : (base == GPK_PREC_FP64 ? 1 : 4);
   hipStream_t s = (hipStream_t)stream;
-  if (prec == GPK_PREC_FP64)
-    return loocv_forward_t<double>(kind, prec, x, y, n, (int)d, theta, nls, jitter, iters, ws,
-                                   Lo, P, result, s);
-  return loocv_forward_t<float>(kind, prec, x, y, n, (int)d, theta, nls, jitter, iters, ws, Lo,
-                                P, result, s);
+  if (base == GPK_PREC_FP64)
+    return loocv_forward_t<double>(kind, ar, x, y, n, (int)d, theta, nls, jitter, iters, ws, Lo,
+                                   P, result, s);
+  return loocv_forward_t<float>(kind, ar, x, y, n, (int)d, theta, nls, jitter, iters, ws, Lo, P,
+                                result, s);
 }
 
 int gpk_loocv_backward(int kind, int prec, const double* x, int64_t n, int64_t d,
@@ -936,17 +1052,17 @@ int gpk_loocv_backward(int kind, int prec, const double* x, int64_t n, int64_t d
     set_error("loocv_backward: null ws/grad");
     return GPK_ERR_BAD_ARG;
   }
-  const Layout Lo = make_layout(n, d, prec);
+  Arith3 ar;
+  const int base = decode_prec(prec, &ar);
+  const Layout Lo = make_layout(n, d, base);
   const LooLayout P = make_loo_layout(Lo);
-  if (ws_bytes < P.total) {
-    set_error("loocv_backward: workspace %zu bytes < required %zu", ws_bytes, P.total);
-    return GPK_ERR_WORKSPACE;
-  }
+  if ((rc = check_ws("loocv_backward", ws, ws_bytes, P.total))) return rc;
   hipStream_t s = (hipStream_t)stream;
-  if (prec == GPK_PREC_FP64)
-    return loocv_backward_t<double>(kind, x, n, (int)d, theta, nls, gbar_host, ws, Lo, P, grad,
-                                    s);
-  return loocv_backward_t<float>(kind, x, n, (int)d, theta, nls, gbar_host, ws, Lo, P, grad, s);
+  if (base == GPK_PREC_FP64)
+    return loocv_backward_t<double>(kind, ar, x, n, (int)d, theta, nls, gbar_host, ws, Lo, P,
+                                    grad, s);
+  return loocv_backward_t<float>(kind, ar, x, n, (int)d, theta, nls, gbar_host, ws, Lo, P, grad,
+                                 s);
 }
 
 size_t gpk_batched_workspace_bytes(const int64_t* offsets_host, int64_t count, int64_t d) {
@@ -971,6 +1087,8 @@ int gpk_batched_plan(const int64_t* offsets_host, int64_t count, int64_t d, void
                         hipMemcpyHostToDevice));
   GPK_HIP_TRY(hipMemcpy(at<int64_t>(ws, P.off_xoff), offsets_host,
                         (size_t)(count + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+  std::lock_guard<std::mutex> g(g_plan_mu);
+  g_plans[ws] = PlanRecord{count, d, hash_offsets(offsets_host, count)};
   return GPK_OK;
 }
 
@@ -994,6 +1112,19 @@ int gpk_batched_mll_grad(int kind, const double* x, const double* y,
   if (reinterpret_cast<uintptr_t>(ws) % ALIGN) {
     set_error("batched_mll_grad: workspace must be %zu-byte aligned", ALIGN);
     return GPK_ERR_BAD_ARG;
+  }
+  {
+    std::lock_guard<std::mutex> g(g_plan_mu);
+    auto it = g_plans.find(ws);
+    if (it == g_plans.end()) {
+      set_error("batched_mll_grad: no gpk_batched_plan has run on this workspace");
+      return GPK_ERR_BAD_ARG;
+    }
+    const PlanRecord& r = it->second;
+    if (r.count != count || r.d != d || r.hash != hash_offsets(offsets_host, count)) {
+      set_error("batched_mll_grad: the workspace was planned for other offsets/count/d");
+      return GPK_ERR_BAD_ARG;
+    }
   }
   const int iters = refine_iters >= 0 ? refine_iters : 1;
   return batched_mll_grad(kind, x, y, offsets_host, count, (int)d, theta, nls, jitter, iters, ws,

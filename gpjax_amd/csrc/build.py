@@ -22,7 +22,8 @@ CSRC = Path(__file__).resolve().parent
 PKG = CSRC.parent
 OUT = PKG / "libgpk.so"
 BUILD = CSRC / "_build"
-SOURCES = ["api.hip", "gram.hip", "leaf.hip", "gemm.hip", "adjoint.hip", "vec.hip", "elbo.hip"]
+SOURCES = ["api.hip", "gram.hip", "leaf.hip", "gemm.hip", "adjoint.hip", "vec.hip", "elbo.hip",
+           "fit.hip"]
 ARCH = "gfx950"
 # Packed fp32 ops are disabled for every device pass: some v_pk_*_f32 forms have
 # produced silent wrong results on these parts.

@@ -922,6 +922,10 @@ int gpk_elbo_backward(int kind, int prec, const double* x, const double* y, int6
     set_error("collapsed ELBO: workspace %zu bytes < required %zu", ws_bytes, Lo.total);
     return GPK_ERR_WORKSPACE;
   }
+  if (reinterpret_cast<uintptr_t>(ws) % 256) {
+    set_error("collapsed ELBO: workspace must be 256-byte aligned");
+    return GPK_ERR_BAD_ARG;
+  }
   return elbo_backward_impl(kind, x, y, n, z, m, (int)d, theta, nls, gbar_host, ws, Lo, grad,
                             grad_z, (hipStream_t)stream);
 }

@@ -26,9 +26,20 @@ from .mean_functions import Constant
 
 
 def _precision_id(precision: str | None) -> int:
+    """``prec`` code of "fp64", "mixed" or "mixed:P,T,L" (the products of the
+    Cholesky updates, the triangular inverse and the fused Sigma^-1 pass)."""
     p = precision or config.precision
+    if isinstance(p, str) and p.startswith("mixed:"):
+        try:
+            stages = [int(v) for v in p[len("mixed:"):].split(",")]
+        except ValueError:
+            stages = []
+        if len(stages) != 3:
+            raise ValueError(f"mixed arithmetic must read 'mixed:P,T,L', got {p!r}")
+        return _lib.mixed_arith(*stages)
     if p not in _lib.PRECISIONS:
-        raise ValueError(f"precision must be one of {sorted(_lib.PRECISIONS)}, got {p!r}")
+        raise ValueError(f"precision must be one of {sorted(_lib.PRECISIONS)} or "
+                         f"'mixed:P,T,L', got {p!r}")
     return _lib.PRECISIONS[p]
 
 

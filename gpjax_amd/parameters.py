@@ -65,7 +65,9 @@ class Real(Parameter):
 
 
 def softplus(x: torch.Tensor) -> torch.Tensor:
-    return torch.nn.functional.softplus(x)
+    """log(1 + e^x) as max(x, 0) + log1p(e^-|x|): no overflow, and the same
+    formula as the device Adam step (``gpk_adam_step``)."""
+    return torch.clamp(x, min=0.0) + torch.log1p(torch.exp(-x.abs()))
 
 
 def inverse_softplus(y: torch.Tensor) -> torch.Tensor:

@@ -34,7 +34,9 @@ extern "C" {
 /* status codes */
 #define GPK_OK 0
 #define GPK_ERR_BAD_ARG 1
-#define GPK_ERR_NOT_PD 2
+/* 2 is unused: a non-positive pivot is found on the device, so it is reported
+ * through result[GPK_RES_INFO] (1 + pivot index) and a NaN objective, never by
+ * a status code (which would need a host sync). */
 #define GPK_ERR_NO_DEVICE 3
 #define GPK_ERR_HIP 4
 #define GPK_ERR_WORKSPACE 5
@@ -45,9 +47,22 @@ extern "C" {
 #define GPK_KERNEL_MATERN32 2
 #define GPK_KERNEL_MATERN52 3
 
-/* arithmetic of the factorisation path */
+/* arithmetic of the factorisation path (the `prec` argument) */
 #define GPK_PREC_FP64 0  /* float64 storage, FP64 MFMA products                    */
 #define GPK_PREC_MIXED 1 /* float32 storage, BF16 split products, FP64 panels      */
+/*
+ * The mixed path with the products of each stage named per call: P for the
+ * Cholesky updates, T for the triangular inverse, L for the fused Sigma^-1
+ * pass, each 6 (bf16 hi/mid/lo split, six products: about float32-product
+ * accuracy), 3 (bf16 hi/lo split, three products), 4 (fp16 hi/lo split of
+ * power-of-two scaled operands) or 0 (float64 products of the float32
+ * operands).  GPK_PREC_MIXED is GPK_PREC_MIXED_ARITH(6, 3, 6).  No counterpart
+ * in the reference (which has one float64 path); it trades speed against the
+ * accuracy figures the benchmark reports.
+ */
+#define GPK_PREC_ARITH_FLAG 0x100
+#define GPK_PREC_MIXED_ARITH(p, t, l) \
+  (GPK_PREC_MIXED | GPK_PREC_ARITH_FLAG | ((p) << 12) | ((t) << 16) | ((l) << 20))
 
 /* result[] slots written by gpk_mll_forward */
 #define GPK_RES_MLL 0      /* marginal log-likelihood (NaN if not PD)             */
@@ -55,6 +70,11 @@ extern "C" {
 #define GPK_RES_QUAD 2     /* r^T Sigma^-1 r                                      */
 #define GPK_RES_RESID 3    /* ||r - Sigma alpha|| / ||r|| after refinement        */
 #define GPK_RES_INFO 4     /* 0, or 1 + index of the first non-positive pivot     */
+/* Mixed path only (0 in the FP64 path): a-posteriori check of the computed
+ * inverse factor X ~ L^-1 with E = I - Sigma X^T X, Sigma in float64, over
+ * Rademacher probes z:                                                        */
+#define GPK_RES_PROBE 5    /* sqrt(mean ||E z||^2 / n)  (~ ||E||_F / sqrt(n))     */
+#define GPK_RES_TRACE 6    /* mean z^T E z  (estimates tr E ~ error of log|Sigma|) */
 #define GPK_RESULT_LEN 8
 
 const char* gpk_version(void);
@@ -105,9 +125,12 @@ int gpk_mll_forward(int kind, int prec, const double* x, const double* y, int64_
  * Sigma^-1) dSigma/dtheta), layout of theta.  Must follow gpk_mll_forward on the
  * same workspace and stream.  Sigma^-1 is never stored: each tile of
  * L^-T L^-1 is formed on the matrix cores and reduced against the kernel
- * derivative tile, recomputed from x, in the same pass.  The factor L in the
- * workspace is replaced by L^-T (read along rows by that pass); L^-1 and alpha
- * stay, so the backward may be repeated.
+ * derivative tile, recomputed from x, in the same pass.
+ * The workspace is an in/out buffer of this call: the factor L in it is
+ * replaced by L^-T (read along rows by that pass); L^-1 and alpha stay, so the
+ * backward may be repeated.  A caller that treats its buffers as immutable
+ * (an XLA FFI handler) declares ws as aliased input/output
+ * (input_output_aliases) or passes a copy.
  */
 int gpk_mll_backward(int kind, int prec, const double* x, int64_t n, int64_t d,
                      const double* theta, int nls, double gbar_host, void* ws, size_t ws_bytes,
@@ -199,7 +222,9 @@ int gpk_loocv_forward(int kind, int prec, const double* x, const double* y, int6
  * tests/test_objectives.py:123-126).  Must follow gpk_loocv_forward on the same
  * workspace and stream: forms Sigma^-1 = L^-T L^-1 in the buffer that held L and
  * reduces W = alpha w^T + w alpha^T - Sigma^-1 diag(c^2) Sigma^-1 against
- * dSigma/dtheta (w = Sigma^-1 (alpha / d), c^2 = 1/d + alpha^2/d^2).
+ * dSigma/dtheta (w = Sigma^-1 (alpha / d), c^2 = 1/d + alpha^2/d^2).  Like
+ * gpk_mll_backward, the workspace is an in/out buffer (L is overwritten; L^-1,
+ * alpha and d stay, so the backward may be repeated).
  */
 int gpk_loocv_backward(int kind, int prec, const double* x, int64_t n, int64_t d,
                        const double* theta, int nls, double gbar_host, void* ws, size_t ws_bytes,
@@ -225,7 +250,10 @@ size_t gpk_batched_workspace_bytes(const int64_t* offsets_host, int64_t count, i
 /*
  * Writes the batch's problem map into ws (synchronous host-to-device copies).
  * Call once for a given (offsets, ws) before gpk_batched_mll_grad, outside any
- * HIP-graph capture.
+ * HIP-graph capture.  The library remembers, per workspace address, which
+ * (offsets, count, d) it planned (a mutex-guarded host table: with the autotune
+ * cache, its only process-wide state), and gpk_batched_mll_grad refuses a
+ * workspace planned for anything else.
  */
 int gpk_batched_plan(const int64_t* offsets_host, int64_t count, int64_t d, void* ws,
                      size_t ws_bytes);
@@ -243,15 +271,23 @@ int gpk_batched_mll_grad(int kind, const double* x, const double* y, const int64
                          double* grad, void* stream);
 
 /*
- * Products per element used by the GPK_PREC_MIXED stages (3 = bf16 hi/lo
- * split, 6 = hi/mid/lo split, 4 = fp16 hi/lo split of power-of-two scaled
- * operands, 0 = float64 products of the float32 operands):
- * Cholesky updates, triangular inverse, and the
- * fused Sigma^-1 adjoint.  Process-wide; defaults 6, 3, 6.  No counterpart in
- * the reference (which has a single float64 path); it exists to trade speed
- * against the accuracy figures the benchmark reports.
+ * One optimiser step of gpjax.fit (gpjax/fit.py:133-180) for `count` float64
+ * coordinates, in one launch: with f = softplus where softplus_mask[i] != 0
+ * (DEFAULT_BIJECTION "positive" / "non_negative", gpjax/parameters.py:140-146)
+ * and the identity elsewhere, and grad the loss gradient w.r.t. the
+ * constrained values f(raw):
+ *   g = grad * f'(raw);  m = b1 m + (1-b1) g;  v = b2 v + (1-b2) g^2;  step += 1;
+ *   raw -= lr (m / (1 - b1^step)) / (sqrt(v / (1 - b2^step)) + eps);
+ *   value = f(raw);  history[step - 1] = *loss  (when history is non-null and
+ *   step <= history_len).
+ * This is optax.adam (fit.py:161-162) with eps_root = 0.  All pointers are
+ * device pointers; `step` is one float64 (0 before the first step).  Launch
+ * only, so a whole training step may be captured in a HIP graph.
  */
-int gpk_set_mixed_arith(int potrf_products, int trtri_products, int lauum_products);
+int gpk_adam_step(int64_t count, const double* grad, const uint8_t* softplus_mask, double* raw,
+                  double* m, double* v, double* step, double lr, double b1, double b2, double eps,
+                  double* value, const double* loss, double* history, int64_t history_len,
+                  void* stream);
 
 /*
  * Phase profiling (no counterpart in the reference).  While enabled, each

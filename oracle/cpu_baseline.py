@@ -78,6 +78,8 @@ def other_workload(a) -> dict:
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=6144)
+    ap.add_argument("--ns", default="2048,4096,8192",
+                    help="exact_large: sample sizes whose times are fitted by c * N^3")
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
     ap.add_argument("--runs", type=int, default=3)
     ap.add_argument("--target-n", type=int, default=65536)
@@ -93,23 +95,28 @@ def main() -> None:
     from gpjax_amd import workloads
     from oracle import gp_oracle as O
 
-    w = workloads.exact_large(n=a.n)
-    args = (w["kind"], w["X"], w["y"], w["lengthscale"], w["variance"], w["obs_stddev"],
-            w["jitter"])
-    O.conjugate_mll_grad(*args)
+    ns = [int(v) for v in a.ns.split(",")]
     times = []
-    for _ in range(a.runs):
+    for i, n in enumerate(ns):
+        w = workloads.exact_large(n=n)
+        args = (w["kind"], w["X"], w["y"], w["lengthscale"], w["variance"], w["obs_stddev"],
+                w["jitter"])
+        if i == 0:
+            O.conjugate_mll_grad(*args)  # warm-up (BLAS threads, page faults)
         t0 = time.perf_counter()
         O.conjugate_mll_grad(*args)
         times.append(time.perf_counter() - t0)
-    t = statistics.median(times)
-    scale = (a.n / a.target_n) ** 3
+    # t = c N^3 with c the mean of t / N^3: least squares on the relative residuals
+    c = sum(t / n ** 3 for n, t in zip(ns, times)) / len(ns)
+    t_target = c * a.target_n ** 3
     print(json.dumps({
-        "value": 1.0 / t * scale, "unit": "evals/s", "cores": a.threads, "kind": "port",
+        "value": 1.0 / t_target, "unit": "evals/s", "cores": a.threads, "kind": "port",
         "sample": (f"float64 oracle (LU slogdet/solve + explicit inverse for the gradient) on "
-                   f"exact_large with N={a.n}, D=16: median {t:.3f}s of {a.runs} runs after 1 "
-                   f"warm-up, {a.threads} threads; rate extrapolated to N={a.target_n} by N^3"),
-        "sample_seconds_per_eval": t,
+                   f"exact_large, D=16, at N={ns}: one timed run each after one warm-up, "
+                   f"{a.threads} threads; times {[round(t, 3) for t in times]} s fitted by "
+                   f"t = c N^3 (c = {c:.4e} s), rate extrapolated to N={a.target_n}"),
+        "sample_seconds": [round(t, 4) for t in times], "fit_c_seconds_per_n3": c,
+        "fit_max_rel_residual": max(abs(t - c * n ** 3) / (c * n ** 3) for n, t in zip(ns, times)),
     }))
 
 

@@ -109,12 +109,47 @@ def test_elbo_predict_workspace_and_validation():
     assert rc == 1 and "xt" in lib.gpk_last_error().decode()
 
 
-def test_mixed_arith_option_validation():
+def test_mixed_arith_is_a_per_call_precision_code():
+    """The stage arithmetic travels in the call's `prec` argument
+    (GPK_PREC_MIXED_ARITH); there is no process-wide setting."""
     lib = _lib.lib()
-    assert lib.gpk_set_mixed_arith(5, 3, 6) == 1
-    assert lib.gpk_set_mixed_arith(4, 3, 4) == 0   # fp16 hi/lo split of scaled operands
-    assert lib.gpk_set_mixed_arith(0, 3, 0) == 0   # float64 products of float32 operands
-    assert lib.gpk_set_mixed_arith(6, 3, 6) == 0
+    assert not hasattr(lib, "gpk_set_mixed_arith")
+    plain = lib.gpk_mll_workspace_bytes(3000, 3, 1)
+    for stages in ((6, 3, 6), (4, 3, 4), (0, 3, 0), (6, 6, 6)):
+        code = _lib.mixed_arith(*stages)
+        assert code & 0xFF == 1 and code & _lib.ARITH_FLAG
+        assert lib.gpk_mll_workspace_bytes(3000, 3, code) == plain
+    with pytest.raises(ValueError):
+        _lib.mixed_arith(5, 3, 6)
+    bad = 1 | _lib.ARITH_FLAG | (5 << 12) | (3 << 16) | (6 << 20)   # 5 products: no such split
+    for code in (bad, 0 | _lib.ARITH_FLAG, 1 | (1 << 9), 2):
+        assert lib.gpk_mll_workspace_bytes(3000, 3, code) == 0
+        dummy = ctypes.c_void_p(0x1000)
+        rc = lib.gpk_mll_forward(0, code, dummy, dummy, 100, 2, dummy, 1, 1e-6, -1, dummy,
+                                 1 << 40, dummy, None)
+        assert rc == 1 and "precision" in lib.gpk_last_error().decode()
+
+
+def test_backward_entry_points_check_workspace_alignment():
+    lib = _lib.lib()
+    odd = ctypes.c_void_p(0x1008)
+    dummy = ctypes.c_void_p(0x1000)
+    rc = lib.gpk_mll_backward(0, 0, dummy, 100, 2, dummy, 1, 1.0, odd, 1 << 40, dummy, None)
+    assert rc == 1 and "aligned" in lib.gpk_last_error().decode()
+    rc = lib.gpk_loocv_backward(0, 1, dummy, 100, 2, dummy, 1, 1.0, odd, 1 << 40, dummy, None)
+    assert rc == 1 and "aligned" in lib.gpk_last_error().decode()
+    rc = lib.gpk_elbo_backward(0, 0, dummy, dummy, 100, dummy, 10, 2, dummy, 1, 1.0, odd,
+                               1 << 40, dummy, dummy, None)
+    assert rc == 1 and "aligned" in lib.gpk_last_error().decode()
+
+
+def test_adam_step_validation():
+    lib = _lib.lib()
+    d = ctypes.c_void_p(0x1000)
+    assert lib.gpk_adam_step(0, d, d, d, d, d, d, 0.1, 0.9, 0.999, 1e-8, d, d, d, 1, None) == 1
+    assert lib.gpk_adam_step(3, d, d, d, d, d, d, 0.0, 0.9, 0.999, 1e-8, d, d, d, 1, None) == 1
+    assert lib.gpk_adam_step(3, d, d, d, d, d, d, 0.1, 1.0, 0.999, 1e-8, d, d, d, 1, None) == 1
+    assert lib.gpk_adam_step(3, None, d, d, d, d, d, 0.1, 0.9, 0.999, 1e-8, d, d, d, 1, None) == 1
 
 
 def _offsets(*rows):
@@ -166,6 +201,10 @@ def test_batched_workspace_and_offsets_checks():
     bad = _offsets(5, 7)
     bad[0] = 1
     assert lib.gpk_batched_plan(bad, 2, 3, dummy, 1 << 40) == 1
+    # a workspace that no plan has written is refused before any launch
+    rc = lib.gpk_batched_mll_grad(0, dummy, dummy, off, 2, 3, dummy, 1, 1e-6, -1,
+                                  ctypes.c_void_p(0x7f0000), 1 << 40, dummy, dummy, None)
+    assert rc == 1 and "plan" in lib.gpk_last_error().decode()
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):

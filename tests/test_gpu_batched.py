@@ -114,7 +114,7 @@ def test_batched_fit_graph_replay_matches_eager(cuda):
                                                variance=1.0, obs_stddev=0.2)
 
     def objective(mdl, data):
-        return mdl.mll().sum()
+        return -mdl.mll().sum()
 
     m1, h1 = gpx.fit(model=model(), objective=objective, train_data=None, num_iters=10,
                      learning_rate=0.05)
@@ -126,3 +126,53 @@ def test_batched_fit_graph_replay_matches_eager(cuda):
     assert float(h1[-1]) < float(h1[0])
     assert torch.allclose(m1.lengthscale.value.detach(), m2.lengthscale.value.detach(),
                           rtol=1e-12)
+
+
+def _c5_sample(w, count=16, seed=0):
+    """Problems of the many-small-GPs recipe to check against the oracle: the
+    smallest and the largest N_i, the four smallest noise levels (~1e-3), the
+    four shortest lengthscales and random others."""
+    sizes = np.asarray(w["sizes"])
+    sd = np.asarray(w["obs_stddev"])
+    lmin = np.asarray(w["lengthscale"]).min(1)
+    pick = [int(np.argmin(sizes)), int(np.argmax(sizes))]
+    pick += [int(p) for p in np.argsort(sd)[:4]] + [int(p) for p in np.argsort(lmin)[:4]]
+    rng = np.random.default_rng(seed)
+    for p in rng.permutation(len(sizes)):
+        if len(set(pick)) >= count:
+            break
+        pick.append(int(p))
+    return sorted(set(pick))
+
+
+def test_many_small_gps_recipe_against_oracle(cuda):
+    """The full 4,096-problem batch of the many-small-GPs recipe (N_i in 32..1024,
+    D = 4, RBF ARD, noise down to 1e-3) in one call; 16 sampled problems --
+    including the smallest and largest N_i and the noise-1e-3 ones -- against the
+    float64 oracle (LU slogdet / solve, closed-form gradient) at 1e-10 on the MLL
+    and 1e-8 (norm-wise) on the gradient."""
+    w = workloads.batched()
+    m = gpx.batched.BatchedConjugateMLL(w["Xs"], w["ys"], kernel="rbf",
+                                        lengthscale=w["lengthscale"], variance=w["variance"],
+                                        obs_stddev=w["obs_stddev"], jitter=w["jitter"])
+    mll = m.mll()
+    mll.sum().backward()
+    assert bool((m.last_result[:, 4] == 0).all())
+    got_mll = mll.detach().cpu().numpy()
+    gls = m.lengthscale.value.grad.cpu().numpy()
+    gvar = m.variance.value.grad.cpu().numpy()
+    gsd = m.obs_stddev.value.grad.cpu().numpy()
+    worst = (0.0, 0.0)
+    for p in _c5_sample(w):
+        ref = O.conjugate_mll_grad("rbf", w["Xs"][p], w["ys"][p], w["lengthscale"][p],
+                                   w["variance"][p], w["obs_stddev"][p], w["jitter"])
+        got = np.concatenate([gls[p], [gvar[p], gsd[p]]])
+        want = np.concatenate([ref["lengthscale"], [ref["variance"], ref["obs_stddev"]]])
+        e_mll = abs(got_mll[p] - ref["mll"]) / abs(ref["mll"])
+        e_g = np.linalg.norm(got - want) / np.linalg.norm(want)
+        print(f"problem {p}: n={len(w['Xs'][p])} sd={w['obs_stddev'][p]:.2e} "
+              f"rel_mll={e_mll:.2e} rel_grad={e_g:.2e}")
+        worst = (max(worst[0], e_mll), max(worst[1], e_g))
+        assert e_mll <= 1e-10, (p, e_mll)
+        assert e_g <= 1e-8, (p, e_g)
+    print(f"worst: mll {worst[0]:.2e}, grad {worst[1]:.2e}")

@@ -109,9 +109,9 @@ def test_mll_grad_fp64_parity(cuda, n, d, kind, ard):
     assert np.linalg.norm(g - gr) <= 1e-8 * np.linalg.norm(gr) + 1e-12, (g, gr)
 
 
-def _mixed_vs_oracle(cuda, X, y, kind, ls, var, sd):
+def _mixed_vs_oracle(cuda, X, y, kind, ls, var, sd, precision="mixed"):
     post, D, k = _model(cuda, X, y, kind, ls, var, sd)
-    mll, res = gpx.objectives.conjugate_mll_with_info(post, D, precision="mixed")
+    mll, res = gpx.objectives.conjugate_mll_with_info(post, D, precision=precision)
     mll.backward()
     ref = O.conjugate_mll_grad(kind, X, y, ls, var, sd)
     g = np.concatenate([np.atleast_1d(k.lengthscale.value.grad.cpu().numpy()),
@@ -145,14 +145,11 @@ def test_fp16_split_arithmetic(cuda, arith):
     Cholesky updates and/or the fused Sigma^-1 pass.  It is about four times
     less accurate per product than BF16x6 (2^-22 against 2^-24): within 2e-6 on
     the MLL and 5e-4 on the gradient here (at N = 65,536: 6.3e-7 and 1.3e-4, so it
-    is not the default).  Process-wide setting, restored afterwards."""
+    is not the default).  Selected per call through the precision code."""
     w = workloads.exact_large(n=2048)
-    assert _lib.lib().gpk_set_mixed_arith(*arith) == 0
-    try:
-        rel_mll, rel_g, resid = _mixed_vs_oracle(cuda, w["X"], w["y"], "rbf", w["lengthscale"],
-                                                 w["variance"], w["obs_stddev"])
-    finally:
-        assert _lib.lib().gpk_set_mixed_arith(6, 3, 6) == 0
+    rel_mll, rel_g, resid = _mixed_vs_oracle(cuda, w["X"], w["y"], "rbf", w["lengthscale"],
+                                             w["variance"], w["obs_stddev"],
+                                             precision="mixed:%d,%d,%d" % arith)
     assert rel_mll <= 2e-6
     assert rel_g <= 5e-4
     assert resid <= 1e-10
@@ -169,3 +166,45 @@ def test_mixed_accuracy_envelope(cuda, n, d, kind):
     assert rel_mll <= 1e-5
     assert rel_g <= 1e-2
     assert resid <= 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_not_positive_definite_reports_pivot_and_nan(cuda, precision):
+    """An indefinite Sigma: 300 points on a grid 6 lengthscales apart (so K is
+    the identity to 1e-8) with row 200 a copy of row 5, and jitter + obs_sd^2 =
+    -0.1 (the reference accepts any jitter, linalg/utils.py:39-65).  The pair
+    (5, 200) has the block [[0.9, 1], [1, 0.9]], so the Cholesky pivot of row 200
+    is 0.9 - 1/0.9 < 0 and every pivot before it is 0.9.
+
+    Contract: NaN MLL, result[GPK_RES_INFO] = 1 + 200, NaN gradient.  This is a
+    deliberate divergence from the reference, whose LU slogdet discards the sign
+    (linalg/operations.py:163-165) and returns a finite number; the oracle shows
+    that number, so the test also pins what the reference would have said."""
+    g = np.stack(np.meshgrid(np.arange(20.0), np.arange(15.0)), -1).reshape(-1, 2) * 3.0
+    X = g[:300].copy()
+    X[200] = X[5]
+    y = np.sin(X[:, 0]) + np.cos(X[:, 1])
+    post, D, k = _model(cuda, X, y, "rbf", 0.5, 1.0, 0.1, jitter=-0.11)
+    mll, res = gpx.objectives.conjugate_mll_with_info(post, D, precision=precision)
+    mll.backward()
+    r = res.cpu().numpy()
+    assert np.isnan(float(mll.detach()))
+    assert int(r[_lib.RES_INFO]) == 201
+    assert np.isnan(k.lengthscale.value.grad.item())
+    assert np.isnan(k.variance.value.grad.item())
+    ref = O.conjugate_mll_parts("rbf", X, y, 0.5, 1.0, 0.1, jitter=-0.11)
+    assert np.isfinite(ref["mll"])   # the reference's LU path: finite, sign dropped
+
+
+def test_not_positive_definite_first_pivot(cuda):
+    """A negative kernel variance makes the very first pivot negative: info = 1."""
+    rng = np.random.default_rng(3)
+    X = rng.uniform(-2, 2, (150, 2))
+    y = rng.standard_normal(150)
+    post, D, k = _model(cuda, X, y, "matern32", 0.7, 1.0, 0.1)
+    theta_var = k.variance.value
+    with torch.no_grad():
+        theta_var.fill_(-1.0)
+    mll, res = gpx.objectives.conjugate_mll_with_info(post, D, precision="fp64")
+    assert np.isnan(float(mll.detach()))
+    assert int(res[_lib.RES_INFO].item()) == 1
