@@ -46,6 +46,7 @@ SIGNATURES = {
     "gpk_mll_workspace_bytes": (_S, [_L, _L, _I]),
     "gpk_mll_forward": (_I, [_I, _I, _P, _P, _L, _L, _P, _I, _D, _I, _P, _S, _P, _P]),
     "gpk_mll_backward": (_I, [_I, _I, _P, _L, _L, _P, _I, _D, _P, _S, _P, _P]),
+    "gpk_mll_forward_noalloc": (_I, [_I, _I, _P, _P, _L, _L, _P, _I, _D, _I, _P, _P, _P]),
     "gpk_mfma_selftest": (_I, [_P, _P, _P]),
     "gpk_elbo_workspace_bytes": (_S, [_L, _L, _L, _I]),
     "gpk_elbo_forward": (_I, [_I, _I, _P, _P, _L, _P, _L, _L, _P, _I, _D, _P, _S, _P, _P]),

@@ -899,12 +899,91 @@ int batched_mll_grad(int kind, const double* x, const double* y, const int64_t* 
   return GPK_OK;
 }
 
+// ---- allocation-free exact-GP evaluation ----
+// For callers that cannot allocate device memory (code under HIP-graph
+// capture, serving loops, harnesses that forbid hipMalloc): the workspace is a
+// file-scope device array instead of a caller-provided buffer.  Each device
+// holds its own copy for the life of the process; host code reaches it only
+// through the address hipGetSymbolAddress returns on that device (the array's
+// name is not a device address in host code).
+constexpr int64_t NOALLOC_MAX_N = 1024;
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)20 << 20;  // 20 MiB
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Address of this device's scratch array, looked up once per device (with
+// that device current) and kept.
+int device_scratch(void** out) {
+  int dev = -1;
+  GPK_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    set_error("noalloc: device %d outside the %d devices supported", dev, NOALLOC_MAX_DEVICES);
+    return GPK_ERR_NO_DEVICE;
+  }
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    GPK_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      set_error("noalloc: hipGetSymbolAddress gave no address for the scratch array");
+      return GPK_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return GPK_OK;
+}
+
 }  // namespace
 }  // namespace gpk
 
 using namespace gpk;
 
 extern "C" {
+
+int gpk_mll_forward_noalloc(int kind, int prec, const double* x, const double* y, int64_t n,
+                            int64_t d, const double* theta, int nls, double jitter,
+                            int refine_iters, double* result, double* grad, void* stream) {
+  int rc = check_common(kind, prec, x, n, d, theta, nls);
+  if (rc) return rc;
+  if (!y || !result) {
+    set_error("mll_forward_noalloc: null y/result");
+    return GPK_ERR_BAD_ARG;
+  }
+  if (n > NOALLOC_MAX_N) {
+    set_error("mll_forward_noalloc: n = %lld exceeds the %lld rows its fixed scratch holds",
+              (long long)n, (long long)NOALLOC_MAX_N);
+    return GPK_ERR_BAD_ARG;
+  }
+  Arith3 ar;
+  const int base = decode_prec(prec, &ar);
+  const Layout Lo = make_layout(n, d, base);
+  if (Lo.total > NOALLOC_SCRATCH_BYTES) {
+    set_error("mll_forward_noalloc: %zu bytes of scratch needed, %zu available", Lo.total,
+              NOALLOC_SCRATCH_BYTES);
+    return GPK_ERR_WORKSPACE;
+  }
+  void* ws = nullptr;
+  if ((rc = device_scratch(&ws))) return rc;
+  const int iters = refine_iters >= 0 ? refine_iters : (base == GPK_PREC_FP64 ? 1 : 4);
+  hipStream_t s = (hipStream_t)stream;
+  if (base == GPK_PREC_FP64) {
+    GPK_TRY(mll_forward_t<double>(kind, ar, x, y, n, (int)d, theta, nls, jitter, iters, ws, Lo,
+                                  result, s));
+    if (grad)
+      GPK_TRY(mll_backward_t<double>(kind, ar, x, n, (int)d, theta, nls, 1.0, ws, Lo, grad, s));
+    return GPK_OK;
+  }
+  GPK_TRY(mll_forward_t<float>(kind, ar, x, y, n, (int)d, theta, nls, jitter, iters, ws, Lo,
+                               result, s));
+  if (grad)
+    GPK_TRY(mll_backward_t<float>(kind, ar, x, n, (int)d, theta, nls, 1.0, ws, Lo, grad, s));
+  return GPK_OK;
+}
 
 const char* gpk_version(void) { return "gpk 0.1.0 (gfx950)"; }
 

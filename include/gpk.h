@@ -136,6 +136,22 @@ int gpk_mll_backward(int kind, int prec, const double* x, int64_t n, int64_t d,
                      const double* theta, int nls, double gbar_host, void* ws, size_t ws_bytes,
                      double* grad, void* stream);
 
+/*
+ * gpk_mll_forward, and with grad != NULL also gpk_mll_backward (gbar = 1), with
+ * no workspace argument and no allocation: the library's own fixed device
+ * scratch array (20 MiB per device, held until the process ends) takes the
+ * workspace's place, so n <= 1024 (larger n is refused with GPK_ERR_BAD_ARG).
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP-graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc), e.g. many small GPs in a Bayesian-optimisation loop.
+ * Results equal gpk_mll_forward / gpk_mll_backward's.  Every call on a device
+ * shares that device's scratch: calls that overlap on one device (concurrent
+ * streams or threads) race.
+ */
+int gpk_mll_forward_noalloc(int kind, int prec, const double* x, const double* y, int64_t n,
+                            int64_t d, const double* theta, int nls, double jitter,
+                            int refine_iters, double* result, double* grad, void* stream);
+
 /* Bytes of workspace gpk_elbo_forward / gpk_elbo_backward need. */
 size_t gpk_elbo_workspace_bytes(int64_t n, int64_t m, int64_t d, int prec);
 

@@ -212,3 +212,13 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIB_PATH", tmp_path / "libgpk.so")
     with pytest.raises(RuntimeError, match="no host fallback"):
         _lib.lib()
+
+
+def test_noalloc_limits_checked_on_the_host():
+    """gpk_mll_forward_noalloc refuses n above its fixed scratch (1,024 rows)
+    before touching the device; the FP64 layout at its limit fits the 20 MiB."""
+    lib = _lib.lib()
+    d = ctypes.c_void_p(0x1000)
+    rc = lib.gpk_mll_forward_noalloc(0, 0, d, d, 1025, 4, d, 1, 1e-6, -1, d, None, None)
+    assert rc == 1 and "1024" in lib.gpk_last_error().decode()
+    assert lib.gpk_mll_workspace_bytes(1024, 32, 0) <= 20 << 20
