@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -317,6 +318,8 @@ def main() -> int:
     g_m = grads(params)
     mll_m = float(mll_m.detach())
     resid_m = float(res_m[_lib.RES_RESID])
+    probe_m = float(res_m[_lib.RES_PROBE])
+    flag_m = float(res_m[_lib.RES_FLAG])
     print(f"# mixed: {t_mixed / args.steps:.3f}s/eval mll={mll_m:.6f} resid={resid_m:.2e} "
           f"phases={phases_m}", file=sys.stderr, flush=True)
 
@@ -398,6 +401,11 @@ def main() -> int:
                             "mll": mll_m, "mll_fp64": fp64["mll"],
                             "refine_resid_mixed": resid_m,
                             "gate_mll_1e-6": rel_mll <= 1e-6, "gate_grad_1e-4": rel_g <= 1e-4}
+    # the mixed path's own a-posteriori check (Rademacher probes of I - Sigma X^T X)
+    line["mixed_check"] = {"probe_frob_norm": probe_m * math.sqrt(args.n),
+                           "flag": flag_m, "flag_threshold": 0.75,
+                           "meaning": "flag 0: mixed result kept; 1: flagged; 2: recomputed in "
+                                      "float64 (then the timed steps are not the mixed path)"}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -8,11 +8,11 @@ User code shaped like GPJax's examples works against these modules:
 All arithmetic runs in the HIP library ``libgpk.so`` (C ABI in include/gpk.h).
 """
 
-from . import (batched, config, gps, kernels, likelihoods, mean_functions, objectives,
+from . import (batched, config, gps, kernels, likelihoods, linalg, mean_functions, objectives,
                parameters, variational_families)
 from .dataset import Dataset
 from .fit import fit
 
 __version__ = "0.1.0"
-__all__ = ["Dataset", "batched", "config", "fit", "gps", "kernels", "likelihoods",
+__all__ = ["Dataset", "batched", "config", "fit", "gps", "kernels", "likelihoods", "linalg",
            "mean_functions", "objectives", "parameters", "variational_families"]

@@ -29,6 +29,7 @@ def mixed_arith(potrf: int, trtri: int, lauum: int) -> int:
     return PRECISIONS["mixed"] | ARITH_FLAG | (potrf << 12) | (trtri << 16) | (lauum << 20)
 RESULT_LEN = 8
 RES_MLL, RES_LOGDET, RES_QUAD, RES_RESID, RES_INFO = 0, 1, 2, 3, 4
+RES_PROBE, RES_TRACE, RES_FLAG = 5, 6, 7
 
 _c = ctypes
 _P = _c.c_void_p
@@ -59,10 +60,17 @@ SIGNATURES = {
                               _P, _P, _P]),
     "gpk_predict_workspace_bytes": (_S, [_L, _L, _L, _I, _I]),
     "gpk_predict": (_I, [_I, _I, _P, _P, _L, _L, _P, _L, _P, _I, _D, _D, _I, _I, _P, _S, _P, _P,
-                         _P]),
+                         _P, _P]),
     "gpk_batched_workspace_bytes": (_S, [_P, _L, _L]),
     "gpk_batched_plan": (_I, [_P, _L, _L, _P, _S]),
     "gpk_batched_mll_grad": (_I, [_I, _P, _P, _P, _L, _L, _P, _I, _D, _I, _P, _S, _P, _P, _P]),
+    "gpk_linalg_workspace_bytes": (_S, [_L, _L, _I]),
+    "gpk_potrf": (_I, [_I, _P, _L, _L, _P, _L, _P, _P, _P, _S, _P]),
+    "gpk_trsm": (_I, [_P, _L, _L, _P, _L, _L, _P, _S, _P]),
+    "gpk_potri": (_I, [_P, _L, _L, _P, _L, _P, _S, _P]),
+    "gpk_logdet_tri": (_I, [_P, _L, _L, _P, _P]),
+    "gpk_mvn_workspace_bytes": (_S, [_L, _I]),
+    "gpk_mvn_log_prob": (_I, [_P, _P, _L, _L, _I, _P, _P, _P, _S, _P]),
     "gpk_adam_step": (_I, [_L, _P, _P, _P, _P, _P, _P, _D, _D, _D, _D, _P, _P, _P, _L, _P]),
     "gpk_profile_enable": (_I, [_I]),
     "gpk_profile_collect": (_I, [_P, _P]),

@@ -10,9 +10,19 @@ from __future__ import annotations
 
 device = "cuda"
 precision = "fp64"
-# Float64 iterative-refinement steps for alpha = Sigma^-1 r; None = library
-# default (1 for fp64, 4 for mixed).
+# Most float64 iterative-refinement steps for alpha = Sigma^-1 r (they stop
+# early once the residual is below 1e-12); None = library default (1 for fp64,
+# 6 for mixed).
 refine_iters = None
+# What a mixed-precision evaluation does when its a-posteriori check (the
+# Rademacher probes of I - Sigma X^T X, result slot GPK_RES_FLAG) says the
+# result may be outside 1e-6 (MLL) / 1e-4 (gradient) of the float64 answer:
+#   "fallback" recompute in float64 and return that (result slot FLAG = 2),
+#   "raise"    raise objectives.MixedPrecisionError,
+#   "warn"     return the mixed result with a warning,
+#   "off"      no check (and no host sync for it).
+# Under HIP-graph capture the check is skipped (it needs a host read).
+mixed_check = "fallback"
 
 
 def set_precision(p: str) -> None:

@@ -116,7 +116,11 @@ struct Layout {
   int64_t htop = 0;
   size_t ts = 0;
   size_t off_L = 0, off_X = 0, off_T = 0, off_leafinv = 0, off_leafld = 0, off_leafinfo = 0;
+  // r, alpha, u and v each lead a block of nvec vectors (np apart): with the
+  // mixed path's probes, [r z1 z2], [alpha w1 w2], [u u1 u2], [v v1 v2].
+  int nvec = 1;
   size_t off_r = 0, off_alpha = 0, off_u = 0, off_v = 0, off_res = 0, off_delta = 0;
+  size_t off_skip = 0;
   size_t off_dblk = 0;
   size_t off_part = 0, off_gpart = 0, off_scal = 0, off_probe = 0, total = 0;
 };
@@ -137,14 +141,17 @@ Layout make_layout(int64_t n, int64_t d, int prec) {
   L.off_leafinv = o; o += al((size_t)np * NB * sizeof(double));
   L.off_leafld = o; o += al((size_t)nleaf * sizeof(double));
   L.off_leafinfo = o; o += al((size_t)nleaf * sizeof(int));
-  L.off_r = o; o += al((size_t)np * sizeof(double));
-  L.off_alpha = o; o += al((size_t)np * sizeof(double));
-  L.off_u = o; o += al((size_t)np * sizeof(double));
-  L.off_v = o; o += al((size_t)np * sizeof(double));
+  L.nvec = prec == GPK_PREC_FP64 ? 1 : 1 + NPROBE;
+  const size_t VB = al((size_t)L.nvec * np * sizeof(double));
+  L.off_r = o; o += VB;
+  L.off_alpha = o; o += VB;
+  L.off_u = o; o += VB;
+  L.off_v = o; o += VB;
   L.off_res = o; o += al((size_t)np * sizeof(double));
   L.off_delta = o; o += al((size_t)np * sizeof(double));
   L.off_dblk = o; o += al(prec == GPK_PREC_FP64 ? 0 : (size_t)np * NB * sizeof(double));
-  L.off_part = o; o += al((size_t)nchunk * np * sizeof(double));
+  L.off_part = o; o += al((size_t)L.nvec * nchunk * np * sizeof(double));
+  L.off_skip = o; o += al(sizeof(int));
   L.off_gpart = o; o += al((size_t)ntile * (d + 2) * sizeof(double));
   L.off_scal = o; o += al(16 * sizeof(double));
   L.off_probe = o; o += al(2 * NPROBE * sizeof(double));
@@ -168,6 +175,7 @@ __global__ void prep_r_kernel(const double* __restrict__ y, const double* __rest
 
 __global__ void sub_kernel(const double* __restrict__ a, const double* __restrict__ b,
                            int64_t np, double* __restrict__ out, Batch bt) {
+  if (bskip(bt)) return;
   a = bws(a, bt);
   b = bws(b, bt);
   out = bws(out, bt);
@@ -177,10 +185,38 @@ __global__ void sub_kernel(const double* __restrict__ a, const double* __restric
 
 __global__ void add_kernel(double* __restrict__ a, const double* __restrict__ b, int64_t np,
                            Batch bt) {
+  if (bskip(bt)) return;
   a = bws(a, bt);
   b = bws(b, bt);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < np) a[i] += b[i];
+}
+
+// Refinement stop flag of each member: cleared at the start of a forward (a
+// kernel, not a memset, so that the forward stays capturable).
+__global__ void skip_clear_kernel(int* skip, Batch bt) {
+  if (threadIdx.x == 0) *bws(skip, bt) = 0;
+}
+
+__device__ double block_sum(double v, double* red);
+
+// Sets the member's stop flag once ||r - Sigma alpha|| <= REFINE_TOL ||r||.
+constexpr double REFINE_TOL = 1e-12;
+__global__ __launch_bounds__(256) void refine_check_kernel(const double* __restrict__ r,
+                                                           const double* __restrict__ res,
+                                                           int64_t np, int* skip, Batch bt) {
+  __shared__ double red[4];
+  if (bskip(bt)) return;
+  r = bws(r, bt);
+  res = bws(res, bt);
+  double rr = 0.0, ee = 0.0;
+  for (int64_t i = threadIdx.x; i < np; i += 256) {
+    rr += r[i] * r[i];
+    ee += res[i] * res[i];
+  }
+  rr = block_sum(rr, red);
+  ee = block_sum(ee, red);
+  if (threadIdx.x == 0 && ee <= REFINE_TOL * REFINE_TOL * rr) *bws(skip, bt) = 1;
 }
 
 __device__ double block_sum(double v, double* red) {
@@ -280,7 +316,18 @@ __global__ __launch_bounds__(256) void probe_stats_kernel(const double* __restri
   }
 }
 
-// result[GPK_RES_PROBE] = sqrt(mean ||E z||^2 / n), result[GPK_RES_TRACE] = mean z^T E z.
+// result[GPK_RES_PROBE] = sqrt(mean ||E z||^2 / n), result[GPK_RES_TRACE] = mean z^T E z,
+// result[GPK_RES_FLAG] = 1 when the estimate of ||E||_F = sqrt(mean ||E z||^2)
+// exceeds PROBE_FLAG_FROB, or when the refinement of alpha stopped above a
+// relative residual of 1e-10.  To first order the trace terms of the gradient
+// carry an error tr(Sigma^-1 E dSigma) <= ||E||_F ||Sigma^-1 dSigma||_F, and
+// log|Sigma| one of about tr E: with ||E||_F near 1 neither is controlled.  The
+// threshold is calibrated on 36 problems (N = 333 .. 65,536, D = 1 .. 16,
+// obs_sd 1e-3 .. 0.3, RBF and Matern-5/2): every one outside the gates
+// (|dMLL|/|MLL| > 1e-6 or gradient > 1e-4 relative to the float64 path) read
+// ||E||_F >= 0.98; the N = 65,536, D = 16 benchmark problem reads 0.40.
+constexpr double PROBE_FLAG_FROB = 0.75;
+
 __global__ void probe_finish_kernel(const double* __restrict__ stats, int64_t n,
                                     double* __restrict__ result) {
   if (threadIdx.x != 0) return;
@@ -289,8 +336,13 @@ __global__ void probe_finish_kernel(const double* __restrict__ stats, int64_t n,
     ee += stats[2 * p];
     ze += stats[2 * p + 1];
   }
-  result[GPK_RES_PROBE] = sqrt(ee / NPROBE / (double)n);
+  const double frob = sqrt(ee / NPROBE);
+  result[GPK_RES_PROBE] = frob / sqrt((double)n);
   result[GPK_RES_TRACE] = ze / NPROBE;
+  // also flagged: a refinement of alpha that did not reach 1e-10 in its steps
+  const bool bad = frob > PROBE_FLAG_FROB || !(frob == frob) ||
+                   !(result[GPK_RES_RESID] <= 1e-10);
+  result[GPK_RES_FLAG] = bad ? 1.0 : 0.0;
 }
 
 // One workgroup per gradient slot: sum the tile partials in a fixed order.
@@ -428,6 +480,19 @@ int mll_forward_t(int kind, const Arith3& ar, const double* x, const double* y, 
 
   hipLaunchKernelGGL(prep_r_kernel, gv, dim3(256), 0, s, y, theta, nls, n, np, r, bt);
   GPK_LAUNCH_CHECK();
+  const bool probes = !f64 && !bt.gp;
+  if (probes) {
+    // Rademacher probes z_p of the a-posteriori check, behind r
+    for (int p = 0; p < NPROBE; ++p) {
+      hipLaunchKernelGGL(probe_fill_kernel, gv, dim3(256), 0, s, r + (p + 1) * np, n, np, p);
+      GPK_LAUNCH_CHECK();
+    }
+  }
+  int* skip = at<int>(ws, Lo.off_skip);
+  hipLaunchKernelGGL(skip_clear_kernel, dim3(1, 1, bt.count), dim3(64), 0, s, skip, bt);
+  GPK_LAUNCH_CHECK();
+  Batch bts = bt;  // the refinement's launches, which stop once it has converged
+  bts.skip = skip;
   {
     PhaseScope ps(GPK_PHASE_GRAM, s);
     GPK_TRY(launch_kernel_matrix<TS>(kind, x, n, x, n, d, theta, nls, jitter, 2, L, np, np, np,
@@ -442,18 +507,41 @@ int mll_forward_t(int kind, const Arith3& ar, const double* x, const double* y, 
     GPK_TRY(F.trtri(0, np));
   }
   PhaseScope ps_solve(GPK_PHASE_SOLVE, s);
-  // alpha0 = X^T X r
-  GPK_TRY(launch_trmv<TS>(X, np, np, r, u, false, part, s, bt));
-  GPK_TRY(launch_trmv<TS>(X, np, np, u, alpha, true, part, s, bt));
+  // alpha0 = X^T X r; the probes ride along: w_p = X^T X z_p
+  const int nv = probes ? 1 + NPROBE : 1;
+  GPK_TRY(launch_trmv<TS>(X, np, np, r, u, false, part, s, bt, nv));
+  GPK_TRY(launch_trmv<TS>(X, np, np, u, alpha, true, part, s, bt, nv));
+  // Refinement: alpha += X^T X (r - Sigma alpha) until ||r - Sigma alpha|| <=
+  // 1e-12 ||r|| or `iters` steps; the first Sigma product also forms Sigma w_p.
   for (int it = 0; it < iters; ++it) {
-    GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, alpha, v, part, s, bt));
-    hipLaunchKernelGGL(sub_kernel, gv, dim3(256), 0, s, r, v, np, res, bt);
+    GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, alpha, v, part, s,
+                                it == 0 ? bt : bts, it == 0 ? nv : 1));
+    hipLaunchKernelGGL(sub_kernel, gv, dim3(256), 0, s, r, v, np, res, bts);
     GPK_LAUNCH_CHECK();
-    GPK_TRY(launch_trmv<TS>(X, np, np, res, u, false, part, s, bt));
-    GPK_TRY(launch_trmv<TS>(X, np, np, u, delta, true, part, s, bt));
-    hipLaunchKernelGGL(add_kernel, gv, dim3(256), 0, s, alpha, delta, np, bt);
+    if (it > 0) {
+      hipLaunchKernelGGL(refine_check_kernel, dim3(1, 1, bt.count), dim3(256), 0, s, r, res, np,
+                         skip, bts);
+      GPK_LAUNCH_CHECK();
+    }
+    GPK_TRY(launch_trmv<TS>(X, np, np, res, u, false, part, s, bts));
+    GPK_TRY(launch_trmv<TS>(X, np, np, u, delta, true, part, s, bts));
+    hipLaunchKernelGGL(add_kernel, gv, dim3(256), 0, s, alpha, delta, np, bts);
     GPK_LAUNCH_CHECK();
   }
+  if (probes && iters == 0) {
+    GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, alpha, v, part, s, bt,
+                                nv));
+  }
+  if (probes) {
+    // a-posteriori check of X ~ L^-1: e_p = z_p - Sigma X^T X z_p
+    double* stats = at<double>(ws, Lo.off_probe);
+    for (int p = 0; p < NPROBE; ++p) {
+      hipLaunchKernelGGL(probe_stats_kernel, dim3(1), dim3(256), 0, s, r + (p + 1) * np,
+                         v + (p + 1) * np, np, p, stats);
+      GPK_LAUNCH_CHECK();
+    }
+  }
+  // the final residual (also for a refinement that stopped early: alpha no longer changed)
   GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, alpha, v, part, s, bt));
   hipLaunchKernelGGL(sub_kernel, gv, dim3(256), 0, s, r, v, np, res, bt);
   GPK_LAUNCH_CHECK();
@@ -461,20 +549,9 @@ int mll_forward_t(int kind, const Arith3& ar, const double* x, const double* y, 
                      at<double>(ws, Lo.off_leafld), at<int>(ws, Lo.off_leafinfo), np / NB, r,
                      alpha, res, n, np, result, bt);
   GPK_LAUNCH_CHECK();
-  if (!f64 && !bt.gp) {
-    // a-posteriori check of X = L^-1: probes through X, X^T and Sigma in float64
-    // (u, v, res and delta are free once the objective is final)
-    double* stats = at<double>(ws, Lo.off_probe);
-    for (int p = 0; p < NPROBE; ++p) {
-      hipLaunchKernelGGL(probe_fill_kernel, gv, dim3(256), 0, s, delta, n, np, p);
-      GPK_LAUNCH_CHECK();
-      GPK_TRY(launch_trmv<TS>(X, np, np, delta, u, false, part, s));
-      GPK_TRY(launch_trmv<TS>(X, np, np, u, v, true, part, s));
-      GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, v, res, part, s));
-      hipLaunchKernelGGL(probe_stats_kernel, dim3(1), dim3(256), 0, s, delta, res, np, p, stats);
-      GPK_LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(probe_finish_kernel, dim3(1), dim3(64), 0, s, stats, n, result);
+  if (probes) {
+    hipLaunchKernelGGL(probe_finish_kernel, dim3(1), dim3(64), 0, s,
+                       at<double>(ws, Lo.off_probe), n, result);
     GPK_LAUNCH_CHECK();
   }
   return GPK_OK;
@@ -563,9 +640,10 @@ template <typename TS>
 int predict_t(int kind, const Arith3& ar, const double* x, const double* y, int64_t n, int d,
               const double* xt, int64_t t, const double* theta, int nls, double jitter_post,
               double jitter_prior, int full_cov, int iters, void* ws, const Layout& Lo,
-              const PredLayout& P, double* mean, double* cov, hipStream_t s) {
+              const PredLayout& P, double* mean, double* cov, double* result_out,
+              hipStream_t s) {
   const int64_t np = Lo.np, tp = P.tp;
-  double* result = at<double>(ws, P.off_res);
+  double* result = result_out ? result_out : at<double>(ws, P.off_res);
   // L, X = L^-1 and alpha = Sigma^-1 (y - m), Sigma with the posterior's jitter (gps.py:506-511)
   GPK_TRY(mll_forward_t<TS>(kind, ar, x, y, n, d, theta, nls, jitter_post, iters, ws, Lo,
                             result, s));
@@ -969,7 +1047,7 @@ int gpk_mll_forward_noalloc(int kind, int prec, const double* x, const double* y
   }
   void* ws = nullptr;
   if ((rc = device_scratch(&ws))) return rc;
-  const int iters = refine_iters >= 0 ? refine_iters : (base == GPK_PREC_FP64 ? 1 : 4);
+  const int iters = refine_iters >= 0 ? refine_iters : (base == GPK_PREC_FP64 ? 1 : 6);
   hipStream_t s = (hipStream_t)stream;
   if (base == GPK_PREC_FP64) {
     GPK_TRY(mll_forward_t<double>(kind, ar, x, y, n, (int)d, theta, nls, jitter, iters, ws, Lo,
@@ -1034,7 +1112,7 @@ int gpk_mll_forward(int kind, int prec, const double* x, const double* y, int64_
   const int base = decode_prec(prec, &ar);
   const Layout Lo = make_layout(n, d, base);
   if ((rc = check_ws("mll_forward", ws, ws_bytes, Lo.total))) return rc;
-  const int iters = refine_iters >= 0 ? refine_iters : (base == GPK_PREC_FP64 ? 1 : 4);
+  const int iters = refine_iters >= 0 ? refine_iters : (base == GPK_PREC_FP64 ? 1 : 6);
   hipStream_t s = (hipStream_t)stream;
   if (base == GPK_PREC_FP64)
     return mll_forward_t<double>(kind, ar, x, y, n, (int)d, theta, nls, jitter, iters, ws, Lo,
@@ -1072,7 +1150,7 @@ size_t gpk_predict_workspace_bytes(int64_t n, int64_t d, int64_t t, int prec, in
 int gpk_predict(int kind, int prec, const double* x, const double* y, int64_t n, int64_t d,
                 const double* xt, int64_t t, const double* theta, int nls, double jitter_post,
                 double jitter_prior, int full_cov, int refine_iters, void* ws, size_t ws_bytes,
-                double* mean, double* cov, void* stream) {
+                double* mean, double* cov, double* result, void* stream) {
   int rc = check_common(kind, prec, x, n, d, theta, nls);
   if (rc) return rc;
   if (!y || !xt || t < 1 || !ws || !mean || !cov) {
@@ -1084,13 +1162,13 @@ int gpk_predict(int kind, int prec, const double* x, const double* y, int64_t n,
   const Layout Lo = make_layout(n, d, base);
   const PredLayout P = make_pred_layout(Lo, t, full_cov);
   if ((rc = check_ws("predict", ws, ws_bytes, P.total))) return rc;
-  const int iters = refine_iters >= 0 ? refine_iters : (base == GPK_PREC_FP64 ? 1 : 4);
+  const int iters = refine_iters >= 0 ? refine_iters : (base == GPK_PREC_FP64 ? 1 : 6);
   hipStream_t s = (hipStream_t)stream;
   if (base == GPK_PREC_FP64)
     return predict_t<double>(kind, ar, x, y, n, (int)d, xt, t, theta, nls, jitter_post,
-                             jitter_prior, full_cov, iters, ws, Lo, P, mean, cov, s);
+                             jitter_prior, full_cov, iters, ws, Lo, P, mean, cov, result, s);
   return predict_t<float>(kind, ar, x, y, n, (int)d, xt, t, theta, nls, jitter_post,
-                          jitter_prior, full_cov, iters, ws, Lo, P, mean, cov, s);
+                          jitter_prior, full_cov, iters, ws, Lo, P, mean, cov, result, s);
 }
 
 size_t gpk_loocv_workspace_bytes(int64_t n, int64_t d, int prec) {
@@ -1113,7 +1191,7 @@ int gpk_loocv_forward(int kind, int prec, const double* x, const double* y, int6
   const Layout Lo = make_layout(n, d, base);
   const LooLayout P = make_loo_layout(Lo);
   if ((rc = check_ws("loocv_forward", ws, ws_bytes, P.total))) return rc;
-  const int iters = refine_iters >= 0 ? refine_iters : (base == GPK_PREC_FP64 ? 1 : 4);
+  const int iters = refine_iters >= 0 ? refine_iters : (base == GPK_PREC_FP64 ? 1 : 6);
   hipStream_t s = (hipStream_t)stream;
   if (base == GPK_PREC_FP64)
     return loocv_forward_t<double>(kind, ar, x, y, n, (int)d, theta, nls, jitter, iters, ws, Lo,

@@ -1,6 +1,7 @@
 """Build libgpk.so (gfx950) in-tree with hipcc.
 
-Usage: python gpjax_amd/csrc/build.py [--jobs N] [--verbose]
+Usage: python gpjax_amd/csrc/build.py [--jobs N] [--verbose] [--define NAME=VALUE]
+       [--out PATH]
 
 Each .hip source compiles to an object under gpjax_amd/csrc/_build/ and the
 objects link into gpjax_amd/libgpk.so.  Every invocation recompiles (sources
@@ -23,7 +24,7 @@ PKG = CSRC.parent
 OUT = PKG / "libgpk.so"
 BUILD = CSRC / "_build"
 SOURCES = ["api.hip", "gram.hip", "leaf.hip", "gemm.hip", "adjoint.hip", "vec.hip", "elbo.hip",
-           "fit.hip"]
+           "fit.hip", "linalg.hip"]
 ARCH = "gfx950"
 # Packed fp32 ops are disabled for every device pass: some v_pk_*_f32 forms have
 # produced silent wrong results on these parts.
@@ -41,10 +42,10 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: libgpk needs the ROCm toolchain to build")
 
 
-def _compile(src: str, verbose: bool) -> Path:
-    obj = BUILD / (Path(src).stem + ".o")
+def _compile(src: str, verbose: bool, defines=(), build_dir: Path = BUILD) -> Path:
+    obj = build_dir / (Path(src).stem + ".o")
     tmp = obj.with_suffix(".o.tmp")
-    cmd = [hipcc(), *FLAGS, "-c", str(CSRC / src), "-o", str(tmp)]
+    cmd = [hipcc(), *FLAGS, *[f"-D{d}" for d in defines], "-c", str(CSRC / src), "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)
@@ -58,24 +59,30 @@ def _compile(src: str, verbose: bool) -> Path:
     return obj
 
 
-def build(jobs: int = 6, verbose: bool = False) -> Path:
-    BUILD.mkdir(exist_ok=True)
+def build(jobs: int = 6, verbose: bool = False, defines=(), out: Path = OUT) -> Path:
+    """Compile every source and link ``out``; ``defines`` (NAME=VALUE strings)
+    select compile-time variants of the kernels for A/B builds."""
+    build_dir = BUILD if not defines else BUILD / ("_".join(defines).replace("=", "-"))
+    build_dir.mkdir(parents=True, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
-    tmp = OUT.with_name(OUT.name + ".tmp")
+        objs = list(ex.map(lambda s: _compile(s, verbose, defines, build_dir), SOURCES))
+    out = Path(out)
+    tmp = out.with_name(out.name + ".tmp")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         raise RuntimeError(f"link failed:\n{proc.stderr}")
-    tmp.replace(OUT)
-    return OUT
+    tmp.replace(out)
+    return out
 
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=6)
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--define", action="append", default=[], metavar="NAME=VALUE")
+    ap.add_argument("--out", default=str(OUT))
     a = ap.parse_args()
-    print(build(a.jobs, a.verbose))
+    print(build(a.jobs, a.verbose, tuple(a.define), Path(a.out)))

@@ -117,7 +117,19 @@ constexpr int F64_BK = 16;
 constexpr int F64_LDP = TILE + 1;  // padded row length (doubles)
 constexpr size_t F64_SMEM = 2 /*A,B*/ * 2 /*buffers*/ * F64_BK * F64_LDP * sizeof(double);
 constexpr int BF_BK = 32;
-constexpr int BF_LDK = BF_BK + 16;  // padded k length (bf16): 96-byte rows measured fastest
+#ifndef GPK_BF_SWIZZLE
+#define GPK_BF_SWIZZLE 0
+#endif
+// LDS rows of the bf16 planes.  Padded: 96-byte rows.  Swizzled: unpadded
+// 64-byte rows whose four 16-byte chunks are permuted by chunk ^ ((row >> 2) & 3),
+// so that sixteen lanes reading (or writing) 16 bytes each -- sixteen rows of
+// one chunk for the fragment reads, four rows of four chunks for the staging
+// stores -- touch sixteen different 16-byte bank groups.
+constexpr bool BF_SWZ = GPK_BF_SWIZZLE != 0;
+constexpr int BF_LDK = BF_SWZ ? BF_BK : BF_BK + 16;  // row length (bf16 elements)
+__device__ __forceinline__ int bf_off(int row, int chunk) {
+  return row * BF_LDK + 8 * (BF_SWZ ? (chunk ^ ((row >> 2) & 3)) : chunk);
+}
 constexpr int BF_CHUNK = 4;        // slabs per fp32 chunk: 128 k
 template <int AR>
 constexpr int bf_planes() { return AR == AR_BF16X6 ? 3 : 2; }
@@ -309,7 +321,7 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
     else split_pack8<NP>(r, pk);
 #pragma unroll
     for (int pl = 0; pl < NP; ++pl)
-      *reinterpret_cast<uint4*>(base + (pl * TILE + row) * BF_LDK + ko) = pk[pl];
+      *reinterpret_cast<uint4*>(base + pl * TILE * BF_LDK + bf_off(row, ko / 8)) = pk[pl];
   };
   // Slab s sits in register set s & 1 (DEEP) and is stored into LDS buffer s & 1.
   auto sstore = [&](int buf) {
@@ -339,7 +351,7 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
       uint4 a0[4], b0[NJ], a1[4], b1[NJ];
       auto frag = [&](const unsigned short* P, int pl, int base, int i) {
         return *reinterpret_cast<const uint4*>(
-            P + (pl * TILE + base + i * 16 + (lane & 15)) * BF_LDK + 8 * (lane >> 4));
+            P + pl * TILE * BF_LDK + bf_off(base + i * 16 + (lane & 15), lane >> 4));
       };
 #pragma unroll
       for (int i = 0; i < 4; ++i) a0[i] = frag(As, 0, wm * 64, i);

@@ -58,7 +58,22 @@ struct Batch {
   const int64_t* xoff = nullptr;
   int64_t theta_stride = 0;
   unsigned count = 1;
+  // Optional device flag (member z's at skip + z * ws_stride bytes): the
+  // vector kernels of an iterative refinement step return at once when it is
+  // set, so a converged refinement skips its remaining steps without a host
+  // sync.
+  const int* skip = nullptr;
 };
+
+// True when this member's refinement has converged (see Batch::skip).
+__device__ __forceinline__ bool bskip(const Batch& b) {
+  if (!b.skip) return false;
+  const int* p = b.skip;
+  if (b.ws_stride)
+    p = reinterpret_cast<const int*>(reinterpret_cast<const char*>(p) +
+                                     (int64_t)blockIdx.z * b.ws_stride);
+  return *p != 0;
+}
 
 // The shift is byte arithmetic on the pointer itself (not through an integer),
 // so the compiler keeps the global address space and emits global (not flat)
@@ -201,15 +216,18 @@ template <typename TS>
 int launch_lower_to_upper(const TS* X, TS* U, int64_t ld, int64_t np, hipStream_t s);
 
 // y = X r (trans = false) or y = X^T r (trans = true), X lower triangular
-// np x np (only k <= i is read); float64 accumulation through `partial`.
+// np x np (only k <= i is read); float64 accumulation through `partial`
+// (nv x ceil(np / 1024) x np doubles).  nv = 1 or 3 right-hand sides, stored
+// np apart in r and in y, share each pass over X.
 template <typename TS>
 int launch_trmv(const TS* X, int64_t ld, int64_t np, const double* r, double* y, bool trans,
-                double* partial, hipStream_t s, const Batch& bt = Batch());
+                double* partial, hipStream_t s, const Batch& bt = Batch(), int nv = 1);
 
-// v = Sigma a, Sigma recomputed from x in float64 (identity pad for i >= n).
+// v = Sigma a, Sigma recomputed from x in float64 (identity pad for i >= n);
+// nv = 1 or 3 vectors np apart share each kernel evaluation.
 int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
                         const double* theta, int nls, double jitter, const double* a, double* v,
-                        double* partial, hipStream_t s, const Batch& bt = Batch());
+                        double* partial, hipStream_t s, const Batch& bt = Batch(), int nv = 1);
 
 // y_j = sum_i M[i][j] v_i (v non-null) or sum_i M[i][j]^2 (v null), dense
 // rows x cols (float64 or float32 entries, float64 sums); partial holds

@@ -14,127 +14,171 @@ constexpr int TRMV_RC = 1024;  // rows per chunk, transposed
 // y_i partial over columns [c*CW, (c+1)*CW), k <= i.  One wave per row, 16 rows
 // per workgroup; each lane reads 16 bytes per load (four float32 or two
 // float64 entries, 1 KB per wave), the row's tail up to the diagonal entry by
-// entry.
-template <typename TS>
+// entry.  NV right-hand sides (stride np) share every load of X.
+template <typename TS, int NV>
 __global__ __launch_bounds__(256) void trmv_n_kernel(const TS* __restrict__ X, int64_t ld,
                                                      int64_t np, const double* __restrict__ r,
                                                      double* __restrict__ partial, Batch bt) {
+  if (bskip(bt)) return;
   X = bws(X, bt);
   r = bws(r, bt);
   partial = bws(partial, bt);
   const int64_t c = blockIdx.y;
+  const int64_t nc = gridDim.y;
   const int64_t row0 = (int64_t)blockIdx.x * 16;
   const int64_t k0 = c * TRMV_CW;
   if (k0 > row0 + 15) return;
-  __shared__ double rs[TRMV_CW];
-  for (int e = threadIdx.x; e < TRMV_CW; e += 256) rs[e] = (k0 + e < np) ? r[k0 + e] : 0.0;
+  __shared__ double rs[NV][TRMV_CW];
+  for (int e = threadIdx.x; e < TRMV_CW; e += 256)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) rs[q][e] = (k0 + e < np) ? r[q * np + k0 + e] : 0.0;
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   constexpr int V = 16 / sizeof(TS);  // entries per 16-byte load
   typedef typename std::conditional<sizeof(TS) == 4, float4, double2>::type Vec;
-  for (int q = 0; q < 4; ++q) {
-    const int64_t i = row0 + w * 4 + q;
+  for (int qr = 0; qr < 4; ++qr) {
+    const int64_t i = row0 + w * 4 + qr;
     const int64_t kend = (i + 1 < k0 + TRMV_CW) ? i + 1 : k0 + TRMV_CW;
-    double acc = 0.0;
+    double acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
     const TS* xr = X + i * ld;
     for (int64_t kb = k0 + V * lane; kb < kend; kb += 64 * V) {
-      const double* rr = rs + (kb - k0);
+      const int64_t o = kb - k0;
       if (kb + V <= kend) {
         const Vec v = *reinterpret_cast<const Vec*>(xr + kb);
         const TS* e = reinterpret_cast<const TS*>(&v);
 #pragma unroll
-        for (int u = 0; u < V; ++u) acc += (double)e[u] * rr[u];
+        for (int u = 0; u < V; ++u)
+#pragma unroll
+          for (int q = 0; q < NV; ++q) acc[q] += (double)e[u] * rs[q][o + u];
       } else {
-        for (int u = 0; kb + u < kend; ++u) acc += (double)xr[kb + u] * rr[u];
+        for (int u = 0; kb + u < kend; ++u)
+#pragma unroll
+          for (int q = 0; q < NV; ++q) acc[q] += (double)xr[kb + u] * rs[q][o + u];
       }
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    if (lane == 0) partial[c * np + i] = acc;
+    for (int q = 0; q < NV; ++q) {
+      double a = acc[q];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+      if (lane == 0) partial[(q * nc + c) * np + i] = a;
+    }
   }
 }
 
-__global__ void trmv_n_reduce(const double* __restrict__ partial, int64_t np,
-                              double* __restrict__ y, Batch bt) {
+__global__ void trmv_n_reduce(const double* __restrict__ partial, int64_t np, int64_t nc,
+                              int nv, double* __restrict__ y, Batch bt) {
+  if (bskip(bt)) return;
   partial = bws(partial, bt);
   y = bws(y, bt);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  const int64_t nc = i / TRMV_CW;
-  double s = 0.0;
-  for (int64_t c = 0; c <= nc; ++c) s += partial[c * np + i];
-  y[i] = s;
+  const int64_t last = i / TRMV_CW;
+  for (int q = 0; q < nv; ++q) {
+    double s = 0.0;
+    for (int64_t c = 0; c <= last; ++c) s += partial[(q * nc + c) * np + i];
+    y[q * np + i] = s;
+  }
 }
 
 // y_k partial over rows [c*RC, (c+1)*RC), i >= k.  64 columns per workgroup;
 // its four waves take every fourth row (each wave reads 64 consecutive
 // columns of a row, 512 or 256 bytes), so four times as many row loads are in
-// flight per column, and the four row sums are added in a fixed order.
+// flight per column, and the four row sums are added in a fixed order.  NV
+// right-hand sides (stride np) share every load of X.
 constexpr int TRMV_TC = 64;  // columns per workgroup, transposed
-template <typename TS>
+template <typename TS, int NV>
 __global__ __launch_bounds__(256) void trmv_t_kernel(const TS* __restrict__ X, int64_t ld,
                                                      int64_t np, const double* __restrict__ r,
                                                      double* __restrict__ partial, Batch bt) {
+  if (bskip(bt)) return;
   X = bws(X, bt);
   r = bws(r, bt);
   partial = bws(partial, bt);
   const int64_t c = blockIdx.y;
+  const int64_t nc = gridDim.y;
   const int64_t col0 = (int64_t)blockIdx.x * TRMV_TC;
   const int64_t i0 = c * TRMV_RC;
   const int64_t i1 = (i0 + TRMV_RC < np) ? i0 + TRMV_RC : np;
   if (i1 <= col0) return;
-  __shared__ double rs[TRMV_RC];
-  __shared__ double red[4][TRMV_TC];
-  for (int e = threadIdx.x; e < TRMV_RC; e += 256) rs[e] = (i0 + e < np) ? r[i0 + e] : 0.0;
+  __shared__ double rs[NV][TRMV_RC];
+  __shared__ double red[NV][4][TRMV_TC];
+  for (int e = threadIdx.x; e < TRMV_RC; e += 256)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) rs[q][e] = (i0 + e < np) ? r[q * np + i0 + e] : 0.0;
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t k = col0 + lane;
   const int64_t ib = (k > i0) ? k : i0;
-  // Four independent sums per thread (rows i, i+4, i+8, i+12) keep four loads
-  // in flight; they are combined in a fixed order.
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  // Four independent sums per thread and vector (rows i, i+4, i+8, i+12) keep
+  // four loads in flight; they are combined in a fixed order.
+  double a0[NV], a1[NV], a2[NV], a3[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) a0[q] = a1[q] = a2[q] = a3[q] = 0.0;
   int64_t i = ib + w;
   for (; i + 12 < i1; i += 16) {
-    a0 += (double)X[i * ld + k] * rs[i - i0];
-    a1 += (double)X[(i + 4) * ld + k] * rs[i + 4 - i0];
-    a2 += (double)X[(i + 8) * ld + k] * rs[i + 8 - i0];
-    a3 += (double)X[(i + 12) * ld + k] * rs[i + 12 - i0];
+    const double x0 = (double)X[i * ld + k], x1 = (double)X[(i + 4) * ld + k];
+    const double x2 = (double)X[(i + 8) * ld + k], x3 = (double)X[(i + 12) * ld + k];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      a0[q] += x0 * rs[q][i - i0];
+      a1[q] += x1 * rs[q][i + 4 - i0];
+      a2[q] += x2 * rs[q][i + 8 - i0];
+      a3[q] += x3 * rs[q][i + 12 - i0];
+    }
   }
-  for (; i < i1; i += 4) a0 += (double)X[i * ld + k] * rs[i - i0];
-  red[w][lane] = (a0 + a1) + (a2 + a3);
+  for (; i < i1; i += 4) {
+    const double x0 = (double)X[i * ld + k];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) a0[q] += x0 * rs[q][i - i0];
+  }
+#pragma unroll
+  for (int q = 0; q < NV; ++q) red[q][w][lane] = (a0[q] + a1[q]) + (a2[q] + a3[q]);
   __syncthreads();
-  if (w == 0) partial[c * np + k] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+  if (w == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+      partial[(q * nc + c) * np + k] =
+          ((red[q][0][lane] + red[q][1][lane]) + red[q][2][lane]) + red[q][3][lane];
+  }
 }
 
 __global__ void trmv_t_reduce(const double* __restrict__ partial, int64_t np, int64_t nchunks,
-                              double* __restrict__ y, Batch bt) {
+                              int nv, double* __restrict__ y, Batch bt) {
+  if (bskip(bt)) return;
   partial = bws(partial, bt);
   y = bws(y, bt);
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= np) return;
-  double s = 0.0;
-  for (int64_t c = k / TRMV_RC; c < nchunks; ++c) s += partial[c * np + k];
-  y[k] = s;
+  for (int q = 0; q < nv; ++q) {
+    double s = 0.0;
+    for (int64_t c = k / TRMV_RC; c < nchunks; ++c) s += partial[(q * nchunks + c) * np + k];
+    y[q * np + k] = s;
+  }
 }
 
 // v_i partial = sum_{j in chunk} Sigma_ij a_j, Sigma recomputed in float64.
 constexpr int SMV_CW = 4096;  // columns per chunk
 constexpr int SMV_JT = 256;   // columns staged per LDS pass
 
-template <int DMAX>
+template <int DMAX, int NV>
 __global__ __launch_bounds__(256) void sigma_mv_kernel(int kind, const double* __restrict__ x,
                                                        int64_t n, int64_t np, int d,
                                                        const double* __restrict__ theta,
                                                        int nls, double jitter,
                                                        const double* __restrict__ a,
                                                        double* __restrict__ partial, Batch bt) {
+  if (bskip(bt)) return;
   __shared__ __align__(16) double xj_s[SMV_JT][DMAX];  // unpadded: 16-byte broadcast reads
-  __shared__ double aj_s[SMV_JT];
+  __shared__ double aj_s[NV][SMV_JT];
   bmember(bt, d, x, n, theta);
   a = bws(a, bt);
   partial = bws(partial, bt);
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t c = blockIdx.y;
+  const int64_t nc = gridDim.y;
   const int64_t j0c = c * SMV_CW;
   const double var = theta[nls];
   const double sd = theta[nls + 1];
@@ -142,7 +186,9 @@ __global__ __launch_bounds__(256) void sigma_mv_kernel(int kind, const double* _
 #pragma unroll
   for (int q = 0; q < DMAX; ++q)
     xi[q] = (q < d && i < n) ? x[i * d + q] / theta[nls == 1 ? 0 : q] : 0.0;
-  double acc = 0.0;
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
   for (int64_t j0 = j0c; j0 < j0c + SMV_CW && j0 < np; j0 += SMV_JT) {
     __syncthreads();
     for (int e = threadIdx.x; e < SMV_JT * DMAX; e += 256) {
@@ -150,7 +196,9 @@ __global__ __launch_bounds__(256) void sigma_mv_kernel(int kind, const double* _
       const int64_t gj = j0 + r;
       xj_s[r][q] = (q < d && gj < n) ? x[gj * d + q] / theta[nls == 1 ? 0 : q] : 0.0;
     }
-    aj_s[threadIdx.x] = (j0 + threadIdx.x < np) ? a[j0 + threadIdx.x] : 0.0;
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+      aj_s[q][threadIdx.x] = (j0 + threadIdx.x < np) ? a[q * np + j0 + threadIdx.x] : 0.0;
     __syncthreads();
     if (i < n) {
       const int64_t jend = (j0 + SMV_JT < n) ? SMV_JT : n - j0;
@@ -161,29 +209,36 @@ __global__ __launch_bounds__(256) void sigma_mv_kernel(int kind, const double* _
           const double df = xi[q] - xj_s[jj][q];
           d2 += df * df;
         }
-        acc += var * kval(kind, d2) * aj_s[jj];
+        const double kv = var * kval(kind, d2);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) acc[q] += kv * aj_s[q][jj];
       }
     }
   }
   if (i < np) {
     // diagonal additions (jitter + obs_sd^2) and the identity pad
-    if (i >= j0c && i < j0c + SMV_CW) {
-      if (i < n) acc += (jitter + sd * sd) * a[i];
-      else acc += a[i];
+    const bool mine = i >= j0c && i < j0c + SMV_CW;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double v = acc[q];
+      if (mine) v += (i < n ? jitter + sd * sd : 1.0) * a[q * np + i];
+      partial[(q * nc + c) * np + i] = v;
     }
-    partial[c * np + i] = acc;
   }
 }
 
 __global__ void sum_chunks(const double* __restrict__ partial, int64_t np, int64_t nchunks,
-                           double* __restrict__ y, Batch bt) {
+                           int nv, double* __restrict__ y, Batch bt) {
+  if (bskip(bt)) return;
   partial = bws(partial, bt);
   y = bws(y, bt);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  double s = 0.0;
-  for (int64_t c = 0; c < nchunks; ++c) s += partial[c * np + i];
-  y[i] = s;
+  for (int q = 0; q < nv; ++q) {
+    double s = 0.0;
+    for (int64_t c = 0; c < nchunks; ++c) s += partial[(q * nchunks + c) * np + i];
+    y[q * np + i] = s;
+  }
 }
 
 // y_j partial over rows [c*RC, (c+1)*RC) of a dense rows x cols matrix:
@@ -245,46 +300,66 @@ __global__ void mfma_selftest_kernel(double* out64, float* outbf) {
 
 template <typename TS>
 int launch_trmv(const TS* X, int64_t ld, int64_t np, const double* r, double* y, bool trans,
-                double* partial, hipStream_t s, const Batch& bt) {
-  if (np % 256) {
-    set_error("trmv: size %lld not a multiple of 256", (long long)np);
+                double* partial, hipStream_t s, const Batch& bt, int nv) {
+  if (np % NB) {
+    set_error("trmv: size %lld not a multiple of %d", (long long)np, NB);
+    return GPK_ERR_BAD_ARG;
+  }
+  if (nv != 1 && nv != 3) {
+    set_error("trmv: %d right-hand sides (1 or 3 supported)", nv);
     return GPK_ERR_BAD_ARG;
   }
   const dim3 gvec((unsigned)((np + 255) / 256), 1, bt.count);
   if (!trans) {
     const int64_t nc = (np + TRMV_CW - 1) / TRMV_CW;
-    hipLaunchKernelGGL(trmv_n_kernel<TS>, dim3((unsigned)(np / 16), (unsigned)nc, bt.count),
-                       dim3(256), 0, s, X, ld, np, r, partial, bt);
+    const dim3 grid((unsigned)(np / 16), (unsigned)nc, bt.count);
+    if (nv == 1)
+      hipLaunchKernelGGL((trmv_n_kernel<TS, 1>), grid, dim3(256), 0, s, X, ld, np, r, partial, bt);
+    else
+      hipLaunchKernelGGL((trmv_n_kernel<TS, 3>), grid, dim3(256), 0, s, X, ld, np, r, partial, bt);
     GPK_LAUNCH_CHECK();
-    hipLaunchKernelGGL(trmv_n_reduce, gvec, dim3(256), 0, s, partial, np, y, bt);
+    hipLaunchKernelGGL(trmv_n_reduce, gvec, dim3(256), 0, s, partial, np, nc, nv, y, bt);
   } else {
     const int64_t nc = (np + TRMV_RC - 1) / TRMV_RC;
-    hipLaunchKernelGGL(trmv_t_kernel<TS>, dim3((unsigned)(np / TRMV_TC), (unsigned)nc, bt.count),
-                       dim3(256), 0, s, X, ld, np, r, partial, bt);
+    const dim3 grid((unsigned)(np / TRMV_TC), (unsigned)nc, bt.count);
+    if (nv == 1)
+      hipLaunchKernelGGL((trmv_t_kernel<TS, 1>), grid, dim3(256), 0, s, X, ld, np, r, partial, bt);
+    else
+      hipLaunchKernelGGL((trmv_t_kernel<TS, 3>), grid, dim3(256), 0, s, X, ld, np, r, partial, bt);
     GPK_LAUNCH_CHECK();
-    hipLaunchKernelGGL(trmv_t_reduce, gvec, dim3(256), 0, s, partial, np, nc, y, bt);
+    hipLaunchKernelGGL(trmv_t_reduce, gvec, dim3(256), 0, s, partial, np, nc, nv, y, bt);
   }
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }
 
 template int launch_trmv<double>(const double*, int64_t, int64_t, const double*, double*, bool,
-                                 double*, hipStream_t, const Batch&);
+                                 double*, hipStream_t, const Batch&, int);
 template int launch_trmv<float>(const float*, int64_t, int64_t, const double*, double*, bool,
-                                double*, hipStream_t, const Batch&);
+                                double*, hipStream_t, const Batch&, int);
 
 int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
                         const double* theta, int nls, double jitter, const double* a, double* v,
-                        double* partial, hipStream_t s, const Batch& bt) {
+                        double* partial, hipStream_t s, const Batch& bt, int nv) {
   if (d < 1 || d > MAX_D) {
     set_error("sigma matvec: input dimension %d outside [1, %d]", d, MAX_D);
     return GPK_ERR_BAD_ARG;
   }
+  if (nv != 1 && nv != 3) {
+    set_error("sigma matvec: %d right-hand sides (1 or 3 supported)", nv);
+    return GPK_ERR_BAD_ARG;
+  }
   const int64_t nc = (np + SMV_CW - 1) / SMV_CW;
   dim3 grid((unsigned)((np + 255) / 256), (unsigned)nc, bt.count);
-#define GPK_SMV(DM)                                                                          \
-  hipLaunchKernelGGL(sigma_mv_kernel<DM>, grid, dim3(256), 0, s, kind, x, n, np, d, theta, nls, \
-                     jitter, a, partial, bt)
+#define GPK_SMV(DM)                                                                           \
+  do {                                                                                        \
+    if (nv == 1)                                                                              \
+      hipLaunchKernelGGL((sigma_mv_kernel<DM, 1>), grid, dim3(256), 0, s, kind, x, n, np, d,  \
+                         theta, nls, jitter, a, partial, bt);                                 \
+    else                                                                                      \
+      hipLaunchKernelGGL((sigma_mv_kernel<DM, 3>), grid, dim3(256), 0, s, kind, x, n, np, d,  \
+                         theta, nls, jitter, a, partial, bt);                                 \
+  } while (0)
   if (d <= 1) GPK_SMV(1);
   else if (d <= 2) GPK_SMV(2);
   else if (d <= 4) GPK_SMV(4);
@@ -294,7 +369,7 @@ int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
 #undef GPK_SMV
   GPK_LAUNCH_CHECK();
   hipLaunchKernelGGL(sum_chunks, dim3((unsigned)((np + 255) / 256), 1, bt.count), dim3(256), 0, s,
-                     partial, np, nc, v, bt);
+                     partial, np, nc, nv, v, bt);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }

@@ -4,7 +4,8 @@ Holds a mean vector and a covariance (torch float64): either the dense [T, T]
 matrix or, for a diagonal covariance (``predict(..., full_covariance=False)``),
 the [T] vector of variances, so that a predictive at large T needs O(T) memory.
 ``variance`` and ``stddev`` read the diagonal; ``covariance()`` forms the dense
-matrix (for a diagonal one, only when asked).
+matrix (for a diagonal one, only when asked); ``log_prob`` and ``entropy`` run
+on the device Cholesky (``gpjax_amd.linalg``).
 """
 
 from __future__ import annotations
@@ -44,3 +45,15 @@ class GaussianDistribution:
     @property
     def event_shape(self):
         return tuple(self.loc.shape)
+
+    def log_prob(self, y: torch.Tensor) -> torch.Tensor:
+        """log N(y; loc, scale) = -1/2 [n log 2 pi + log|scale| + (y - loc)^T scale^-1
+        (y - loc)] (``gpjax/distributions.py:115-134``), through the device Cholesky
+        (``gpk_mvn_log_prob``); a diagonal scale is evaluated directly."""
+        from .linalg import mvn_log_prob
+        return mvn_log_prob(self.loc, self.scale, y)[0]
+
+    def entropy(self) -> torch.Tensor:
+        """1/2 [n (1 + log 2 pi) + log|scale|] (``gpjax/distributions.py:82-86``)."""
+        from .linalg import mvn_log_prob
+        return mvn_log_prob(self.loc, self.scale, self.loc)[3]

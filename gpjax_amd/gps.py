@@ -57,7 +57,7 @@ class ConjugatePosterior:
         recorded through predict.
         """
         from . import _lib, config
-        from .objectives import _mean_constant, _precision_id
+        from .objectives import _mean_constant, _precision_id, mixed_check_says_recompute
 
         kernel = self.prior.kernel
         X = kernel.slice_input(train_data.X)
@@ -73,16 +73,29 @@ class ConjugatePosterior:
             c = _mean_constant(self.prior.mean_function, X.device)
             theta = kernel.theta_tensor(d, self.likelihood.obs_stddev.value, c)
             full = 1 if full_covariance else 0
-            nbytes = _lib.lib().gpk_predict_workspace_bytes(n, d, t, prec, full)
-            ws = torch.empty(nbytes, dtype=torch.uint8, device=X.device)
-            mean = torch.empty(t, dtype=torch.float64, device=X.device)
-            cov = torch.empty((t, t) if full else (t,), dtype=torch.float64, device=X.device)
             iters = -1 if config.refine_iters is None else int(config.refine_iters)
-            _lib.call("gpk_predict", kernel.kind_id, prec, _lib.ptr(X), _lib.ptr(y), n, d,
-                      _lib.ptr(T), t, _lib.ptr(theta), nls, float(self.jitter),
-                      float(self.prior.jitter), full, iters, _lib.ptr(ws), nbytes,
-                      _lib.ptr(mean), _lib.ptr(cov), _lib.stream_ptr(X.device))
-        return GaussianDistribution(mean, cov)
+
+            def run(prec):
+                nbytes = _lib.lib().gpk_predict_workspace_bytes(n, d, t, prec, full)
+                ws = torch.empty(nbytes, dtype=torch.uint8, device=X.device)
+                mean = torch.empty(t, dtype=torch.float64, device=X.device)
+                cov = torch.empty((t, t) if full else (t,), dtype=torch.float64,
+                                  device=X.device)
+                res = torch.empty(_lib.RESULT_LEN, dtype=torch.float64, device=X.device)
+                _lib.call("gpk_predict", kernel.kind_id, prec, _lib.ptr(X), _lib.ptr(y), n, d,
+                          _lib.ptr(T), t, _lib.ptr(theta), nls, float(self.jitter),
+                          float(self.prior.jitter), full, iters, _lib.ptr(ws), nbytes,
+                          _lib.ptr(mean), _lib.ptr(cov), _lib.ptr(res),
+                          _lib.stream_ptr(X.device))
+                return mean, cov, res
+
+            mean, cov, res = run(prec)
+            if mixed_check_says_recompute(res, prec, n, "predict"):
+                mean, cov, res = run(_lib.PRECISIONS["fp64"])
+                res[_lib.RES_FLAG] = 2.0
+        dist = GaussianDistribution(mean, cov)
+        dist.factor_result = res
+        return dist
 
 
 def construct_posterior(prior: Prior, likelihood):

@@ -18,11 +18,43 @@ returns NaN and reports the failing pivot (``info``) instead.
 from __future__ import annotations
 
 import math
+import warnings
 
 import torch
 
 from . import _lib, config
 from .mean_functions import Constant
+
+
+class MixedPrecisionError(RuntimeError):
+    """The mixed-precision path's a-posteriori check flagged its result."""
+
+
+def _is_mixed(prec_id: int) -> bool:
+    return (prec_id & 0xFF) == _lib.PRECISIONS["mixed"]
+
+
+def mixed_check_says_recompute(res: torch.Tensor, prec_id: int, n: int, what: str) -> bool:
+    """Apply ``config.mixed_check`` to a mixed-precision result buffer: True when
+    the caller should recompute in float64.  Reads two result slots (a host sync)
+    unless the check is off, the path is float64, or a graph is being captured."""
+    if not _is_mixed(prec_id) or config.mixed_check == "off":
+        return False
+    if torch.cuda.is_current_stream_capturing():
+        return False
+    info, flag, probe = res[[_lib.RES_INFO, _lib.RES_FLAG, _lib.RES_PROBE]].tolist()
+    if info != 0.0 or flag != 1.0:
+        return False
+    msg = (f"{what}: the mixed-precision a-posteriori check flagged this problem "
+           f"(||I - Sigma X^T X||_F ~ {probe * math.sqrt(n):.3g}, n = {n}); its MLL / "
+           "gradient may be outside 1e-6 / 1e-4 of the float64 answer")
+    if config.mixed_check == "raise":
+        raise MixedPrecisionError(msg)
+    if config.mixed_check == "warn":
+        warnings.warn(msg, RuntimeWarning, stacklevel=3)
+        return False
+    warnings.warn(msg + "; recomputing in float64", RuntimeWarning, stacklevel=3)
+    return True
 
 
 def _precision_id(precision: str | None) -> int:
@@ -61,13 +93,22 @@ class _ConjugateMLL(torch.autograd.Function):
         ws_fn, fwd_fn, bwd_fn = _EXACT[which]
         theta = torch.cat([ls.reshape(-1)[:nls], var.reshape(1), sd.reshape(1),
                            c.reshape(1)]).to(dev, torch.float64).contiguous()
-        nbytes = getattr(_lib.lib(), ws_fn)(n, d, prec_id)
-        ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
-        res = torch.empty(_lib.RESULT_LEN, dtype=torch.float64, device=dev)
         it = -1 if refine is None else int(refine)
-        _lib.call(fwd_fn, kind_id, prec_id, _lib.ptr(X), _lib.ptr(y), n, d,
-                  _lib.ptr(theta), nls, float(jitter), it, _lib.ptr(ws), nbytes, _lib.ptr(res),
-                  _lib.stream_ptr(dev))
+
+        def run(prec):
+            nbytes = getattr(_lib.lib(), ws_fn)(n, d, prec)
+            ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+            res = torch.empty(_lib.RESULT_LEN, dtype=torch.float64, device=dev)
+            _lib.call(fwd_fn, kind_id, prec, _lib.ptr(X), _lib.ptr(y), n, d,
+                      _lib.ptr(theta), nls, float(jitter), it, _lib.ptr(ws), nbytes,
+                      _lib.ptr(res), _lib.stream_ptr(dev))
+            return ws, nbytes, res
+
+        ws, nbytes, res = run(prec_id)
+        if mixed_check_says_recompute(res, prec_id, n, f"conjugate_{which}"):
+            prec_id = _lib.PRECISIONS["fp64"]
+            ws, nbytes, res = run(prec_id)
+            res[_lib.RES_FLAG] = 2.0
         ctx.ws, ctx.nbytes, ctx.theta, ctx.bwd_fn = ws, nbytes, theta, bwd_fn
         ctx.X, ctx.kind_id, ctx.prec_id, ctx.nls = X, kind_id, prec_id, nls
         ctx.shapes = (ls.shape, var.shape, sd.shape, c.shape)

@@ -75,6 +75,9 @@ extern "C" {
  * Rademacher probes z:                                                        */
 #define GPK_RES_PROBE 5    /* sqrt(mean ||E z||^2 / n)  (~ ||E||_F / sqrt(n))     */
 #define GPK_RES_TRACE 6    /* mean z^T E z  (estimates tr E ~ error of log|Sigma|) */
+#define GPK_RES_FLAG 7     /* 1 when ||E||_F > 0.75: the mixed result may be      */
+                           /* outside 1e-6 (MLL) / 1e-4 (gradient); recompute in  */
+                           /* GPK_PREC_FP64                                        */
 #define GPK_RESULT_LEN 8
 
 const char* gpk_version(void);
@@ -110,7 +113,9 @@ size_t gpk_mll_workspace_bytes(int64_t n, int64_t d, int prec);
  *   mll = -1/2 [ n log 2pi + log|Sigma| + r^T Sigma^-1 r ],
  *   Sigma = K + (jitter + obs_stddev^2) I,  r = y - c.
  * Computes L = chol(Sigma) and L^-1 in the workspace, alpha = Sigma^-1 r with
- * `refine_iters` steps of float64 iterative refinement, and writes
+ * at most `refine_iters` steps of float64 iterative refinement (< 0: 1 for
+ * FP64, 6 for mixed; the steps stop once ||r - Sigma alpha|| <= 1e-12 ||r||,
+ * decided on the device), and writes
  * result[GPK_RESULT_LEN] (slots GPK_RES_*).  The workspace then holds the state
  * that gpk_mll_backward consumes.
  */
@@ -208,12 +213,14 @@ size_t gpk_predict_workspace_bytes(int64_t n, int64_t d, int64_t t, int prec, in
  *   cov  = Ktt - (L^-1 Kxt)^T (L^-1 Kxt) + jitter_prior I.
  * full_cov != 0 writes the dense t x t covariance (row-major, ld = t) to cov;
  * full_cov == 0 writes only its diagonal (t entries).  The factorisation runs
- * in `prec`; everything after it runs in float64.
+ * in `prec`; everything after it runs in float64.  result (optional, device,
+ * GPK_RESULT_LEN): the factorisation's slots as gpk_mll_forward writes them
+ * (pivot status, refinement residual, the mixed path's accuracy check).
  */
 int gpk_predict(int kind, int prec, const double* x, const double* y, int64_t n, int64_t d,
                 const double* xt, int64_t t, const double* theta, int nls, double jitter_post,
                 double jitter_prior, int full_cov, int refine_iters, void* ws, size_t ws_bytes,
-                double* mean, double* cov, void* stream);
+                double* mean, double* cov, double* result, void* stream);
 
 /* Bytes of workspace gpk_loocv_forward / gpk_loocv_backward need. */
 size_t gpk_loocv_workspace_bytes(int64_t n, int64_t d, int prec);
@@ -285,6 +292,64 @@ int gpk_batched_mll_grad(int kind, const double* x, const double* y, const int64
                          int64_t count, int64_t d, const double* theta, int nls, double jitter,
                          int refine_iters, void* ws, size_t ws_bytes, double* result,
                          double* grad, void* stream);
+
+/*
+ * Dense linear algebra behind gpjax/linalg/operations.py, on the same
+ * recursive Cholesky / triangular-inverse kernels as the objectives.  float64
+ * row-major matrices with a leading dimension; workspace from
+ * gpk_linalg_workspace_bytes(n, m, prec) (m = right-hand-side columns of
+ * gpk_trsm, else 0), 256-byte aligned.
+ */
+size_t gpk_linalg_workspace_bytes(int64_t n, int64_t m, int prec);
+
+/*
+ * L = chol(A), lower with zeros above (lower_cholesky of a Dense operator,
+ * gpjax/linalg/operations.py:54-55, i.e. jnp.linalg.cholesky with its
+ * symmetrized input (A + A^T)/2).  logdet (device double, optional) =
+ * log|A| = 2 sum log L_ii, the Dense logdet for a PD A (operations.py:163-165);
+ * info (device int, optional) = 0, or 1 + the first non-positive pivot, in
+ * which case logdet is NaN and L is NaN from that pivot on.  prec
+ * GPK_PREC_MIXED factors in float32 storage with BF16x6 products and float64
+ * diagonal blocks (L to about float32 accuracy).
+ */
+int gpk_potrf(int prec, const double* A, int64_t n, int64_t lda, double* L, int64_t ldl,
+              double* logdet, int* info, void* ws, size_t ws_bytes, void* stream);
+
+/*
+ * B <- L^-1 B for a lower-triangular L (n x n) and B (n x m): solve of a
+ * Triangular operator, jax.scipy.linalg.solve_triangular(L, b, lower=True)
+ * (gpjax/linalg/operations.py:105-107).  Computed as L^-1 (triangular inverse
+ * on the matrix cores, float64) times B.
+ */
+int gpk_trsm(const double* L, int64_t n, int64_t ldl, double* B, int64_t m, int64_t ldb,
+             void* ws, size_t ws_bytes, void* stream);
+
+/*
+ * Ainv = (L L^T)^-1 = L^-T L^-1, full symmetric (n x n): the dense inverse of
+ * a PD matrix from its factor, which GPJax forms with jnp.linalg.inv
+ * (gpjax/objectives.py:171) or solve(A, I).
+ */
+int gpk_potri(const double* L, int64_t n, int64_t ldl, double* Ainv, int64_t lda, void* ws,
+              size_t ws_bytes, void* stream);
+
+/* out (device double) = sum_i log L_ii: logdet of a Triangular operator
+ * (gpjax/linalg/operations.py:145-147). */
+int gpk_logdet_tri(const double* L, int64_t n, int64_t ldl, double* out, void* stream);
+
+/* Bytes of workspace gpk_mvn_log_prob needs (diag != 0: a diagonal covariance). */
+size_t gpk_mvn_workspace_bytes(int64_t n, int diag);
+
+/*
+ * GaussianDistribution.log_prob and entropy (gpjax/distributions.py:115-134,
+ * 82-86): out[0] = -1/2 [n log 2pi + log|S| + (y - loc)^T S^-1 (y - loc)],
+ * out[1] = log|S|, out[2] = the quadratic term, out[3] = the entropy
+ * 1/2 [n (1 + log 2pi) + log|S|].  S is dense n x n (lds) or, with diag != 0,
+ * the n variances of a diagonal covariance.  The dense path factors S by
+ * Cholesky (a NaN result if S is not PD, where the reference's LU slogdet
+ * drops the sign).
+ */
+int gpk_mvn_log_prob(const double* loc, const double* S, int64_t n, int64_t lds, int diag,
+                     const double* y, double* out, void* ws, size_t ws_bytes, void* stream);
 
 /*
  * One optimiser step of gpjax.fit (gpjax/fit.py:133-180) for `count` float64

@@ -214,6 +214,28 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
         _lib.lib()
 
 
+def test_linalg_entry_points_validate_on_the_host():
+    lib = _lib.lib()
+    d = ctypes.c_void_p(0x1000)
+    assert lib.gpk_linalg_workspace_bytes(0, 0, 0) == 0
+    assert lib.gpk_linalg_workspace_bytes(300, 0, 7) == 0
+    w64 = lib.gpk_linalg_workspace_bytes(300, 0, 0)
+    w32 = lib.gpk_linalg_workspace_bytes(300, 0, 1)
+    assert 2 * 512 * 512 * 8 <= w64 and 2 * 512 * 512 * 4 <= w32 < w64
+    # right-hand sides add two padded n x m buffers
+    assert lib.gpk_linalg_workspace_bytes(300, 5, 0) >= w64 + 2 * 512 * 128 * 8
+    assert lib.gpk_potrf(0, d, 10, 5, d, 10, None, None, d, 1 << 40, None) == 1   # lda < n
+    assert lib.gpk_potrf(3, d, 10, 10, d, 10, None, None, d, 1 << 40, None) == 1  # precision
+    assert lib.gpk_potrf(0, d, 10, 10, d, 10, None, None, d, 16, None) == 5       # workspace
+    assert lib.gpk_potrf(0, d, 10, 10, d, 10, None, None, ctypes.c_void_p(0x1008), 1 << 40,
+                         None) == 1                                               # alignment
+    assert lib.gpk_trsm(d, 10, 10, d, 0, 1, d, 1 << 40, None) == 1               # m < 1
+    assert lib.gpk_potri(d, 10, 10, None, 10, d, 1 << 40, None) == 1
+    assert lib.gpk_logdet_tri(d, 0, 1, d, None) == 1
+    assert lib.gpk_mvn_log_prob(d, d, 10, 10, 0, d, d, d, 16, None) == 5
+    assert lib.gpk_mvn_workspace_bytes(10, 1) == 256
+
+
 def test_noalloc_limits_checked_on_the_host():
     """gpk_mll_forward_noalloc refuses n above its fixed scratch (1,024 rows)
     before touching the device; the FP64 layout at its limit fits the 20 MiB."""
