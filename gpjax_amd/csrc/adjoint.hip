@@ -25,6 +25,8 @@
 namespace gpk {
 namespace {
 
+constexpr int ADJ_DCH = 32;  // input dimensions staged per pass in the epilogue
+
 template <int AR, typename TS, int DMAX, bool XT>
 __global__ __launch_bounds__(WaveLayout<AR>::NT) void adjoint_kernel(int kind,
                                                                      const TS* __restrict__ X,
@@ -43,13 +45,15 @@ __global__ __launch_bounds__(WaveLayout<AR>::NT) void adjoint_kernel(int kind,
   alpha = bws(alpha, bt);
   beta = bws(beta, bt);
   partial = bws(partial, bt);
+  // Inputs wider than ADJ_DCH are staged in chunks of ADJ_DCH dimensions.
+  constexpr int DS = DMAX > ADJ_DCH ? ADJ_DCH : DMAX;
   constexpr size_t ML = smem_bytes<AR>();
-  constexpr size_t XS = (size_t)2 * TILE * (DMAX + 1) * sizeof(double);
+  constexpr size_t XS = (size_t)2 * TILE * (DS + 1) * sizeof(double);
   constexpr size_t AB = (size_t)4 * TILE * sizeof(double);
-  constexpr size_t WB = (size_t)WL::NW * NS * 64 * sizeof(AS);
+  constexpr size_t WB = (size_t)WL::NW * NS * 64 * sizeof(AS) + DMAX * sizeof(double);
   constexpr size_t EP = XS + AB + WB;
   __shared__ __align__(16) char smem[ML > EP ? ML : EP];
-  __shared__ double red_s[WL::NW][DMAX + 2];
+  __shared__ double red_s[WL::NW][DS + 2];
 
   int64_t mt, nt;
   lower_tile_banded(blockIdx.x, np / TILE, mt, nt);
@@ -67,26 +71,34 @@ __global__ __launch_bounds__(WaveLayout<AR>::NT) void adjoint_kernel(int kind,
   }
   __syncthreads();
 
-  // Epilogue staging: scaled inputs and alpha of the tile's rows (i) and
-  // columns (j), and a per-lane strip through which each lane re-reads its own
-  // accumulators with a rolled loop.
-  double* xm = reinterpret_cast<double*>(smem);        // [TILE][DMAX+1]
-  double* xn = xm + TILE * (DMAX + 1);                 // [TILE][DMAX+1]
-  double* am = xn + TILE * (DMAX + 1);                 // [TILE]
+  const int t = threadIdx.x;
+  const int lane = t & 63, w = t >> 6;
+  const int wm = w / WL::WN, wn = w % WL::WN;
+  const double wt = (mt == nt) ? 1.0 : 2.0;
+  const bool ard = nls > 1;
+  const int nv = nls + 2;
+  // Epilogue staging: scaled inputs (DS dimensions at a time) and alpha of the
+  // tile's rows (i) and columns (j), and a per-lane strip through which each
+  // lane re-reads its own accumulators with a rolled loop.
+  double* xm = reinterpret_cast<double*>(smem);        // [TILE][DS+1]
+  double* xn = xm + TILE * (DS + 1);                   // [TILE][DS+1]
+  double* am = xn + TILE * (DS + 1);                   // [TILE]
   double* an = am + TILE;                              // [TILE]
   double* bm = an + TILE;                              // [TILE]
   double* bn = bm + TILE;                              // [TILE]
   AS* wbuf = reinterpret_cast<AS*>(bn + TILE);         // [waves][NS][64 lanes]
-  const int t = threadIdx.x;
-  for (int e = t; e < TILE * DMAX; e += WL::NT) {
-    const int r = e / DMAX, q = e % DMAX;
-    if (q < d) {
-      const double l = theta[nls == 1 ? 0 : q];
-      const int64_t gi = m0 + r, gj = n0 + r;
-      xm[r * (DMAX + 1) + q] = gi < n ? x[gi * d + q] / l : 0.0;
-      xn[r * (DMAX + 1) + q] = gj < n ? x[gj * d + q] / l : 0.0;
+  auto stage = [&](int dc) {
+    for (int e = t; e < TILE * DS; e += WL::NT) {
+      const int r = e / DS, q = e % DS;
+      if (dc + q < d) {
+        const double l = theta[nls == 1 ? 0 : dc + q];
+        const int64_t gi = m0 + r, gj = n0 + r;
+        xm[r * (DS + 1) + q] = gi < n ? x[gi * d + dc + q] / l : 0.0;
+        xn[r * (DS + 1) + q] = gj < n ? x[gj * d + dc + q] / l : 0.0;
+      }
     }
-  }
+  };
+  stage(0);
   if (t < TILE) {
     am[t] = alpha[m0 + t];
     an[t] = alpha[n0 + t];
@@ -96,79 +108,198 @@ __global__ __launch_bounds__(WaveLayout<AR>::NT) void adjoint_kernel(int kind,
     }
   }
   __syncthreads();
-
-  const int lane = t & 63, w = t >> 6;
-  const int wm = w / WL::WN, wn = w % WL::WN;
-  const double wt = (mt == nt) ? 1.0 : 2.0;
-  const bool ard = nls > 1;
-  double s_var = 0.0, s_tr = 0.0;
-  double s_ls[DMAX];
-#pragma unroll
-  for (int q = 0; q < DMAX; ++q) s_ls[q] = 0.0;
   AS* mine = wbuf + w * NS * 64 + lane;
 
+  if constexpr (DMAX <= ADJ_DCH) {
+    double s_var = 0.0, s_tr = 0.0;
+    double s_ls[DMAX];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    // Only this lane reads these NS slots back, so no barrier is needed.
+    for (int q = 0; q < DMAX; ++q) s_ls[q] = 0.0;
 #pragma unroll
-    for (int j = 0; j < WL::NJ; ++j)
+    for (int i = 0; i < 4; ++i) {
+      // Only this lane reads these NS slots back, so no barrier is needed.
 #pragma unroll
-      for (int r = 0; r < 4; ++r) mine[(j * 4 + r) * 64] = acc[i][j][r];
+      for (int j = 0; j < WL::NJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) mine[(j * 4 + r) * 64] = acc[i][j][r];
 #pragma unroll 1
-    for (int e = 0; e < NS; ++e) {
-      const int j = e >> 2, r = e & 3;
-      const int li = wm * 64 + i * 16 + acc_row<AR>(lane, r);
-      const int lj = wn * WL::WC + j * 16 + (lane & 15);
-      const int64_t gi = m0 + li, gj = n0 + lj;
-      if (gi >= n || gj >= n) continue;
-      const double wij = (beta ? am[li] * bn[lj] + bm[li] * an[lj] : am[li] * an[lj]) -
-                         (double)mine[e * 64];
-      const double* pi = xm + li * (DMAX + 1);
-      const double* pj = xn + lj * (DMAX + 1);
-      double d2 = 0.0;
-#pragma unroll
-      for (int q = 0; q < DMAX; ++q) {
-        if (q < d) {
-          const double df = pi[q] - pj[q];
-          d2 += df * df;
-        }
-      }
-      double kv, fv;
-      kval_fac(kind, d2, kv, fv);
-      const double g = wt * wij;
-      s_var += g * kv;
-      if (gi == gj) s_tr += wij;
-      const double gf = g * fv;
-      if (ard) {
+      for (int e = 0; e < NS; ++e) {
+        const int j = e >> 2, r = e & 3;
+        const int li = wm * 64 + i * 16 + acc_row<AR>(lane, r);
+        const int lj = wn * WL::WC + j * 16 + (lane & 15);
+        const int64_t gi = m0 + li, gj = n0 + lj;
+        if (gi >= n || gj >= n) continue;
+        const double wij = (beta ? am[li] * bn[lj] + bm[li] * an[lj] : am[li] * an[lj]) -
+                           (double)mine[e * 64];
+        const double* pi = xm + li * (DMAX + 1);
+        const double* pj = xn + lj * (DMAX + 1);
+        double d2 = 0.0;
 #pragma unroll
         for (int q = 0; q < DMAX; ++q) {
           if (q < d) {
             const double df = pi[q] - pj[q];
-            s_ls[q] += gf * df * df;
+            d2 += df * df;
           }
         }
-      } else {
-        s_ls[0] += gf * d2;
+        double kv, fv;
+        kval_fac(kind, d2, kv, fv);
+        const double g = wt * wij;
+        s_var += g * kv;
+        if (gi == gj) s_tr += wij;
+        const double gf = g * fv;
+        if (ard) {
+#pragma unroll
+          for (int q = 0; q < DMAX; ++q) {
+            if (q < d) {
+              const double df = pi[q] - pj[q];
+              s_ls[q] += gf * df * df;
+            }
+          }
+        } else {
+          s_ls[0] += gf * d2;
+        }
       }
     }
-  }
 
-  // Workgroup reduction: wave shuffles, then across the waves in order.
-  const int nv = nls + 2;
+    // Workgroup reduction: wave shuffles, then across the waves in order.
 #pragma unroll
-  for (int q = 0; q < DMAX + 2; ++q) {
-    double v = q < DMAX ? s_ls[q] : (q == DMAX ? s_var : s_tr);
+    for (int q = 0; q < DMAX + 2; ++q) {
+      double v = q < DMAX ? s_ls[q] : (q == DMAX ? s_var : s_tr);
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if (lane == 0) red_s[w][q] = v;
-  }
-  __syncthreads();
-  if (t < nv) {
-    const int q = t < nls ? t : (t == nls ? DMAX : DMAX + 1);
-    double v = red_s[0][q];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if (lane == 0) red_s[w][q] = v;
+    }
+    __syncthreads();
+    if (t < nv) {
+      const int q = t < nls ? t : (t == nls ? DMAX : DMAX + 1);
+      double v = red_s[0][q];
 #pragma unroll
-    for (int u = 1; u < WL::NW; ++u) v += red_s[u][q];
-    partial[(int64_t)blockIdx.x * nv + t] = v;
+      for (int u = 1; u < WL::NW; ++u) v += red_s[u][q];
+      partial[(int64_t)blockIdx.x * nv + t] = v;
+    }
+  } else {
+    // Inputs wider than ADJ_DCH: per 16-row block, the squared distances of the
+    // lane's NS elements are summed over the dimension chunks (the chunk staged
+    // in turn), then each element's derivative weight gf is formed, and a second
+    // sweep over the chunks adds gf * (x_id - x_jd)^2 into per-dimension sums,
+    // which are reduced across the workgroup chunk by chunk into LDS (acc_ls).
+    double* acc_ls = reinterpret_cast<double*>(wbuf + WL::NW * NS * 64);  // [DMAX]
+    if (t < DMAX) acc_ls[t] = 0.0;
+    double s_var = 0.0, s_tr = 0.0, s_iso = 0.0;
+    const int nch = (d + DS - 1) / DS;
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int j = 0; j < WL::NJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) mine[(j * 4 + r) * 64] = acc[i][j][r];
+      double d2[NS], gf[NS];
+#pragma unroll
+      for (int e = 0; e < NS; ++e) d2[e] = 0.0;
+      for (int c = 0; c < nch; ++c) {
+        if (c > 0 || i > 0) {
+          __syncthreads();
+          stage(c * DS);
+          __syncthreads();
+        }
+#pragma unroll
+        for (int e = 0; e < NS; ++e) {
+          const int li = wm * 64 + i * 16 + acc_row<AR>(lane, e & 3);
+          const int lj = wn * WL::WC + (e >> 2) * 16 + (lane & 15);
+          const double* pi = xm + li * (DS + 1);
+          const double* pj = xn + lj * (DS + 1);
+#pragma unroll
+          for (int q = 0; q < DS; ++q) {
+            if (c * DS + q < d) {
+              const double df = pi[q] - pj[q];
+              d2[e] += df * df;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < NS; ++e) {
+        const int li = wm * 64 + i * 16 + acc_row<AR>(lane, e & 3);
+        const int lj = wn * WL::WC + (e >> 2) * 16 + (lane & 15);
+        const int64_t gi = m0 + li, gj = n0 + lj;
+        gf[e] = 0.0;
+        if (gi >= n || gj >= n) continue;
+        const double wij = (beta ? am[li] * bn[lj] + bm[li] * an[lj] : am[li] * an[lj]) -
+                           (double)mine[e * 64];
+        double kv, fv;
+        kval_fac(kind, d2[e], kv, fv);
+        const double g = wt * wij;
+        s_var += g * kv;
+        if (gi == gj) s_tr += wij;
+        gf[e] = g * fv;
+        s_iso += gf[e] * d2[e];
+      }
+      if (!ard) continue;
+      for (int c = nch - 1; c >= 0; --c) {
+        // the last chunk staged is still in LDS; the others are staged again
+        if (c != nch - 1) {
+          __syncthreads();
+          stage(c * DS);
+          __syncthreads();
+        }
+        double s_ls[DS];
+#pragma unroll
+        for (int q = 0; q < DS; ++q) s_ls[q] = 0.0;
+#pragma unroll
+        for (int e = 0; e < NS; ++e) {
+          const int li = wm * 64 + i * 16 + acc_row<AR>(lane, e & 3);
+          const int lj = wn * WL::WC + (e >> 2) * 16 + (lane & 15);
+          const double* pi = xm + li * (DS + 1);
+          const double* pj = xn + lj * (DS + 1);
+#pragma unroll
+          for (int q = 0; q < DS; ++q) {
+            const double df = pi[q] - pj[q];
+            s_ls[q] += gf[e] * df * df;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < DS; ++q) {
+          double v = s_ls[q];
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+          if (lane == 0) red_s[w][q] = v;
+        }
+        __syncthreads();
+        if (t < DS && c * DS + t < d) {
+          double v = red_s[0][t];
+#pragma unroll
+          for (int u = 1; u < WL::NW; ++u) v += red_s[u][t];
+          acc_ls[c * DS + t] += v;
+        }
+      }
+    }
+    // variance and trace (and the isotropic lengthscale) sums
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      double v = q == 0 ? s_var : (q == 1 ? s_tr : s_iso);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if (lane == 0) red_s[w][q] = v;
+    }
+    __syncthreads();
+    if (t < nv) {
+      double v;
+      if (t < nls) {
+        if (ard) {
+          v = acc_ls[t];
+        } else {
+          v = red_s[0][2];
+#pragma unroll
+          for (int u = 1; u < WL::NW; ++u) v += red_s[u][2];
+        }
+      } else {
+        const int q = t == nls ? 0 : 1;
+        v = red_s[0][q];
+#pragma unroll
+        for (int u = 1; u < WL::NW; ++u) v += red_s[u][q];
+      }
+      partial[(int64_t)blockIdx.x * nv + t] = v;
+    }
   }
 }
 
@@ -195,7 +326,8 @@ int launch_ar(int kind, const TS* X, int64_t ld, int64_t np, const double* x, in
   else if (d <= 4) GPK_ADJ(4);
   else if (d <= 8) GPK_ADJ(8);
   else if (d <= 16) GPK_ADJ(16);
-  else GPK_ADJ(32);
+  else if (d <= 32) GPK_ADJ(32);
+  else GPK_ADJ(64);
 #undef GPK_ADJ
   GPK_LAUNCH_CHECK();
   return GPK_OK;

@@ -127,7 +127,7 @@ struct Layout {
 
 Layout make_layout(int64_t n, int64_t d, int prec) {
   Layout L;
-  L.np = round_up(n, 256);
+  L.np = round_up(n, NB);
   L.ts = prec == GPK_PREC_FP64 ? sizeof(double) : sizeof(float);
   const int64_t np = L.np;
   L.htop = np > NB ? split_point(np) : 0;
@@ -541,9 +541,11 @@ int mll_forward_t(int kind, const Arith3& ar, const double* x, const double* y, 
       GPK_LAUNCH_CHECK();
     }
   }
-  // the final residual (also for a refinement that stopped early: alpha no longer changed)
-  GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, alpha, v, part, s, bt));
-  hipLaunchKernelGGL(sub_kernel, gv, dim3(256), 0, s, r, v, np, res, bt);
+  // The final residual r - Sigma alpha.  A refinement that stopped early left it
+  // in `res` already (alpha has not changed since), so these launches return
+  // at once then; otherwise they measure the last correction.
+  GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, alpha, v, part, s, bts));
+  hipLaunchKernelGGL(sub_kernel, gv, dim3(256), 0, s, r, v, np, res, bts);
   GPK_LAUNCH_CHECK();
   hipLaunchKernelGGL(finalize_fwd_kernel, dim3(1, 1, bt.count), dim3(256), 0, s,
                      at<double>(ws, Lo.off_leafld), at<int>(ws, Lo.off_leafinfo), np / NB, r,
@@ -864,7 +866,7 @@ int loocv_backward_t(int kind, const Arith3& ar3, const double* x, int64_t n, in
 }
 
 // ---- batched exact GPs ----
-// Problems are grouped by padded size np = round_up(n, 256) (stable in the
+// Problems are grouped by padded size np = round_up(n, 128) (stable in the
 // problem index); each group runs the exact-GP forward and backward once, with
 // one member per blockIdx.z.  Workspace: the member -> problem map and the row
 // offsets (uploaded by gpk_batched_plan), then per group `count` member
@@ -906,7 +908,7 @@ BatchPlan make_batch_plan(const int64_t* off, int64_t count, int64_t d) {
   BatchPlan P;
   P.order.resize((size_t)count);
   for (int64_t p = 0; p < count; ++p) P.order[(size_t)p] = (int)p;
-  auto padded = [&](int p) { return round_up(off[p + 1] - off[p], 256); };
+  auto padded = [&](int p) { return round_up(off[p + 1] - off[p], NB); };
   std::stable_sort(P.order.begin(), P.order.end(),
                    [&](int a, int b) { return padded(a) < padded(b); });
   size_t o = 0;

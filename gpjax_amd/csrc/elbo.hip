@@ -29,7 +29,7 @@
 namespace gpk {
 namespace {
 
-constexpr int EDMAX = 16;   // largest input dimension on the ELBO path
+constexpr int EDMAX = 32;   // largest input dimension on the ELBO path
 constexpr int64_t BW_CHUNK = 65536;  // points per Kzx chunk (statistics, data pass)
 constexpr int64_t EW = 1024;         // points per block of the gradient data pass
 
@@ -552,7 +552,8 @@ inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b);
     else if ((d) <= 2) MACRO(2);    \
     else if ((d) <= 4) MACRO(4);    \
     else if ((d) <= 8) MACRO(8);    \
-    else MACRO(16);                 \
+    else if ((d) <= 16) MACRO(16);  \
+    else MACRO(32);                 \
   } while (0)
 
 int elbo_forward_impl(int kind, const double* x, const double* y, int64_t n, const double* z,
@@ -708,11 +709,12 @@ int elbo_backward_impl(int kind, const double* x, const double* y, int64_t n, co
 #undef GPK_E4
     GPK_LAUNCH_CHECK();
   }
-  int dmax = 16;
+  int dmax = 32;
   if (d <= 1) dmax = 1;
   else if (d <= 2) dmax = 2;
   else if (d <= 4) dmax = 4;
   else if (d <= 8) dmax = 8;
+  else if (d <= 16) dmax = 16;
   const unsigned gblocks = (unsigned)(nls + 3) + cdiv(m * d, 256);
   hipLaunchKernelGGL(elbo_grad_finalize, dim3(gblocks), dim3(256), 0, s,
                      at<double>(ws, Lo.off_tpart), Lo.S2 * Lo.mt, at<double>(ws, Lo.off_tpart2),

@@ -16,8 +16,10 @@ namespace gpk {
 // blocks factored in LDS, and every GEMM tile is NB x NB.  Internal matrices
 // are padded to a multiple of NB with an identity tail.
 constexpr int NB = 128;
-// Largest input dimension the fused kernels keep in LDS / registers.
-constexpr int MAX_D = 32;
+// Largest input dimension of the exact-GP kernels (the Sigma products keep a
+// point's coordinates in registers; the gradient epilogue stages them in LDS in
+// chunks of 32).
+constexpr int MAX_D = 64;
 
 void set_error(const char* fmt, ...);
 

@@ -42,9 +42,12 @@ __global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_
                                                             int* __restrict__ info,
                                                             const double* __restrict__ dsrc,
                                                             Batch bt) {
-  __shared__ double col_s[2][NB];
-  __shared__ double dinv_s[NB];
+  // pub[buf][kk][i]: the step factors of the four columns of the last panel
+  // (row factor of row i; for rows below the step's column also its column
+  // factor).  Double-buffered across panels.
+  __shared__ double pub[2][EB][NB];
   __shared__ double dk_s[NB];
+  __shared__ double dinv_s[NB];
   __shared__ int bad_s;
   __shared__ double tr_s[NB][NB + 1];
   A = bws(A, bt);
@@ -55,8 +58,10 @@ __global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_
   dsrc = bws(dsrc, bt);
 
   const int t = threadIdx.x;
-  const int rb = (t % BPR) * EB;
-  const int cb = (t / BPR) * EB;
+  const int rb = (t % BPR) * EB;   // first row of this thread's block
+  const int cb = (t / BPR) * EB;   // first column (the half-wave t / 32 owns columns cb .. cb+3)
+  const int hl = t % BPR;          // lane within the half-wave: rows 4 hl .. 4 hl + 3
+  const int hbase = (t & 63) & ~(BPR - 1);  // first wave lane of this half-wave
   // Stage the block through LDS so that global reads stay row-contiguous.
   for (int e = t; e < NB * NB; e += LEAF_THREADS) {
     const int i = e / NB, j = e % NB;
@@ -69,88 +74,80 @@ __global__ __launch_bounds__(LEAF_THREADS) void leaf_kernel(TS* A, TS* X, int64_
   for (int a = 0; a < EB; ++a)
 #pragma unroll
     for (int b = 0; b < EB; ++b) s[a][b] = (cb + b <= rb + a) ? tr_s[rb + a][cb + b] : 0.0;
-  // pivot 0 and column 0
-  if (cb == 0) {
-#pragma unroll
-    for (int a = 0; a < EB; ++a) col_s[0][rb + a] = s[a][0];
-    if (rb == 0) {
-      const double piv = s[0][0];
-      const double dk = sqrt(piv);
-      dk_s[0] = dk;
-      dinv_s[0] = 1.0 / dk;
-      if (!(piv > 0.0)) bad_s = 0;
-    }
-  }
-  __syncthreads();
 
-  for (int k = 0; k < NB; ++k) {
-    // Every update at step k touches column k or a column > k; the test is
-    // uniform over each half-wave (one column block).
-    if (cb + EB - 1 >= k) {
-      const double* col = col_s[k & 1];
-      const double dk = dk_s[k];
-      const double inv = dinv_s[k];
-      // Entry (i, j) changes at step k only if j > k and either i >= j (lower
-      // trailing update L[i][k] L[j][k]) or i <= k (strict upper: numerator of
-      // X[j][i] -= L[j][k] X[k][i]).  The j > k mask goes into the column
-      // factors, the i <= k mask into the row factors, and whether i >= j is
-      // fixed per thread: always (rb > cb), never (rb < cb), or a >= b on the
-      // diagonal block, which is known at compile time.  So every entry takes
-      // one FMA, with a zero factor where it must not change.
-      double sci[EB], alo[EB], ahi[EB], scc[EB];
+  for (int p = 0; p < NB / EB; ++p) {
+    const int c0 = p * EB;  // the panel's columns c0 .. c0 + 3
+    if (cb == c0) {
+      // Factor the panel inside this half-wave.  Step k = c0 + kk: the pivot
+      // and the panel's own entries of column k (rows c0 .. c0+3) sit in the
+      // half-wave lane p; each is broadcast from there.  As in the unblocked
+      // sweep, entry (i, j), j > k, takes one FMA with row factor
+      // f_i = s_ik / d_k (1 / d_k on the diagonal) and column factor s_jk / d_k
+      // when i >= j (trailing update of L) or i <= k (numerator of X[j][i]).
 #pragma unroll
-      for (int a = 0; a < EB; ++a) {
-        const double v = col[rb + a] * inv;
-        sci[a] = (rb + a == k) ? inv : v;
-        const double sx = (rb + a <= k) ? sci[a] : 0.0;
-        alo[a] = (rb >= cb) ? sci[a] : sx;  // entries with a >= b
-        ahi[a] = (rb > cb) ? sci[a] : sx;   // entries with a < b
-      }
+      for (int kk = 0; kk < EB; ++kk) {
+        const int k = c0 + kk;
+        double cv[EB];
 #pragma unroll
-      for (int b = 0; b < EB; ++b) scc[b] = (cb + b > k) ? col[cb + b] * inv : 0.0;
+        for (int a = 0; a < EB; ++a) cv[a] = __shfl(s[a][kk], hbase + p);
+        const double piv = cv[kk];
+        const double dk = sqrt(piv);
+        const double inv = 1.0 / dk;
+        double f[EB];
 #pragma unroll
-      for (int a = 0; a < EB; ++a)
+        for (int a = 0; a < EB; ++a) f[a] = (rb + a == k) ? inv : s[a][kk] * inv;
 #pragma unroll
-        for (int b = 0; b < EB; ++b)
-          s[a][b] = fma(-(a >= b ? alo[a] : ahi[a]), scc[b], s[a][b]);
-      // Column k itself: L[i][k] below the diagonal, the pivot on it, X[k][i]
-      // (stored at (i, k)) above; only the half-wave holding column k.
-      if (k >= cb) {
+        for (int b = kk + 1; b < EB; ++b) {
+          const double g = cv[b] * inv;  // column factor of column c0 + b
 #pragma unroll
-        for (int b = 0; b < EB; ++b)
-          if (cb + b == k) {
-#pragma unroll
-            for (int a = 0; a < EB; ++a) s[a][b] = (rb + a == k) ? dk : sci[a];
+          for (int a = 0; a < EB; ++a) {
+            const int i = rb + a, j = c0 + b;
+            const double fa = (i >= j || i <= k) ? f[a] : 0.0;
+            s[a][b] = fma(-fa, g, s[a][b]);
           }
-      }
-      // Publish column k+1 and, from the owner of its diagonal entry, its pivot.
-      const int q = k + 1 - cb;
-      if (k + 1 < NB && q >= 0 && q < EB) {
-        double v[EB];
-#pragma unroll
-        for (int a = 0; a < EB; ++a) {
-          v[a] = s[a][0];
-#pragma unroll
-          for (int b = 1; b < EB; ++b) v[a] = (q == b) ? s[a][b] : v[a];
-          col_s[(k + 1) & 1][rb + a] = v[a];
         }
-        const int p = k + 1 - rb;
-        if (p >= 0 && p < EB) {
-          double piv = v[0];
 #pragma unroll
-          for (int a = 1; a < EB; ++a) piv = (p == a) ? v[a] : piv;
-          const double dkn = sqrt(piv);
-          dk_s[k + 1] = dkn;
-          dinv_s[k + 1] = 1.0 / dkn;
-          if (!(piv > 0.0) && k + 1 < bad_s) bad_s = k + 1;
+        for (int a = 0; a < EB; ++a) s[a][kk] = (rb + a == k) ? dk : f[a];
+#pragma unroll
+        for (int a = 0; a < EB; ++a) pub[p & 1][kk][rb + a] = f[a];
+        if (hl == p) {
+          dk_s[k] = dk;
+          dinv_s[k] = inv;
+          if (!(piv > 0.0) && k < bad_s) bad_s = k;
         }
       }
     }
     __syncthreads();
+    if (cb > c0) {
+      // Rank-4 update of the columns right of the panel, steps in order.
+      const double* P = &pub[p & 1][0][0];
+#pragma unroll
+      for (int kk = 0; kk < EB; ++kk) {
+        const int k = c0 + kk;
+        double fr[EB], g[EB];
+#pragma unroll
+        for (int a = 0; a < EB; ++a) fr[a] = P[kk * NB + rb + a];
+#pragma unroll
+        for (int b = 0; b < EB; ++b) g[b] = P[kk * NB + cb + b];
+#pragma unroll
+        for (int a = 0; a < EB; ++a) {
+          const int i = rb + a;
+          // j = cb + b > k always here; the mask is i >= j or i <= k
+          const double flo = fr[a];
+          const double fhi = (i <= k) ? fr[a] : 0.0;
+#pragma unroll
+          for (int b = 0; b < EB; ++b) {
+            const double fa = (i >= cb + b) ? flo : fhi;
+            s[a][b] = fma(-fa, g[b], s[a][b]);
+          }
+        }
+      }
+    }
   }
 
   // L (lower incl. diagonal) and X (from the upper part, transposed) staged
   // in LDS for row-contiguous writes.
+  __syncthreads();
 #pragma unroll
   for (int a = 0; a < EB; ++a)
 #pragma unroll

@@ -29,7 +29,7 @@ struct LinLayout {
 
 LinLayout make_lin_layout(int64_t n, int64_t m, int prec) {
   LinLayout P;
-  P.np = round_up(n, 256);
+  P.np = round_up(n, NB);
   P.mp = m > 0 ? round_up(m, NB) : 0;
   P.ts = prec == GPK_PREC_FP64 ? sizeof(double) : sizeof(float);
   const int64_t np = P.np, nleaf = np / NB;

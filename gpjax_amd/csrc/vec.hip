@@ -365,7 +365,8 @@ int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
   else if (d <= 4) GPK_SMV(4);
   else if (d <= 8) GPK_SMV(8);
   else if (d <= 16) GPK_SMV(16);
-  else GPK_SMV(32);
+  else if (d <= 32) GPK_SMV(32);
+  else GPK_SMV(64);
 #undef GPK_SMV
   GPK_LAUNCH_CHECK();
   hipLaunchKernelGGL(sum_chunks, dim3((unsigned)((np + 255) / 256), 1, bt.count), dim3(256), 0, s,

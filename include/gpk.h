@@ -168,7 +168,7 @@ size_t gpk_elbo_workspace_bytes(int64_t n, int64_t m, int64_t d, int prec);
  * accumulated on the matrix cores chunk by chunk (Kzx evaluated once per chunk
  * of 65,536 points into the workspace): no n x m matrix is stored.  result slots: [GPK_RES_MLL] = ELBO, [1] = log|B|,
  * [2] = the quadratic term, [3] = the trace term, [GPK_RES_INFO] = pivot status.
- * d <= 16; prec must be GPK_PREC_FP64.
+ * d <= 32; prec must be GPK_PREC_FP64.
  */
 int gpk_elbo_forward(int kind, int prec, const double* x, const double* y, int64_t n,
                      const double* z, int64_t m, int64_t d, const double* theta, int nls,
@@ -274,9 +274,8 @@ size_t gpk_batched_workspace_bytes(const int64_t* offsets_host, int64_t count, i
  * Writes the batch's problem map into ws (synchronous host-to-device copies).
  * Call once for a given (offsets, ws) before gpk_batched_mll_grad, outside any
  * HIP-graph capture.  The library remembers, per workspace address, which
- * (offsets, count, d) it planned (a mutex-guarded host table: with the autotune
- * cache, its only process-wide state), and gpk_batched_mll_grad refuses a
- * workspace planned for anything else.
+ * (offsets, count, d) it planned (a mutex-guarded host table), and
+ * gpk_batched_mll_grad refuses a workspace planned for anything else.
  */
 int gpk_batched_plan(const int64_t* offsets_host, int64_t count, int64_t d, void* ws,
                      size_t ws_bytes);

@@ -51,7 +51,7 @@ def test_version_and_workspace_sizes():
     (9, 0, 10, 2, 1, 1),     # unknown kernel
     (0, 7, 10, 2, 1, 1),     # unknown precision
     (0, 0, 0, 2, 1, 1),      # n < 1
-    (0, 0, 10, 99, 1, 1),    # d above the fused-kernel limit
+    (0, 0, 10, 99, 1, 1),    # d above the kernels' limit (64)
     (0, 0, 10, 3, 2, 1),     # nls neither 1 nor d
 ])
 def test_argument_validation_without_device(kind, prec, n, d, nls, code):
@@ -165,18 +165,18 @@ def _offsets(*rows):
 def test_batched_workspace_groups_by_padded_size():
     lib = _lib.lib()
     d = 4
-    off = _offsets(10, 290, 10, 256)           # padded sizes 256, 512, 256, 256
+    off = _offsets(10, 290, 10, 256)           # padded sizes 128, 384, 128, 256
     got = lib.gpk_batched_workspace_bytes(off, 4, d)
-    per = {np_: lib.gpk_mll_workspace_bytes(np_, d, 0) for np_ in (256, 512)}
+    per = {np_: lib.gpk_mll_workspace_bytes(np_, d, 0) for np_ in (128, 256, 384)}
     meta = 256 + 256                            # member map and row offsets, 256-byte aligned
-    assert got == 3 * per[256] + per[512] + meta
+    assert got == 2 * per[128] + per[256] + per[384] + meta
 
 
 @pytest.mark.parametrize("rows,count,d", [
     ((5, 0, 7), 3, 2),       # a problem without rows
     ((5, 7), 0, 2),          # no problems
     ((5, 7), 2, 0),          # d < 1
-    ((5, 7), 2, 99),         # d above the fused-kernel limit
+    ((5, 7), 2, 99),         # d above the kernels' limit (64)
 ])
 def test_batched_validation_without_device(rows, count, d):
     lib = _lib.lib()
@@ -221,9 +221,9 @@ def test_linalg_entry_points_validate_on_the_host():
     assert lib.gpk_linalg_workspace_bytes(300, 0, 7) == 0
     w64 = lib.gpk_linalg_workspace_bytes(300, 0, 0)
     w32 = lib.gpk_linalg_workspace_bytes(300, 0, 1)
-    assert 2 * 512 * 512 * 8 <= w64 and 2 * 512 * 512 * 4 <= w32 < w64
+    assert 2 * 384 * 384 * 8 <= w64 and 2 * 384 * 384 * 4 <= w32 < w64
     # right-hand sides add two padded n x m buffers
-    assert lib.gpk_linalg_workspace_bytes(300, 5, 0) >= w64 + 2 * 512 * 128 * 8
+    assert lib.gpk_linalg_workspace_bytes(300, 5, 0) >= w64 + 2 * 384 * 128 * 8
     assert lib.gpk_potrf(0, d, 10, 5, d, 10, None, None, d, 1 << 40, None) == 1   # lda < n
     assert lib.gpk_potrf(3, d, 10, 10, d, 10, None, None, d, 1 << 40, None) == 1  # precision
     assert lib.gpk_potrf(0, d, 10, 10, d, 10, None, None, d, 16, None) == 5       # workspace

@@ -42,13 +42,17 @@ def _setup(cuda, kind, n, m, d, seed, ard=True, mean=0.15, lsr=(0.2, 0.6)):
 @pytest.mark.parametrize("kind,n,m,d,ard,tol", [("rbf", 500, 40, 2, True, 1e-9),
                                                 ("matern32", 3000, 300, 8, True, 1e-9),
                                                 ("matern52", 1234, 129, 3, False, 1e-9),
-                                                ("matern12", 700, 64, 1, True, 1e-9)])
+                                                ("matern12", 700, 64, 1, True, 1e-9),
+                                                ("matern52", 900, 96, 24, True, 1e-9),
+                                                ("rbf", 640, 80, 32, True, 1e-9)])
 def test_elbo_and_grad_parity(cuda, kind, n, m, d, ard, tol):
-    """For RBF the lengthscales are drawn in [0.05, 0.15] (40 inducing points in
-    the unit square): longer RBF lengthscales make Kzz (jitter 1e-6) so
-    ill-conditioned that the inducing-input gradient is not determined to any
-    digit in float64 by either implementation."""
-    lsr = (0.05, 0.15) if kind == "rbf" else (0.2, 0.6)
+    """For RBF in two dimensions the lengthscales are drawn in [0.05, 0.15] (40
+    inducing points in the unit square): longer RBF lengthscales make Kzz (jitter
+    1e-6) so ill-conditioned that the inducing-input gradient is not determined
+    to any digit in float64 by either implementation.  In 24 and 32 dimensions
+    (the widest inputs the ELBO kernels take) lengthscales in [0.6, 1.6] keep
+    the points within a few lengthscales of each other."""
+    lsr = (0.6, 1.6) if d > 16 else ((0.05, 0.15) if kind == "rbf" else (0.2, 0.6))
     X, y, Z, ls, var, sd, mean, k, post, q, D = _setup(cuda, kind, n, m, d, seed=n + m, ard=ard,
                                                        lsr=lsr)
     elbo, res = gpx.objectives.collapsed_elbo_with_info(q, D)

@@ -267,3 +267,33 @@ def test_not_positive_definite_first_pivot(cuda):
     mll, res = gpx.objectives.conjugate_mll_with_info(post, D, precision="fp64")
     assert np.isnan(float(mll.detach()))
     assert int(res[_lib.RES_INFO].item()) == 1
+
+
+@pytest.mark.parametrize("n,d,kind,ard,precision", [(500, 48, "matern52", True, "fp64"),
+                                                    (300, 40, "rbf", False, "fp64"),
+                                                    (700, 64, "rbf", True, "fp64"),
+                                                    (600, 33, "matern32", True, "mixed")])
+def test_wide_inputs_parity(cuda, n, d, kind, ard, precision):
+    """Inputs wider than 32 dimensions (the gradient epilogue stages them in
+    chunks of 32; GPJax takes any D, stationary/base.py:120-146): MLL and
+    gradient against the oracle, FP64 at 1e-10 / 1e-8, mixed at 1e-6 / 1e-4.
+    Lengthscales in [2, 6] keep the points within a few lengthscales."""
+    rng = np.random.default_rng(n + d)
+    X = rng.uniform(-2, 2, (n, d))
+    y = np.sin(X[:, 0]) + 0.5 * np.cos(X[:, 1]) + 0.1 * rng.standard_normal(n)
+    ls = np.exp(rng.uniform(np.log(2.0), np.log(6.0), d)) if ard else 4.0
+    post, D, k = _model(cuda, X, y, kind, ls, 1.3, 0.2, mean=0.1)
+    mll = gpx.objectives.conjugate_mll(post, D, precision=precision)
+    mll.backward()
+    ref = O.conjugate_mll_grad(kind, X, y, ls, 1.3, 0.2, mean=0.1)
+    g = np.concatenate([np.atleast_1d(k.lengthscale.value.grad.cpu().numpy()),
+                        [k.variance.value.grad.item(),
+                         post.likelihood.obs_stddev.value.grad.item(),
+                         post.prior.mean_function.constant.value.grad.item()]])
+    gr = np.concatenate([np.atleast_1d(ref["lengthscale"]),
+                         [ref["variance"], ref["obs_stddev"], ref["mean"]]])
+    tm, tg = (1e-10, 1e-8) if precision == "fp64" else (1e-6, 1e-4)
+    rel_m = abs(float(mll.detach()) - ref["mll"]) / abs(ref["mll"])
+    rel_g = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+    print(f"wide d={d} {kind} {precision}: rel_mll={rel_m:.2e} rel_grad={rel_g:.2e}")
+    assert rel_m <= tm and rel_g <= tg
