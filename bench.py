@@ -145,6 +145,11 @@ def other_workload(args) -> dict:
 
         t, mll = timed(step, args.steps, args.warmup)
         sizes = [int(s) for s in w["sizes"]]
+        line["autotune"] = [{"padded_size": r["padded_size"], "members": r["members"],
+                             "stored_sigma": r["stored_sigma"],
+                             "transposed_pass": r["transposed_pass"],
+                             "ms": {k: round(v, 3) for k, v in r["ms"].items()}}
+                            for r in m.tune_report]
         # training: 100 Adam steps on the summed negative MLL, one step captured in a
         # HIP graph (device Adam, no host sync inside) and replayed
         fm = gpx.batched.BatchedConjugateMLL(w["Xs"], w["ys"], kernel="rbf",
@@ -159,6 +164,37 @@ def other_workload(args) -> dict:
         torch.cuda.synchronize()
         t_fit = time.perf_counter() - t0
         hist = hist.cpu()
+        # the same batch on the mixed path (float32 storage, bf16 splits, float64
+        # panels; each problem's probe check), timed with the check's host read off
+        mm = gpx.batched.BatchedConjugateMLL(w["Xs"], w["ys"], kernel="rbf",
+                                             lengthscale=w["lengthscale"], variance=w["variance"],
+                                             obs_stddev=w["obs_stddev"], jitter=w["jitter"],
+                                             device=dev, precision="mixed")
+        mparams = [mm.lengthscale, mm.variance, mm.obs_stddev]
+
+        def mstep():
+            for p in mparams:
+                p.value.grad = None
+            mll = mm.mll()
+            mll.sum().backward()
+            return mll
+
+        old_check = gpx.config.mixed_check
+        gpx.config.mixed_check = "off"
+        try:
+            t_mx, mll_mx = timed(mstep, args.steps, args.warmup)
+        finally:
+            gpx.config.mixed_check = old_check
+        flags = mm.last_result[:, 7]
+        kept = flags == 0.0
+        rel = ((mll_mx - mll).abs() / mll.abs())[kept]
+        line["mixed"] = {"ms_per_step": round(t_mx * 1e3, 2),
+                         "speedup_vs_fp64": round(t / t_mx, 3),
+                         "flagged_problems": int((~kept).sum()),
+                         "max_rel_mll_diff_unflagged_vs_fp64":
+                             float(rel.max()) if rel.numel() else None,
+                         "note": "with any problem flagged, mixed_check='fallback' recomputes "
+                                 "the batch in float64"}
         line["fit"] = {"adam_steps": args.fit_steps, "graph_captured": True,
                        "seconds": round(t_fit, 3),
                        "ms_per_step": round(t_fit / args.fit_steps * 1e3, 2),
@@ -424,6 +460,8 @@ def main() -> int:
                             "gate_mll_1e-6": rel_mll <= 1e-6, "gate_grad_1e-4": rel_g <= 1e-4}
     # the mixed path's own a-posteriori check (Rademacher probes of I - Sigma X^T X)
     line["mixed_check"] = {"probe_frob_norm": probe_m * math.sqrt(args.n),
+                           "sinv_frob_est": float(res_m[_lib.RES_SINV]),
+                           "mll_err_est": float(res_m[_lib.RES_MLL_ERR]),
                            "flag": flag_m, "flag_threshold": 0.75,
                            "meaning": "flag 0: mixed result kept; 1: flagged; 2: recomputed in "
                                       "float64 (then the timed steps are not the mixed path)"}

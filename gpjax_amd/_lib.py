@@ -27,9 +27,11 @@ def mixed_arith(potrf: int, trtri: int, lauum: int) -> int:
         if v not in (0, 3, 4, 6):
             raise ValueError(f"stage products must be 0, 3, 4 or 6, got {v}")
     return PRECISIONS["mixed"] | ARITH_FLAG | (potrf << 12) | (trtri << 16) | (lauum << 20)
-RESULT_LEN = 8
+
+
+RESULT_LEN = 10
 RES_MLL, RES_LOGDET, RES_QUAD, RES_RESID, RES_INFO = 0, 1, 2, 3, 4
-RES_PROBE, RES_TRACE, RES_FLAG = 5, 6, 7
+RES_PROBE, RES_TRACE, RES_FLAG, RES_SINV, RES_MLL_ERR = 5, 6, 7, 8, 9
 
 _c = ctypes
 _P = _c.c_void_p
@@ -61,9 +63,12 @@ SIGNATURES = {
     "gpk_predict_workspace_bytes": (_S, [_L, _L, _L, _I, _I]),
     "gpk_predict": (_I, [_I, _I, _P, _P, _L, _L, _P, _L, _P, _I, _D, _D, _I, _I, _P, _S, _P, _P,
                          _P, _P]),
-    "gpk_batched_workspace_bytes": (_S, [_P, _L, _L]),
-    "gpk_batched_plan": (_I, [_P, _L, _L, _P, _S]),
-    "gpk_batched_mll_grad": (_I, [_I, _P, _P, _P, _L, _L, _P, _I, _D, _I, _P, _S, _P, _P, _P]),
+    "gpk_batched_workspace_bytes": (_S, [_P, _L, _L, _I]),
+    "gpk_batched_plan": (_I, [_P, _L, _L, _I, _P, _S]),
+    "gpk_batched_mll_grad": (_I, [_I, _I, _P, _P, _P, _L, _L, _P, _I, _D, _I, _P, _S, _P, _P,
+                                  _P]),
+    "gpk_batched_tune": (_I, [_I, _I, _P, _P, _P, _L, _L, _P, _I, _D, _I, _P, _S, _P, _P, _P,
+                              _I, _P]),
     "gpk_linalg_workspace_bytes": (_S, [_L, _L, _I]),
     "gpk_potrf": (_I, [_I, _P, _L, _L, _P, _L, _P, _P, _P, _S, _P]),
     "gpk_trsm": (_I, [_P, _L, _L, _P, _L, _L, _P, _S, _P]),

@@ -389,8 +389,11 @@ template int launch_adjoint<float>(int, int, const float*, int64_t, int64_t, con
 namespace {
 template <typename TS>
 __global__ __launch_bounds__(256) void lower_to_upper_kernel(const TS* __restrict__ X,
-                                                             TS* __restrict__ U, int64_t ld) {
+                                                             TS* __restrict__ U, int64_t ld,
+                                                             Batch bt) {
   __shared__ TS tl[64][65];
+  X = bws(X, bt);
+  U = bws(U, bt);
   int64_t I, J;
   lower_tile(blockIdx.x, I, J);
   const int64_t r0 = I * TILE + (blockIdx.y >> 1) * 64, c0 = J * TILE + (blockIdx.y & 1) * 64;
@@ -407,19 +410,21 @@ __global__ __launch_bounds__(256) void lower_to_upper_kernel(const TS* __restric
 }  // namespace
 
 template <typename TS>
-int launch_lower_to_upper(const TS* X, TS* U, int64_t ld, int64_t np, hipStream_t s) {
+int launch_lower_to_upper(const TS* X, TS* U, int64_t ld, int64_t np, hipStream_t s,
+                          const Batch& bt) {
   if (np % TILE) {
     set_error("transpose: size %lld not a multiple of %d", (long long)np, TILE);
     return GPK_ERR_BAD_ARG;
   }
   const int64_t nt = np / TILE;
-  hipLaunchKernelGGL(lower_to_upper_kernel<TS>, dim3((unsigned)(nt * (nt + 1) / 2), 4),
-                     dim3(256), 0, s, X, U, ld);
+  hipLaunchKernelGGL(lower_to_upper_kernel<TS>, dim3((unsigned)(nt * (nt + 1) / 2), 4, bt.count),
+                     dim3(256), 0, s, X, U, ld, bt);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }
 template int launch_lower_to_upper<double>(const double*, double*, int64_t, int64_t,
-                                           hipStream_t);
-template int launch_lower_to_upper<float>(const float*, float*, int64_t, int64_t, hipStream_t);
+                                           hipStream_t, const Batch&);
+template int launch_lower_to_upper<float>(const float*, float*, int64_t, int64_t, hipStream_t,
+                                          const Batch&);
 
 }  // namespace gpk

@@ -16,7 +16,10 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <string>
+#include <tuple>
 #include <type_traits>
 #include <unordered_map>
 #include <vector>
@@ -122,11 +125,18 @@ struct Layout {
   size_t off_r = 0, off_alpha = 0, off_u = 0, off_v = 0, off_res = 0, off_delta = 0;
   size_t off_skip = 0;
   size_t off_dblk = 0;
+  // float32 storage: Sigma's lower NB-tiles in float64, packed, and the
+  // partial sums of their matvec (the refinement's Sigma products)
+  bool has_spk = false;
+  size_t off_spk = 0, off_spart = 0;
   size_t off_part = 0, off_gpart = 0, off_scal = 0, off_probe = 0, total = 0;
 };
 
-Layout make_layout(int64_t n, int64_t d, int prec) {
+// spk: room for Sigma's packed float64 lower tiles (stored_sigma); by default
+// in the mixed path only.
+Layout make_layout(int64_t n, int64_t d, int prec, int spk = -1) {
   Layout L;
+  L.has_spk = spk < 0 ? prec != GPK_PREC_FP64 : spk != 0;
   L.np = round_up(n, NB);
   L.ts = prec == GPK_PREC_FP64 ? sizeof(double) : sizeof(float);
   const int64_t np = L.np;
@@ -150,11 +160,14 @@ Layout make_layout(int64_t n, int64_t d, int prec) {
   L.off_res = o; o += al((size_t)np * sizeof(double));
   L.off_delta = o; o += al((size_t)np * sizeof(double));
   L.off_dblk = o; o += al(prec == GPK_PREC_FP64 ? 0 : (size_t)np * NB * sizeof(double));
+  L.off_spk = o; o += al(L.has_spk ? (size_t)ntile * NB * NB * sizeof(double) : 0);
+  L.off_spart = o;
+  o += al(L.has_spk ? (size_t)2 * L.nvec * ntile * NB * sizeof(double) : 0);
   L.off_part = o; o += al((size_t)L.nvec * nchunk * np * sizeof(double));
   L.off_skip = o; o += al(sizeof(int));
   L.off_gpart = o; o += al((size_t)ntile * (d + 2) * sizeof(double));
   L.off_scal = o; o += al(16 * sizeof(double));
-  L.off_probe = o; o += al(2 * NPROBE * sizeof(double));
+  L.off_probe = o; o += al(3 * NPROBE * sizeof(double));
   L.total = o;
   return L;
 }
@@ -289,58 +302,99 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t v) {
   return v ^ (v >> 31);
 }
 
-__global__ void probe_fill_kernel(double* __restrict__ z, int64_t n, int64_t np, int probe) {
+__global__ void probe_fill_kernel(double* __restrict__ z, int64_t n, int64_t np, int probe,
+                                  Batch bt) {
+  z = bws(z, bt);
+  if (bt.gp) {
+    const int64_t p = bt.gp[blockIdx.z];
+    n = bt.xoff[p + 1] - bt.xoff[p];
+  }
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const uint64_t h = splitmix64(((uint64_t)probe << 40) ^ (uint64_t)i);
   z[i] = i < n ? ((h >> 63) ? -1.0 : 1.0) : 0.0;
 }
 
-// stats[2 probe] = ||z - v||^2, stats[2 probe + 1] = z . (z - v); one workgroup.
+// For probe z with w = X^T X z and v = Sigma w: stats[3 probe] = ||z - v||^2,
+// stats[3 probe + 1] = z . (z - v), stats[3 probe + 2] = ||w||^2; one workgroup.
 __global__ __launch_bounds__(256) void probe_stats_kernel(const double* __restrict__ z,
+                                                          const double* __restrict__ w,
                                                           const double* __restrict__ v,
                                                           int64_t np, int probe,
-                                                          double* __restrict__ stats) {
+                                                          double* __restrict__ stats,
+                                                          Batch bt) {
   __shared__ double red[4];
-  double ee = 0.0, ze = 0.0;
+  z = bws(z, bt);
+  w = bws(w, bt);
+  v = bws(v, bt);
+  stats = bws(stats, bt);
+  double ee = 0.0, ze = 0.0, ww = 0.0;
   for (int64_t i = threadIdx.x; i < np; i += 256) {
     const double e = z[i] - v[i];
     ee += e * e;
     ze += z[i] * e;
+    ww += w[i] * w[i];
   }
   ee = block_sum(ee, red);
   ze = block_sum(ze, red);
+  ww = block_sum(ww, red);
   if (threadIdx.x == 0) {
-    stats[2 * probe] = ee;
-    stats[2 * probe + 1] = ze;
+    stats[3 * probe] = ee;
+    stats[3 * probe + 1] = ze;
+    stats[3 * probe + 2] = ww;
   }
 }
 
 // result[GPK_RES_PROBE] = sqrt(mean ||E z||^2 / n), result[GPK_RES_TRACE] = mean z^T E z,
-// result[GPK_RES_FLAG] = 1 when the estimate of ||E||_F = sqrt(mean ||E z||^2)
-// exceeds PROBE_FLAG_FROB, or when the refinement of alpha stopped above a
-// relative residual of 1e-10.  To first order the trace terms of the gradient
-// carry an error tr(Sigma^-1 E dSigma) <= ||E||_F ||Sigma^-1 dSigma||_F, and
-// log|Sigma| one of about tr E: with ||E||_F near 1 neither is controlled.  The
-// threshold is calibrated on 36 problems (N = 333 .. 65,536, D = 1 .. 16,
-// obs_sd 1e-3 .. 0.3, RBF and Matern-5/2): every one outside the gates
-// (|dMLL|/|MLL| > 1e-6 or gradient > 1e-4 relative to the float64 path) read
-// ||E||_F >= 0.98; the N = 65,536, D = 16 benchmark problem reads 0.40.
+// result[GPK_RES_SINV] ~ ||Sigma^-1||_F, result[GPK_RES_MLL_ERR] = the MLL error
+// model below, and result[GPK_RES_FLAG] = 1 when any of these holds:
+//  * the estimate of ||E||_F = sqrt(mean ||E z||^2) exceeds PROBE_FLAG_FROB.
+//    To first order the trace terms of the gradient carry an error
+//    tr(Sigma^-1 E dSigma) <= ||E||_F ||Sigma^-1 dSigma||_F: with ||E||_F near 1
+//    they are not controlled.  Calibrated on 36 problems (N = 333 .. 65,536,
+//    D = 1 .. 16, obs_sd 1e-3 .. 0.3, RBF and Matern-5/2): every one whose
+//    gradient was outside 1e-4 of the float64 path read ||E||_F >= 0.98; the
+//    N = 65,536, D = 16 benchmark problem reads 0.40.
+//  * MLL_ERR_SAFETY * MLL_ERR > 1e-6 |MLL|.  The log-determinant of the
+//    float32-stored factor is off by tr(Sigma^-1 D) for its backward error D,
+//    |D_ij| ~ 2^-24 max|Sigma| with signs that do not line up, growing with the
+//    recursion depth; the model 2^-25 max|Sigma| ||Sigma^-1||_F sqrt(max(1,
+//    n / 1024)) was fitted to 4,108 problems (the 4,096 of the many-small-GPs
+//    recipe, N = 32 .. 1,024 and noise down to 1e-3, and 12 single problems up
+//    to N = 8,000): where the ||E||_F test above passes, the measured |dMLL|
+//    was at most 8.1 times the model (5.1 times at the N = 65,536 benchmark).
+//  * the refinement of alpha stopped above a relative residual of 1e-10.
 constexpr double PROBE_FLAG_FROB = 0.75;
+constexpr double MLL_ERR_SAFETY = 10.0;
 
 __global__ void probe_finish_kernel(const double* __restrict__ stats, int64_t n,
-                                    double* __restrict__ result) {
+                                    const double* __restrict__ theta, int nls, double jitter,
+                                    double* __restrict__ result, Batch bt) {
   if (threadIdx.x != 0) return;
-  double ee = 0.0, ze = 0.0;
+  stats = bws(stats, bt);
+  if (bt.gp) {
+    const int64_t p = bt.gp[blockIdx.z];
+    n = bt.xoff[p + 1] - bt.xoff[p];
+    result += p * GPK_RESULT_LEN;
+    theta += p * bt.theta_stride;
+  }
+  double ee = 0.0, ze = 0.0, ww = 0.0;
   for (int p = 0; p < NPROBE; ++p) {
-    ee += stats[2 * p];
-    ze += stats[2 * p + 1];
+    ee += stats[3 * p];
+    ze += stats[3 * p + 1];
+    ww += stats[3 * p + 2];
   }
   const double frob = sqrt(ee / NPROBE);
   result[GPK_RES_PROBE] = frob / sqrt((double)n);
   result[GPK_RES_TRACE] = ze / NPROBE;
-  // also flagged: a refinement of alpha that did not reach 1e-10 in its steps
+  // ||Sigma^-1||_F ~ sqrt(E ||Sigma^-1 z||^2) with Sigma^-1 z ~ X^T X z
+  const double sinv = sqrt(ww / NPROBE);
+  const double smax = theta[nls] + jitter + theta[nls + 1] * theta[nls + 1];
+  const double mll_err = 0x1p-25 * smax * sinv * sqrt(fmax(1.0, (double)n / 1024.0));
+  result[GPK_RES_SINV] = sinv;
+  result[GPK_RES_MLL_ERR] = mll_err;
   const bool bad = frob > PROBE_FLAG_FROB || !(frob == frob) ||
+                   !(MLL_ERR_SAFETY * mll_err <= 1e-6 * fabs(result[GPK_RES_MLL])) ||
                    !(result[GPK_RES_RESID] <= 1e-10);
   result[GPK_RES_FLAG] = bad ? 1.0 : 0.0;
 }
@@ -440,11 +494,24 @@ int check_ws(const char* fn, const void* ws, size_t have, size_t need) {
   return GPK_OK;
 }
 
+// Launch variants of one exact-GP evaluation, both exact-arithmetic choices:
+// stored_sigma: the refinement's Sigma products read the packed float64 copy
+//   that the Gram kernel writes (when the layout has room for it) instead of
+//   evaluating the kernel again;
+// adj_transposed: the fused Sigma^-1 pass reads U = X^T along k (one transpose
+//   first) instead of X down its columns.
+// The single-problem path runs the defaults; the batched path takes them per
+// size group from the autotuner's table.
+struct Variant {
+  bool stored_sigma = true;
+  bool adj_transposed = true;
+};
+
 template <typename TS>
 int mll_forward_t(int kind, const Arith3& ar, const double* x, const double* y, int64_t n, int d,
                   const double* theta, int nls, double jitter, int iters, void* ws,
                   const Layout& Lo, double* result, hipStream_t s,
-                  const Batch& bt = Batch()) {
+                  const Batch& bt = Batch(), const Variant& vr = Variant()) {
   const int64_t np = Lo.np;
   const dim3 gv(nblk(np), 1, bt.count);
   TS* L = at<TS>(ws, Lo.off_L);
@@ -467,8 +534,18 @@ int mll_forward_t(int kind, const Arith3& ar, const double* x, const double* y, 
                f64 ? (int)AR_F64 : ar.potrf,
                f64 ? (int)AR_F64 : ar.trtri,
                s};
-  // float32 storage: the diagonal NB-blocks live in float64 on the side
+  // float32 storage: the diagonal NB-blocks live in float64 on the side, and
+  // so does all of Sigma's lower triangle (packed tiles), for the float64
+  // products of the refinement
   double* dblk = f64 ? nullptr : at<double>(ws, Lo.off_dblk);
+  double* spk = (Lo.has_spk && vr.stored_sigma) ? at<double>(ws, Lo.off_spk) : nullptr;
+  // v = Sigma a (nvv vectors): read from the float64 copy when there is one,
+  // otherwise recomputed from the inputs in float64
+  auto sigma_mv = [&](const Batch& b, int nvv) -> int {
+    if (spk)
+      return launch_packed_symv(spk, np, alpha, v, at<double>(ws, Lo.off_spart), s, b, nvv);
+    return launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, alpha, v, part, s, b, nvv);
+  };
   F.dblk = dblk;
   F.bt = bt;
   if (!f64) {
@@ -480,11 +557,11 @@ int mll_forward_t(int kind, const Arith3& ar, const double* x, const double* y, 
 
   hipLaunchKernelGGL(prep_r_kernel, gv, dim3(256), 0, s, y, theta, nls, n, np, r, bt);
   GPK_LAUNCH_CHECK();
-  const bool probes = !f64 && !bt.gp;
+  const bool probes = !f64;
   if (probes) {
     // Rademacher probes z_p of the a-posteriori check, behind r
     for (int p = 0; p < NPROBE; ++p) {
-      hipLaunchKernelGGL(probe_fill_kernel, gv, dim3(256), 0, s, r + (p + 1) * np, n, np, p);
+      hipLaunchKernelGGL(probe_fill_kernel, gv, dim3(256), 0, s, r + (p + 1) * np, n, np, p, bt);
       GPK_LAUNCH_CHECK();
     }
   }
@@ -496,7 +573,7 @@ int mll_forward_t(int kind, const Arith3& ar, const double* x, const double* y, 
   {
     PhaseScope ps(GPK_PHASE_GRAM, s);
     GPK_TRY(launch_kernel_matrix<TS>(kind, x, n, x, n, d, theta, nls, jitter, 2, L, np, np, np,
-                                     true, s, dblk, bt));
+                                     true, s, dblk, bt, spk));
   }
   {
     PhaseScope ps(GPK_PHASE_POTRF, s);
@@ -514,8 +591,7 @@ int mll_forward_t(int kind, const Arith3& ar, const double* x, const double* y, 
   // Refinement: alpha += X^T X (r - Sigma alpha) until ||r - Sigma alpha|| <=
   // 1e-12 ||r|| or `iters` steps; the first Sigma product also forms Sigma w_p.
   for (int it = 0; it < iters; ++it) {
-    GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, alpha, v, part, s,
-                                it == 0 ? bt : bts, it == 0 ? nv : 1));
+    GPK_TRY(sigma_mv(it == 0 ? bt : bts, it == 0 ? nv : 1));
     hipLaunchKernelGGL(sub_kernel, gv, dim3(256), 0, s, r, v, np, res, bts);
     GPK_LAUNCH_CHECK();
     if (it > 0) {
@@ -529,22 +605,22 @@ int mll_forward_t(int kind, const Arith3& ar, const double* x, const double* y, 
     GPK_LAUNCH_CHECK();
   }
   if (probes && iters == 0) {
-    GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, alpha, v, part, s, bt,
-                                nv));
+    GPK_TRY(sigma_mv(bt, nv));
   }
   if (probes) {
     // a-posteriori check of X ~ L^-1: e_p = z_p - Sigma X^T X z_p
     double* stats = at<double>(ws, Lo.off_probe);
     for (int p = 0; p < NPROBE; ++p) {
-      hipLaunchKernelGGL(probe_stats_kernel, dim3(1), dim3(256), 0, s, r + (p + 1) * np,
-                         v + (p + 1) * np, np, p, stats);
+      hipLaunchKernelGGL(probe_stats_kernel, dim3(1, 1, bt.count), dim3(256), 0, s,
+                         r + (p + 1) * np, alpha + (p + 1) * np, v + (p + 1) * np, np, p,
+                         stats, bt);
       GPK_LAUNCH_CHECK();
     }
   }
   // The final residual r - Sigma alpha.  A refinement that stopped early left it
   // in `res` already (alpha has not changed since), so these launches return
   // at once then; otherwise they measure the last correction.
-  GPK_TRY(launch_sigma_matvec(kind, x, n, np, d, theta, nls, jitter, alpha, v, part, s, bts));
+  GPK_TRY(sigma_mv(bts, 1));
   hipLaunchKernelGGL(sub_kernel, gv, dim3(256), 0, s, r, v, np, res, bts);
   GPK_LAUNCH_CHECK();
   hipLaunchKernelGGL(finalize_fwd_kernel, dim3(1, 1, bt.count), dim3(256), 0, s,
@@ -552,8 +628,8 @@ int mll_forward_t(int kind, const Arith3& ar, const double* x, const double* y, 
                      alpha, res, n, np, result, bt);
   GPK_LAUNCH_CHECK();
   if (probes) {
-    hipLaunchKernelGGL(probe_finish_kernel, dim3(1), dim3(64), 0, s,
-                       at<double>(ws, Lo.off_probe), n, result);
+    hipLaunchKernelGGL(probe_finish_kernel, dim3(1, 1, bt.count), dim3(64), 0, s,
+                       at<double>(ws, Lo.off_probe), n, theta, nls, jitter, result, bt);
     GPK_LAUNCH_CHECK();
   }
   return GPK_OK;
@@ -562,17 +638,18 @@ int mll_forward_t(int kind, const Arith3& ar, const double* x, const double* y, 
 template <typename TS>
 int mll_backward_t(int kind, const Arith3& ar, const double* x, int64_t n, int d,
                    const double* theta, int nls, double gbar, void* ws, const Layout& Lo,
-                   double* grad, hipStream_t s, const Batch& bt = Batch()) {
+                   double* grad, hipStream_t s, const Batch& bt = Batch(),
+                   const Variant& vr = Variant()) {
   const int64_t np = Lo.np;
   const bool f64 = std::is_same<TS, double>::value;
   double* gpart = at<double>(ws, Lo.off_gpart);
   const float* fs = f64 ? nullptr : at<float>(ws, Lo.off_scal) + 2;
-  if (!bt.gp) {
+  if (vr.adj_transposed) {
     // The factor L is no longer needed: its buffer takes U = X^T, whose panels
     // the fused pass reads along k (16-byte loads) instead of down columns.
     // (The transpose, HBM-bound, is outside the adjoint phase's timing.)
     TS* U = at<TS>(ws, Lo.off_L);
-    GPK_TRY(launch_lower_to_upper<TS>(at<TS>(ws, Lo.off_X), U, np, np, s));
+    GPK_TRY(launch_lower_to_upper<TS>(at<TS>(ws, Lo.off_X), U, np, np, s, bt));
     PhaseScope ps(GPK_PHASE_ADJOINT, s);
     GPK_TRY(launch_adjoint<TS>(kind, f64 ? (int)AR_F64 : ar.lauum, U, np, np, x, n, d, theta,
                                nls, at<double>(ws, Lo.off_alpha), gpart, s, bt, nullptr, false,
@@ -904,7 +981,21 @@ int check_offsets(const int64_t* off, int64_t count, int64_t d) {
   return GPK_OK;
 }
 
-BatchPlan make_batch_plan(const int64_t* off, int64_t count, int64_t d) {
+// The batched path's `prec`: as gpk_mll_forward's, except the fp16 split (its
+// operand scales come from one problem's hyperparameters).  -1 if refused.
+int batched_prec(int prec, Arith3* ar) {
+  Arith3 a;
+  const int base = decode_prec(prec, &a);
+  if (base < 0 || a.potrf == AR_FP16X3 || a.trtri == AR_FP16X3 || a.lauum == AR_FP16X3) {
+    set_error("batched: prec 0x%x refused (FP64, or mixed with bf16 splits: the fp16 split's "
+              "operand scales are one problem's)", prec);
+    return -1;
+  }
+  if (ar) *ar = a;
+  return base;
+}
+
+BatchPlan make_batch_plan(const int64_t* off, int64_t count, int64_t d, int base) {
   BatchPlan P;
   P.order.resize((size_t)count);
   for (int64_t p = 0; p < count; ++p) P.order[(size_t)p] = (int)p;
@@ -920,7 +1011,7 @@ BatchPlan make_batch_plan(const int64_t* off, int64_t count, int64_t d) {
     g.first = i;
     while (i < count && padded(P.order[(size_t)i]) == g.np) ++i;
     g.count = i - g.first;
-    g.lo = make_layout(g.np, d, GPK_PREC_FP64);
+    g.lo = make_layout(g.np, d, base, 1);  // room for either Sigma-product variant
     g.stride = al(g.lo.total);
     g.off = o;
     o += g.stride * (size_t)g.count;
@@ -936,6 +1027,7 @@ BatchPlan make_batch_plan(const int64_t* off, int64_t count, int64_t d) {
 // and y through that map.
 struct PlanRecord {
   int64_t count = 0, d = 0;
+  int base = GPK_PREC_FP64;
   uint64_t hash = 0;
 };
 std::mutex g_plan_mu;
@@ -951,31 +1043,219 @@ uint64_t hash_offsets(const int64_t* off, int64_t count) {
   return h;
 }
 
-int batched_mll_grad(int kind, const double* x, const double* y, const int64_t* off,
-                     int64_t count, int d, const double* theta, int nls, double jitter, int iters,
-                     void* ws, double* result, double* grad, hipStream_t s) {
-  const BatchPlan P = make_batch_plan(off, count, d);
-  const int* gp = at<int>(ws, P.off_gp);
-  const int64_t* xoff = at<int64_t>(ws, P.off_xoff);
-  for (const BatchGroup& g : P.groups) {
-    if (g.count > 65535) {
-      set_error("batched: %lld problems of padded size %lld exceed one launch",
-                (long long)g.count, (long long)g.np);
-      return GPK_ERR_BAD_ARG;
-    }
-    Batch bt;
-    bt.ws_stride = (int64_t)g.stride;
-    bt.gp = gp + g.first;
-    bt.xoff = xoff;
-    bt.theta_stride = nls + 3;
-    bt.count = (unsigned)g.count;
-    void* wsg = static_cast<char*>(ws) + g.off;
-    // n is taken per member inside the kernels; np fixes the shapes.
-    GPK_TRY(mll_forward_t<double>(kind, Arith3(), x, y, g.np, d, theta, nls, jitter, iters,
-                                  wsg, g.lo, result, s, bt));
-    GPK_TRY(mll_backward_t<double>(kind, Arith3(), x, g.np, d, theta, nls, 1.0, wsg, g.lo, grad,
-                                   s, bt));
+// ---- autotuning of the batched path ----
+// gpk_batched_tune times the launch variants (Variant: 4 combinations) of each
+// size group on the caller's own problems and keeps the fastest in a
+// process-wide table keyed by (device architecture, kernel kind, prec, padded
+// size, member-count class = ceil(log2 members), d), guarded by a mutex;
+// gpk_batched_mll_grad reads it per group.  Groups without an entry run the
+// untuned default (recomputed Sigma products, column reads in the fused pass).
+struct TuneKey {
+  std::string arch;
+  int kind = 0, prec = 0, d = 0, members_class = 0;
+  int64_t np = 0;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(arch, kind, prec, d, members_class, np) <
+           std::tie(o.arch, o.kind, o.prec, o.d, o.members_class, o.np);
   }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, Variant> g_tune;
+constexpr int NVARIANT = 4;
+
+Variant variant_of(int id) {
+  Variant v;
+  v.stored_sigma = (id & 1) != 0;
+  v.adj_transposed = (id & 2) != 0;
+  return v;
+}
+
+// gcnArchName of the current device (cached per device ordinal).
+int current_arch(std::string* arch) {
+  static std::mutex mu;
+  static std::map<int, std::string> names;
+  int dev = 0;
+  GPK_HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(mu);
+  auto it = names.find(dev);
+  if (it == names.end()) {
+    hipDeviceProp_t prop;
+    GPK_HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    it = names.emplace(dev, std::string(prop.gcnArchName)).first;
+  }
+  *arch = it->second;
+  return GPK_OK;
+}
+
+TuneKey tune_key(const std::string& arch, int kind, int prec, const BatchGroup& g, int d) {
+  TuneKey k;
+  k.arch = arch;
+  k.kind = kind;
+  k.prec = prec;
+  k.d = d;
+  k.np = g.np;
+  int c = 0;
+  while (((int64_t)1 << c) < g.count) ++c;
+  k.members_class = c;
+  return k;
+}
+
+Batch group_batch(const BatchPlan& P, const BatchGroup& g, void* ws, int nls) {
+  Batch bt;
+  bt.ws_stride = (int64_t)g.stride;
+  bt.gp = at<int>(ws, P.off_gp) + g.first;
+  bt.xoff = at<int64_t>(ws, P.off_xoff);
+  bt.theta_stride = nls + 3;
+  bt.count = (unsigned)g.count;
+  return bt;
+}
+
+template <typename TS>
+int run_group(int kind, const Arith3& ar, const double* x, const double* y, int d,
+              const double* theta, int nls, double jitter, int iters, void* ws,
+              const BatchPlan& P, const BatchGroup& g, const Variant& vr, double* result,
+              double* grad, hipStream_t s) {
+  if (g.count > 65535) {
+    set_error("batched: %lld problems of padded size %lld exceed one launch",
+              (long long)g.count, (long long)g.np);
+    return GPK_ERR_BAD_ARG;
+  }
+  const Batch bt = group_batch(P, g, ws, nls);
+  void* wsg = static_cast<char*>(ws) + g.off;
+  // n is taken per member inside the kernels; np fixes the shapes.
+  GPK_TRY(mll_forward_t<TS>(kind, ar, x, y, g.np, d, theta, nls, jitter, iters, wsg, g.lo,
+                            result, s, bt, vr));
+  return mll_backward_t<TS>(kind, ar, x, g.np, d, theta, nls, 1.0, wsg, g.lo, grad, s, bt, vr);
+}
+
+template <typename TS>
+int batched_mll_grad(int kind, int prec, const Arith3& ar, const double* x, const double* y,
+                     const int64_t* off, int64_t count, int d, const double* theta, int nls,
+                     double jitter, int iters, void* ws, double* result, double* grad,
+                     hipStream_t s) {
+  const int base = std::is_same<TS, double>::value ? GPK_PREC_FP64 : GPK_PREC_MIXED;
+  const BatchPlan P = make_batch_plan(off, count, d, base);
+  std::string arch;
+  GPK_TRY(current_arch(&arch));
+  for (const BatchGroup& g : P.groups) {
+    Variant vr;
+    vr.stored_sigma = false;
+    vr.adj_transposed = false;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tune.find(tune_key(arch, kind, prec, g, d));
+      if (it != g_tune.end()) vr = it->second;
+    }
+    GPK_TRY(run_group<TS>(kind, ar, x, y, d, theta, nls, jitter, iters, ws, P, g, vr, result,
+                          grad, s));
+  }
+  return GPK_OK;
+}
+
+// Times every variant of every group (one untimed run, then the mean of
+// `reps` runs between two events on the stream) and records the fastest.
+template <typename TS>
+int batched_tune(int kind, int prec, const Arith3& ar, const double* x, const double* y,
+                 const int64_t* off, int64_t count, int d, const double* theta, int nls,
+                 double jitter, int iters, void* ws, double* result, double* grad,
+                 float* report, int report_len, hipStream_t s) {
+  const int base = std::is_same<TS, double>::value ? GPK_PREC_FP64 : GPK_PREC_MIXED;
+  const BatchPlan P = make_batch_plan(off, count, d, base);
+  std::string arch;
+  GPK_TRY(current_arch(&arch));
+  constexpr int reps = 3;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  GPK_HIP_TRY(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    set_error("tune: hipEventCreate failed");
+    return GPK_ERR_HIP;
+  }
+  int rc = GPK_OK;
+  int gi = 0;
+  for (const BatchGroup& g : P.groups) {
+    float ms[NVARIANT] = {};
+    int best = 0;
+    for (int v = 0; v < NVARIANT && rc == GPK_OK; ++v) {
+      const Variant vr = variant_of(v);
+      rc = run_group<TS>(kind, ar, x, y, d, theta, nls, jitter, iters, ws, P, g, vr, result,
+                         grad, s);
+      if (rc == GPK_OK && hipEventRecord(e0, s) != hipSuccess) {
+        set_error("tune: hipEventRecord failed");
+        rc = GPK_ERR_HIP;
+      }
+      for (int r = 0; r < reps && rc == GPK_OK; ++r)
+        rc = run_group<TS>(kind, ar, x, y, d, theta, nls, jitter, iters, ws, P, g, vr, result,
+                           grad, s);
+      if (rc == GPK_OK && (hipEventRecord(e1, s) != hipSuccess ||
+                           hipEventSynchronize(e1) != hipSuccess ||
+                           hipEventElapsedTime(&ms[v], e0, e1) != hipSuccess)) {
+        set_error("tune: event timing failed");
+        rc = GPK_ERR_HIP;
+      }
+      ms[v] /= reps;
+      if (ms[v] < ms[best]) best = v;
+    }
+    if (rc != GPK_OK) break;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tune[tune_key(arch, kind, prec, g, d)] = variant_of(best);
+    }
+    if (report && (gi + 1) * 7 <= report_len) {
+      float* q = report + gi * 7;
+      q[0] = (float)g.np;
+      q[1] = (float)g.count;
+      q[2] = (float)best;
+      for (int v = 0; v < NVARIANT; ++v) q[3 + v] = ms[v];
+    }
+    // leave this group's outputs as the chosen variant computes them
+    rc = run_group<TS>(kind, ar, x, y, d, theta, nls, jitter, iters, ws, P, g, variant_of(best),
+                       result, grad, s);
+    if (rc != GPK_OK) break;
+    ++gi;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+// Argument, workspace and plan checks shared by gpk_batched_mll_grad and
+// gpk_batched_tune (host only, no sync).
+int check_batched_call(const char* fn, int kind, int prec, const double* x, const double* y,
+                       const int64_t* off, int64_t count, int64_t d, const double* theta,
+                       int nls, const void* ws, size_t ws_bytes, const double* result,
+                       const double* grad, Arith3* ar, int* base_out) {
+  int rc = check_offsets(off, count, d);
+  if (rc) return rc;
+  const int base = batched_prec(prec, ar);
+  if (base < 0) return GPK_ERR_BAD_ARG;
+  rc = check_common(kind, prec, x, off[count], d, theta, nls);
+  if (rc) return rc;
+  if (!y || !ws || !result || !grad) {
+    set_error("%s: null y/ws/result/grad", fn);
+    return GPK_ERR_BAD_ARG;
+  }
+  const size_t need = make_batch_plan(off, count, d, base).total;
+  if (ws_bytes < need) {
+    set_error("%s: workspace %zu bytes < required %zu", fn, ws_bytes, need);
+    return GPK_ERR_WORKSPACE;
+  }
+  if (reinterpret_cast<uintptr_t>(ws) % ALIGN) {
+    set_error("%s: workspace must be %zu-byte aligned", fn, ALIGN);
+    return GPK_ERR_BAD_ARG;
+  }
+  std::lock_guard<std::mutex> g(g_plan_mu);
+  auto it = g_plans.find(ws);
+  if (it == g_plans.end()) {
+    set_error("%s: no gpk_batched_plan has run on this workspace", fn);
+    return GPK_ERR_BAD_ARG;
+  }
+  const PlanRecord& r = it->second;
+  if (r.count != count || r.d != d || r.base != base || r.hash != hash_offsets(off, count)) {
+    set_error("%s: the workspace was planned for other offsets/count/d/prec", fn);
+    return GPK_ERR_BAD_ARG;
+  }
+  *base_out = base;
   return GPK_OK;
 }
 
@@ -1224,16 +1504,20 @@ int gpk_loocv_backward(int kind, int prec, const double* x, int64_t n, int64_t d
                                  s);
 }
 
-size_t gpk_batched_workspace_bytes(const int64_t* offsets_host, int64_t count, int64_t d) {
-  if (check_offsets(offsets_host, count, d) != GPK_OK) return 0;
-  return make_batch_plan(offsets_host, count, d).total;
+size_t gpk_batched_workspace_bytes(const int64_t* offsets_host, int64_t count, int64_t d,
+                                   int prec) {
+  const int base = batched_prec(prec, nullptr);
+  if (base < 0 || check_offsets(offsets_host, count, d) != GPK_OK) return 0;
+  return make_batch_plan(offsets_host, count, d, base).total;
 }
 
-int gpk_batched_plan(const int64_t* offsets_host, int64_t count, int64_t d, void* ws,
+int gpk_batched_plan(const int64_t* offsets_host, int64_t count, int64_t d, int prec, void* ws,
                      size_t ws_bytes) {
   int rc = check_offsets(offsets_host, count, d);
   if (rc) return rc;
-  const BatchPlan P = make_batch_plan(offsets_host, count, d);
+  const int base = batched_prec(prec, nullptr);
+  if (base < 0) return GPK_ERR_BAD_ARG;
+  const BatchPlan P = make_batch_plan(offsets_host, count, d, base);
   if (!ws || ws_bytes < P.total) {
     set_error("batched_plan: workspace %zu bytes < required %zu", ws_bytes, P.total);
     return GPK_ERR_WORKSPACE;
@@ -1247,47 +1531,45 @@ int gpk_batched_plan(const int64_t* offsets_host, int64_t count, int64_t d, void
   GPK_HIP_TRY(hipMemcpy(at<int64_t>(ws, P.off_xoff), offsets_host,
                         (size_t)(count + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
   std::lock_guard<std::mutex> g(g_plan_mu);
-  g_plans[ws] = PlanRecord{count, d, hash_offsets(offsets_host, count)};
+  g_plans[ws] = PlanRecord{count, d, base, hash_offsets(offsets_host, count)};
   return GPK_OK;
 }
 
-int gpk_batched_mll_grad(int kind, const double* x, const double* y,
+int gpk_batched_mll_grad(int kind, int prec, const double* x, const double* y,
                          const int64_t* offsets_host, int64_t count, int64_t d,
                          const double* theta, int nls, double jitter, int refine_iters, void* ws,
                          size_t ws_bytes, double* result, double* grad, void* stream) {
-  int rc = check_offsets(offsets_host, count, d);
+  Arith3 ar;
+  int base = 0;
+  int rc = check_batched_call("batched_mll_grad", kind, prec, x, y, offsets_host, count, d,
+                              theta, nls, ws, ws_bytes, result, grad, &ar, &base);
   if (rc) return rc;
-  rc = check_common(kind, GPK_PREC_FP64, x, offsets_host[count], d, theta, nls);
+  const int iters = refine_iters >= 0 ? refine_iters : (base == GPK_PREC_FP64 ? 1 : 6);
+  if (base == GPK_PREC_FP64)
+    return batched_mll_grad<double>(kind, prec, ar, x, y, offsets_host, count, (int)d, theta,
+                                    nls, jitter, iters, ws, result, grad, (hipStream_t)stream);
+  return batched_mll_grad<float>(kind, prec, ar, x, y, offsets_host, count, (int)d, theta, nls,
+                                 jitter, iters, ws, result, grad, (hipStream_t)stream);
+}
+
+int gpk_batched_tune(int kind, int prec, const double* x, const double* y,
+                     const int64_t* offsets_host, int64_t count, int64_t d, const double* theta,
+                     int nls, double jitter, int refine_iters, void* ws, size_t ws_bytes,
+                     double* result, double* grad, float* report_host, int report_len,
+                     void* stream) {
+  Arith3 ar;
+  int base = 0;
+  int rc = check_batched_call("batched_tune", kind, prec, x, y, offsets_host, count, d, theta,
+                              nls, ws, ws_bytes, result, grad, &ar, &base);
   if (rc) return rc;
-  if (!y || !ws || !result || !grad) {
-    set_error("batched_mll_grad: null y/ws/result/grad");
-    return GPK_ERR_BAD_ARG;
-  }
-  const size_t need = make_batch_plan(offsets_host, count, d).total;
-  if (ws_bytes < need) {
-    set_error("batched_mll_grad: workspace %zu bytes < required %zu", ws_bytes, need);
-    return GPK_ERR_WORKSPACE;
-  }
-  if (reinterpret_cast<uintptr_t>(ws) % ALIGN) {
-    set_error("batched_mll_grad: workspace must be %zu-byte aligned", ALIGN);
-    return GPK_ERR_BAD_ARG;
-  }
-  {
-    std::lock_guard<std::mutex> g(g_plan_mu);
-    auto it = g_plans.find(ws);
-    if (it == g_plans.end()) {
-      set_error("batched_mll_grad: no gpk_batched_plan has run on this workspace");
-      return GPK_ERR_BAD_ARG;
-    }
-    const PlanRecord& r = it->second;
-    if (r.count != count || r.d != d || r.hash != hash_offsets(offsets_host, count)) {
-      set_error("batched_mll_grad: the workspace was planned for other offsets/count/d");
-      return GPK_ERR_BAD_ARG;
-    }
-  }
-  const int iters = refine_iters >= 0 ? refine_iters : 1;
-  return batched_mll_grad(kind, x, y, offsets_host, count, (int)d, theta, nls, jitter, iters, ws,
-                          result, grad, (hipStream_t)stream);
+  const int iters = refine_iters >= 0 ? refine_iters : (base == GPK_PREC_FP64 ? 1 : 6);
+  if (base == GPK_PREC_FP64)
+    return batched_tune<double>(kind, prec, ar, x, y, offsets_host, count, (int)d, theta, nls,
+                                jitter, iters, ws, result, grad, report_host, report_len,
+                                (hipStream_t)stream);
+  return batched_tune<float>(kind, prec, ar, x, y, offsets_host, count, (int)d, theta, nls,
+                             jitter, iters, ws, result, grad, report_host, report_len,
+                             (hipStream_t)stream);
 }
 
 int gpk_profile_enable(int on) {

@@ -163,12 +163,14 @@ constexpr int KM_ZERO_PAD = 4;
 // lower_only: only 128-tiles on or below the diagonal (rows_total == cols_total).
 // dblk (optional, lower_only mode): every diagonal NB x NB block is also
 // written in float64 to dblk[b][NB][NB].
+// spk (optional, lower_only mode): every lower NB-tile (I >= J) is also written
+// in float64, packed: tile I(I+1)/2 + J at spk + (I(I+1)/2 + J) * NB * NB.
 template <typename TS>
 int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, int64_t m,
                          int d, const double* theta, int nls, double jitter, int diag_mode,
                          TS* out, int64_t ldo, int64_t rows_total, int64_t cols_total,
                          bool lower_only, hipStream_t s, double* dblk = nullptr,
-                         const Batch& bt = Batch());
+                         const Batch& bt = Batch(), double* spk = nullptr);
 
 // Leaf: factor the NB x NB diagonal block at (o, o) of A in float64 in LDS,
 // write L (lower, zero upper) to A, L^-1 to X (lower, zero upper) and to
@@ -215,7 +217,8 @@ int launch_adjoint(int kind, int arith, const TS* X, int64_t ld, int64_t np, con
 // U = X^T for lower-triangular X (np x np, np a multiple of NB): upper U with
 // zeros below the diagonal inside its diagonal NB-tiles.
 template <typename TS>
-int launch_lower_to_upper(const TS* X, TS* U, int64_t ld, int64_t np, hipStream_t s);
+int launch_lower_to_upper(const TS* X, TS* U, int64_t ld, int64_t np, hipStream_t s,
+                          const Batch& bt = Batch());
 
 // y = X r (trans = false) or y = X^T r (trans = true), X lower triangular
 // np x np (only k <= i is read); float64 accumulation through `partial`
@@ -230,6 +233,12 @@ int launch_trmv(const TS* X, int64_t ld, int64_t np, const double* r, double* y,
 int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
                         const double* theta, int nls, double jitter, const double* a, double* v,
                         double* partial, hipStream_t s, const Batch& bt = Batch(), int nv = 1);
+
+// v = S a for a symmetric S held as its packed lower NB-tiles in float64 (the
+// spk layout of launch_kernel_matrix), np x np; nv = 1 or 3 vectors np apart
+// share each read of S.  partial holds 2 * nv * T(T+1)/2 * NB doubles, T = np/NB.
+int launch_packed_symv(const double* S, int64_t np, const double* a, double* v,
+                       double* partial, hipStream_t s, const Batch& bt = Batch(), int nv = 1);
 
 // y_j = sum_i M[i][j] v_i (v non-null) or sum_i M[i][j]^2 (v null), dense
 // rows x cols (float64 or float32 entries, float64 sums); partial holds

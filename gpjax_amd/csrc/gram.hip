@@ -33,7 +33,8 @@ __global__ __launch_bounds__(256) void kmat_kernel(int kind, const double* __res
                                                    double jitter, int diag_mode, TS* out,
                                                    int64_t ldo, int64_t rows_total,
                                                    int64_t cols_total, int lower_only,
-                                                   double* __restrict__ dblk, Batch bt) {
+                                                   double* __restrict__ dblk,
+                                                   double* __restrict__ spk, Batch bt) {
   __shared__ double xi_s[KT][DCH + 1];
   __shared__ double zj_s[DCH][KT + 1];
   if (bt.gp) {  // batch member: its own inputs (a Gram matrix, z = x)
@@ -43,6 +44,7 @@ __global__ __launch_bounds__(256) void kmat_kernel(int kind, const double* __res
   }
   out = bws(out, bt);
   dblk = bws(dblk, bt);
+  spk = bws(spk, bt);
 
   int64_t ti, tj;
   if (lower_only) {
@@ -119,6 +121,9 @@ __global__ __launch_bounds__(256) void kmat_kernel(int kind, const double* __res
       const int64_t bk = i / NB;
       dblk[bk * NB * NB + (i - bk * NB) * NB + (j - bk * NB)] = v;
     }
+    // The whole lower triangle in float64, NB-tiles packed in blockIdx.x order
+    // (lower_only mode): the matrix of the float64 refinement's products.
+    if (spk) spk[(int64_t)blockIdx.x * NB * NB + (i % NB) * NB + (j % NB)] = v;
   }
 }
 
@@ -128,7 +133,8 @@ template <typename TS>
 int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, int64_t m,
                          int d, const double* theta, int nls, double jitter, int diag_mode,
                          TS* out, int64_t ldo, int64_t rows_total, int64_t cols_total,
-                         bool lower_only, hipStream_t s, double* dblk, const Batch& bt) {
+                         bool lower_only, hipStream_t s, double* dblk, const Batch& bt,
+                         double* spk) {
   if (rows_total <= 0 || cols_total <= 0) return GPK_OK;
   dim3 grid;
   if (lower_only) {
@@ -148,7 +154,7 @@ int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, 
   }
   hipLaunchKernelGGL(kmat_kernel<TS>, grid, dim3(256), 0, s, kind, x, n, z, m, d, theta, nls,
                      jitter, diag_mode, out, ldo, rows_total, cols_total, lower_only ? 1 : 0,
-                     dblk, bt);
+                     dblk, lower_only ? spk : nullptr, bt);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }
@@ -156,10 +162,10 @@ int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, 
 template int launch_kernel_matrix<double>(int, const double*, int64_t, const double*, int64_t,
                                           int, const double*, int, double, int, double*,
                                           int64_t, int64_t, int64_t, bool, hipStream_t, double*,
-                                          const Batch&);
+                                          const Batch&, double*);
 template int launch_kernel_matrix<float>(int, const double*, int64_t, const double*, int64_t,
                                          int, const double*, int, double, int, float*, int64_t,
                                          int64_t, int64_t, bool, hipStream_t, double*,
-                                         const Batch&);
+                                         const Batch&, double*);
 
 }  // namespace gpk

@@ -241,6 +241,122 @@ __global__ void sum_chunks(const double* __restrict__ partial, int64_t np, int64
   }
 }
 
+// ---- v = S a with S symmetric, stored as packed lower NB-tiles in float64 ----
+// One workgroup per tile t = I(I+1)/2 + J (I >= J), read once: row sums
+// S_IJ a_J go to prow[t] and, below the diagonal, column sums S_IJ^T a_I to
+// pcol[t] (layout [NV][ntile][NB]).  Thread (ti, tj) holds rows 8ti .. 8ti+7
+// and columns {32p + 2tj, 32p + 2tj + 1 : p < 4} of the tile, so each 16-byte
+// load instruction of a wave reads four 256-byte row segments.
+template <int NV>
+__global__ __launch_bounds__(256) void packed_symv_tile_kernel(const double* __restrict__ S,
+                                                               int64_t np,
+                                                               const double* __restrict__ a,
+                                                               double* __restrict__ prow,
+                                                               double* __restrict__ pcol,
+                                                               int64_t ntile, Batch bt) {
+  if (bskip(bt)) return;
+  S = bws(S, bt);
+  a = bws(a, bt);
+  prow = bws(prow, bt);
+  pcol = bws(pcol, bt);
+  const int64_t t = blockIdx.x;
+  int64_t I = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while (I * (I + 1) / 2 > t) --I;
+  while ((I + 1) * (I + 2) / 2 <= t) ++I;
+  const int64_t J = t - I * (I + 1) / 2;
+  const bool diag = I == J;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int ti = tid >> 4, tj = tid & 15;
+  const double* tile = S + t * NB * NB;
+  double2 m[8][4];
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+      m[r][p] = *reinterpret_cast<const double2*>(tile + (ti * 8 + r) * NB + 32 * p + 2 * tj);
+  __shared__ double csum[NV][4][NB];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const double* aq = a + q * np;
+    double aj[4][2], ai[8];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      aj[p][0] = aq[J * NB + 32 * p + 2 * tj];
+      aj[p][1] = aq[J * NB + 32 * p + 2 * tj + 1];
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) ai[r] = aq[I * NB + ti * 8 + r];
+    // row sums over this thread's columns, then over the 16 threads of a row group
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      double v = 0.0;
+#pragma unroll
+      for (int p = 0; p < 4; ++p) v += m[r][p].x * aj[p][0] + m[r][p].y * aj[p][1];
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off);
+      if (tj == 0) prow[((int64_t)q * ntile + t) * NB + ti * 8 + r] = v;
+    }
+    if (!diag) {
+      // column sums over this thread's rows, then over the 4 row groups of a
+      // wave (lanes 16 apart) and, through LDS, over the 4 waves
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        double c0 = 0.0, c1 = 0.0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          c0 += m[r][p].x * ai[r];
+          c1 += m[r][p].y * ai[r];
+        }
+        c0 += __shfl_xor(c0, 16);
+        c0 += __shfl_xor(c0, 32);
+        c1 += __shfl_xor(c1, 16);
+        c1 += __shfl_xor(c1, 32);
+        if (lane < 16) {
+          csum[q][w][32 * p + 2 * tj] = c0;
+          csum[q][w][32 * p + 2 * tj + 1] = c1;
+        }
+      }
+    }
+  }
+  if (diag) return;
+  __syncthreads();
+  for (int e = tid; e < NV * NB; e += 256) {
+    const int q = e / NB, j = e % NB;
+    pcol[((int64_t)q * ntile + t) * NB + j] =
+        ((csum[q][0][j] + csum[q][1][j]) + csum[q][2][j]) + csum[q][3][j];
+  }
+}
+
+// v[I*NB + i] = sum_{J <= I} prow[t(I, J)][i] + sum_{K > I} pcol[t(K, I)][i]:
+// one 512-thread workgroup per row block, four interleaved partial sums per
+// entry combined in a fixed order.
+__global__ __launch_bounds__(512) void packed_symv_reduce(const double* __restrict__ prow,
+                                                          const double* __restrict__ pcol,
+                                                          int64_t T, int64_t ntile, int nv,
+                                                          double* __restrict__ v, Batch bt) {
+  if (bskip(bt)) return;
+  prow = bws(prow, bt);
+  pcol = bws(pcol, bt);
+  v = bws(v, bt);
+  const int64_t I = blockIdx.x;
+  const int g = threadIdx.x >> 7, i = threadIdx.x & (NB - 1);
+  __shared__ double red[4][NB];
+  const int64_t np = T * NB;
+  for (int q = 0; q < nv; ++q) {
+    double acc = 0.0;
+#pragma unroll 4
+    for (int64_t e = g; e < T; e += 4) {
+      const int64_t t = e <= I ? I * (I + 1) / 2 + e : e * (e + 1) / 2 + I;
+      const double* src = e <= I ? prow : pcol;
+      acc += src[((int64_t)q * ntile + t) * NB + i];
+    }
+    red[g][i] = acc;
+    __syncthreads();
+    if (g == 0) v[q * np + I * NB + i] = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
+    __syncthreads();
+  }
+}
+
 // y_j partial over rows [c*RC, (c+1)*RC) of a dense rows x cols matrix:
 // sum_i M[i][j] v_i, or sum_i M[i][j]^2 when v is null (rows i >= j only when
 // lower).  One thread per column, float64 sums.
@@ -371,6 +487,37 @@ int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
   GPK_LAUNCH_CHECK();
   hipLaunchKernelGGL(sum_chunks, dim3((unsigned)((np + 255) / 256), 1, bt.count), dim3(256), 0, s,
                      partial, np, nc, nv, v, bt);
+  GPK_LAUNCH_CHECK();
+  return GPK_OK;
+}
+
+int launch_packed_symv(const double* S, int64_t np, const double* a, double* v,
+                       double* partial, hipStream_t s, const Batch& bt, int nv) {
+  if (np % NB) {
+    set_error("packed symv: size %lld not a multiple of %d", (long long)np, NB);
+    return GPK_ERR_BAD_ARG;
+  }
+  if (nv != 1 && nv != 3) {
+    set_error("packed symv: %d right-hand sides (1 or 3 supported)", nv);
+    return GPK_ERR_BAD_ARG;
+  }
+  const int64_t T = np / NB, ntile = T * (T + 1) / 2;
+  if (ntile > 0x7fffffff) {
+    set_error("packed symv: %lld tiles exceed one launch", (long long)ntile);
+    return GPK_ERR_BAD_ARG;
+  }
+  double* prow = partial;
+  double* pcol = partial + (size_t)nv * ntile * NB;
+  const dim3 grid((unsigned)ntile, 1, bt.count);
+  if (nv == 1)
+    hipLaunchKernelGGL(packed_symv_tile_kernel<1>, grid, dim3(256), 0, s, S, np, a, prow, pcol,
+                       ntile, bt);
+  else
+    hipLaunchKernelGGL(packed_symv_tile_kernel<3>, grid, dim3(256), 0, s, S, np, a, prow, pcol,
+                       ntile, bt);
+  GPK_LAUNCH_CHECK();
+  hipLaunchKernelGGL(packed_symv_reduce, dim3((unsigned)T, 1, bt.count), dim3(512), 0, s, prow,
+                     pcol, T, ntile, nv, v, bt);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }

@@ -43,11 +43,17 @@ def mixed_check_says_recompute(res: torch.Tensor, prec_id: int, n: int, what: st
     if torch.cuda.is_current_stream_capturing():
         return False
     info, flag, probe = res[[_lib.RES_INFO, _lib.RES_FLAG, _lib.RES_PROBE]].tolist()
-    if info != 0.0 or flag != 1.0:
+    if info != 0.0:
+        # float32 storage moves Sigma by ~1e-7 relative; a pivot that then turns
+        # non-positive is a precision failure (float64 may still factor it)
+        msg = (f"{what}: the mixed-precision Cholesky broke down at pivot {int(info) - 1} "
+               f"(n = {n})")
+    elif flag == 1.0:
+        msg = (f"{what}: the mixed-precision a-posteriori check flagged this problem "
+               f"(||I - Sigma X^T X||_F ~ {probe * math.sqrt(n):.3g}, n = {n}); its MLL / "
+               "gradient may be outside 1e-6 / 1e-4 of the float64 answer")
+    else:
         return False
-    msg = (f"{what}: the mixed-precision a-posteriori check flagged this problem "
-           f"(||I - Sigma X^T X||_F ~ {probe * math.sqrt(n):.3g}, n = {n}); its MLL / "
-           "gradient may be outside 1e-6 / 1e-4 of the float64 answer")
     if config.mixed_check == "raise":
         raise MixedPrecisionError(msg)
     if config.mixed_check == "warn":

@@ -75,10 +75,16 @@ extern "C" {
  * Rademacher probes z:                                                        */
 #define GPK_RES_PROBE 5    /* sqrt(mean ||E z||^2 / n)  (~ ||E||_F / sqrt(n))     */
 #define GPK_RES_TRACE 6    /* mean z^T E z  (estimates tr E ~ error of log|Sigma|) */
-#define GPK_RES_FLAG 7     /* 1 when ||E||_F > 0.75: the mixed result may be      */
-                           /* outside 1e-6 (MLL) / 1e-4 (gradient); recompute in  */
-                           /* GPK_PREC_FP64                                        */
-#define GPK_RESULT_LEN 8
+#define GPK_RES_FLAG 7     /* 1 when the check says the mixed result may be       */
+                           /* outside 1e-6 (MLL) / 1e-4 (gradient): recompute in  */
+                           /* GPK_PREC_FP64 (see GPK_RES_MLL_ERR)                  */
+#define GPK_RES_SINV 8     /* sqrt(mean ||X^T X z||^2) (~ ||Sigma^-1||_F)          */
+#define GPK_RES_MLL_ERR 9  /* model of |error| of the MLL from float32 storage:  */
+                           /* 2^-25 max|Sigma| ||Sigma^-1||_F sqrt(max(1, n/1024)) */
+                           /* (measured errors <= 8.1x it); flagged when 10x it   */
+                           /* exceeds 1e-6 |MLL|, or ||E||_F > 0.75, or the       */
+                           /* refinement stopped above 1e-10                      */
+#define GPK_RESULT_LEN 10
 
 const char* gpk_version(void);
 const char* gpk_last_error(void);
@@ -259,7 +265,14 @@ int gpk_loocv_backward(int kind, int prec, const double* x, int64_t n, int64_t d
  * rows, targets and hyperparameters.  Each problem's MLL and gradient are those
  * of gpk_mll_forward / gpk_mll_backward: conjugate_mll and its jax.grad
  * (gpjax/objectives.py:36-107; tests/test_objectives.py:83-85) applied per
- * problem, as a vmap over problems would.  float64.
+ * problem, as a vmap over problems would.
+ *
+ * prec: GPK_PREC_FP64, or GPK_PREC_MIXED (float32 storage, bf16-split products,
+ *   float64 diagonal blocks / refinement / log-determinant, and each problem's
+ *   own a-posteriori probe check in result[GPK_RES_PROBE..GPK_RES_FLAG]), or
+ *   GPK_PREC_MIXED_ARITH(p, t, l) with bf16 stages only (the fp16 split scales
+ *   its operands by one problem's hyperparameters, so it is refused here).
+ *   The workspace, the plan and the call must name the same prec.
  *
  * offsets_host: HOST array of count + 1 row offsets (offsets[0] = 0): problem p
  *   owns rows offsets[p] .. offsets[p+1]-1 (at least one) of the concatenated
@@ -268,16 +281,17 @@ int gpk_loocv_backward(int kind, int prec, const double* x, int64_t n, int64_t d
  * Problems are grouped by padded size; one group runs as one set of launches
  * with one problem per blockIdx.z.
  */
-size_t gpk_batched_workspace_bytes(const int64_t* offsets_host, int64_t count, int64_t d);
+size_t gpk_batched_workspace_bytes(const int64_t* offsets_host, int64_t count, int64_t d,
+                                   int prec);
 
 /*
  * Writes the batch's problem map into ws (synchronous host-to-device copies).
  * Call once for a given (offsets, ws) before gpk_batched_mll_grad, outside any
  * HIP-graph capture.  The library remembers, per workspace address, which
- * (offsets, count, d) it planned (a mutex-guarded host table), and
+ * (offsets, count, d, prec) it planned (a mutex-guarded host table), and
  * gpk_batched_mll_grad refuses a workspace planned for anything else.
  */
-int gpk_batched_plan(const int64_t* offsets_host, int64_t count, int64_t d, void* ws,
+int gpk_batched_plan(const int64_t* offsets_host, int64_t count, int64_t d, int prec, void* ws,
                      size_t ws_bytes);
 
 /*
@@ -285,12 +299,37 @@ int gpk_batched_plan(const int64_t* offsets_host, int64_t count, int64_t d, void
  * (slots as gpk_mll_forward), grad [count][nls + 3] (d MLL / d theta in theta's
  * layout), both device.  Launches only (stream-ordered, no host sync, no
  * allocation), so it may be captured in a HIP graph once gpk_batched_plan has
- * run on ws.  refine_iters < 0 selects the default (1).
+ * run on ws.  refine_iters < 0 selects the default (1 for FP64, 6 for mixed;
+ * each problem stops at 1e-12 as in gpk_mll_forward).
  */
-int gpk_batched_mll_grad(int kind, const double* x, const double* y, const int64_t* offsets_host,
-                         int64_t count, int64_t d, const double* theta, int nls, double jitter,
-                         int refine_iters, void* ws, size_t ws_bytes, double* result,
-                         double* grad, void* stream);
+int gpk_batched_mll_grad(int kind, int prec, const double* x, const double* y,
+                         const int64_t* offsets_host, int64_t count, int64_t d,
+                         const double* theta, int nls, double jitter, int refine_iters, void* ws,
+                         size_t ws_bytes, double* result, double* grad, void* stream);
+
+/*
+ * Autotuner of the batched path.  For every size group of the batch it times
+ * the four launch variants of one evaluation -- the refinement's Sigma products
+ * from a stored float64 copy of Sigma or recomputed from the inputs, and the
+ * fused Sigma^-1 pass reading X^T along k (after one transpose) or X down its
+ * columns; all four compute the same quantities in float64-exact order-free
+ * ways -- on these very inputs (one untimed run, then three timed), and keeps
+ * the fastest in a process-wide table keyed by (device architecture, kernel
+ * kind, prec, padded size, ceil(log2 members), d), guarded by a mutex.
+ * gpk_batched_mll_grad then runs each group's entry; untuned groups run the
+ * recomputed / column-read variant.  Host-synchronous (not for graph capture);
+ * same arguments and outputs as gpk_batched_mll_grad, whose result and grad it
+ * leaves as the chosen variants compute them.  report_host (optional): 7 floats
+ * per group, up to report_len floats -- padded size, members, chosen variant
+ * (bit 0: stored Sigma, bit 1: transposed pass), then the four variants' mean
+ * milliseconds.  No counterpart in the reference (XLA compiles one program per
+ * shape).
+ */
+int gpk_batched_tune(int kind, int prec, const double* x, const double* y,
+                     const int64_t* offsets_host, int64_t count, int64_t d, const double* theta,
+                     int nls, double jitter, int refine_iters, void* ws, size_t ws_bytes,
+                     double* result, double* grad, float* report_host, int report_len,
+                     void* stream);
 
 /*
  * Dense linear algebra behind gpjax/linalg/operations.py, on the same

@@ -40,9 +40,10 @@ def test_version_and_workspace_sizes():
     assert b"gfx950" in lib.gpk_version()
     b64 = lib.gpk_mll_workspace_bytes(65536, 16, 0)
     b32 = lib.gpk_mll_workspace_bytes(65536, 16, 1)
-    # two padded N x N factors dominate: 2 * 65536^2 * 8 bytes, half for mixed
+    # two padded N x N factors dominate: 2 * 65536^2 * 8 bytes; mixed: two float32
+    # factors plus Sigma's lower triangle in float64 (one more N^2 * 4 bytes)
     assert 2 * 65536 ** 2 * 8 <= b64 < 2.4 * 65536 ** 2 * 8
-    assert 2 * 65536 ** 2 * 4 <= b32 < 2.6 * 65536 ** 2 * 4
+    assert 3 * 65536 ** 2 * 4 <= b32 < 3.6 * 65536 ** 2 * 4
     assert lib.gpk_mll_workspace_bytes(0, 1, 0) == 0
     assert lib.gpk_predict_workspace_bytes(2000, 1, 500, 0, 1) > lib.gpk_mll_workspace_bytes(2000, 1, 0)
 
@@ -166,10 +167,22 @@ def test_batched_workspace_groups_by_padded_size():
     lib = _lib.lib()
     d = 4
     off = _offsets(10, 290, 10, 256)           # padded sizes 128, 384, 128, 256
-    got = lib.gpk_batched_workspace_bytes(off, 4, d)
-    per = {np_: lib.gpk_mll_workspace_bytes(np_, d, 0) for np_ in (128, 256, 384)}
     meta = 256 + 256                            # member map and row offsets, 256-byte aligned
-    assert got == 2 * per[128] + per[256] + per[384] + meta
+    # each member: the single-problem layout, plus (FP64) room for Sigma's packed
+    # float64 lower tiles and their matvec partials, which the mixed layout has
+    for prec in (0, 1):
+        got = lib.gpk_batched_workspace_bytes(off, 4, d, prec)
+        per = {}
+        for np_ in (128, 256, 384):
+            t = np_ // 128
+            ntile = t * (t + 1) // 2
+            extra = (ntile * 128 * 128 * 8 + 2 * ntile * 128 * 8) if prec == 0 else 0
+            per[np_] = lib.gpk_mll_workspace_bytes(np_, d, prec) + extra
+        assert got == 2 * per[128] + per[256] + per[384] + meta
+    # the fp16 split is refused in a batch (its scales come from one problem)
+    assert lib.gpk_batched_workspace_bytes(off, 4, d, _lib.mixed_arith(6, 3, 6)) > 0
+    assert lib.gpk_batched_workspace_bytes(off, 4, d, _lib.mixed_arith(4, 3, 6)) == 0
+    assert "fp16" in lib.gpk_last_error().decode()
 
 
 @pytest.mark.parametrize("rows,count,d", [
@@ -181,9 +194,9 @@ def test_batched_workspace_groups_by_padded_size():
 def test_batched_validation_without_device(rows, count, d):
     lib = _lib.lib()
     off = _offsets(*rows)
-    assert lib.gpk_batched_workspace_bytes(off, count, d) == 0
+    assert lib.gpk_batched_workspace_bytes(off, count, d, 0) == 0
     dummy = ctypes.c_void_p(0x1000)
-    rc = lib.gpk_batched_mll_grad(0, dummy, dummy, off, count, d, dummy, 1, 1e-6, -1, dummy,
+    rc = lib.gpk_batched_mll_grad(0, 0, dummy, dummy, off, count, d, dummy, 1, 1e-6, -1, dummy,
                                   1 << 40, dummy, dummy, None)
     assert rc == 1
     assert "batched" in lib.gpk_last_error().decode()
@@ -193,18 +206,23 @@ def test_batched_workspace_and_offsets_checks():
     lib = _lib.lib()
     dummy = ctypes.c_void_p(0x1000)
     off = _offsets(5, 7)
-    rc = lib.gpk_batched_plan(off, 2, 3, dummy, 64)
+    rc = lib.gpk_batched_plan(off, 2, 3, 0, dummy, 64)
     assert rc == 5 and "workspace" in lib.gpk_last_error().decode()
-    rc = lib.gpk_batched_mll_grad(0, dummy, dummy, off, 2, 3, dummy, 1, 1e-6, -1, dummy, 64,
+    rc = lib.gpk_batched_mll_grad(0, 0, dummy, dummy, off, 2, 3, dummy, 1, 1e-6, -1, dummy, 64,
                                   dummy, dummy, None)
     assert rc == 5
     bad = _offsets(5, 7)
     bad[0] = 1
-    assert lib.gpk_batched_plan(bad, 2, 3, dummy, 1 << 40) == 1
+    assert lib.gpk_batched_plan(bad, 2, 3, 0, dummy, 1 << 40) == 1
+    assert lib.gpk_batched_plan(off, 2, 3, 99, dummy, 1 << 40) == 1
     # a workspace that no plan has written is refused before any launch
-    rc = lib.gpk_batched_mll_grad(0, dummy, dummy, off, 2, 3, dummy, 1, 1e-6, -1,
+    rc = lib.gpk_batched_mll_grad(0, 0, dummy, dummy, off, 2, 3, dummy, 1, 1e-6, -1,
                                   ctypes.c_void_p(0x7f0000), 1 << 40, dummy, dummy, None)
     assert rc == 1 and "plan" in lib.gpk_last_error().decode()
+    # the autotuner runs the same checks before it launches anything
+    rc = lib.gpk_batched_tune(0, 0, dummy, dummy, off, 2, 3, dummy, 1, 1e-6, -1,
+                              ctypes.c_void_p(0x7f0000), 1 << 40, dummy, dummy, None, 0, None)
+    assert rc == 1 and "batched_tune" in lib.gpk_last_error().decode()
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):

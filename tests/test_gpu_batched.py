@@ -176,3 +176,102 @@ def test_many_small_gps_recipe_against_oracle(cuda):
         assert e_mll <= 1e-10, (p, e_mll)
         assert e_g <= 1e-8, (p, e_g)
     print(f"worst: mll {worst[0]:.2e}, grad {worst[1]:.2e}")
+
+
+def test_batched_mixed_in_gate_or_flagged(cuda):
+    """The mixed path over the many-small-GPs recipe (512 problems, noise down to
+    1e-3): each sampled problem is within 1e-6 (MLL) / 1e-4 (gradient) of the
+    float64 oracle or carries its own a-posteriori flag; the batch-level check
+    recomputes a flagged batch in float64 under mixed_check="fallback"."""
+    w = workloads.batched(num=512)
+    kw = dict(kernel="rbf", lengthscale=w["lengthscale"], variance=w["variance"],
+              obs_stddev=w["obs_stddev"], jitter=w["jitter"])
+    old = gpx.config.mixed_check
+    gpx.config.mixed_check = "off"
+    try:
+        m = gpx.batched.BatchedConjugateMLL(w["Xs"], w["ys"], precision="mixed", **kw)
+        mll = m.mll()
+        mll.sum().backward()
+    finally:
+        gpx.config.mixed_check = old
+    res = m.last_result.cpu().numpy()
+    # a float32-storage breakdown (non-positive pivot) always carries the flag
+    assert np.all(res[res[:, 4] != 0, 7] == 1.0)
+    flags = res[:, 7]
+    got_mll = mll.detach().cpu().numpy()
+    gls = m.lengthscale.value.grad.cpu().numpy()
+    gvar = m.variance.value.grad.cpu().numpy()
+    gsd = m.obs_stddev.value.grad.cpu().numpy()
+    n_in = 0
+    for p in _c5_sample(w, count=24, seed=1):
+        ref = O.conjugate_mll_grad("rbf", w["Xs"][p], w["ys"][p], w["lengthscale"][p],
+                                   w["variance"][p], w["obs_stddev"][p], w["jitter"])
+        got = np.concatenate([gls[p], [gvar[p], gsd[p]]])
+        want = np.concatenate([ref["lengthscale"], [ref["variance"], ref["obs_stddev"]]])
+        e_mll = abs(got_mll[p] - ref["mll"]) / abs(ref["mll"])
+        e_g = np.linalg.norm(got - want) / np.linalg.norm(want)
+        inside = bool(e_mll <= 1e-6 and e_g <= 1e-4)
+        n_in += inside
+        print(f"mixed problem {p}: n={len(w['Xs'][p])} sd={w['obs_stddev'][p]:.2e} "
+              f"rel_mll={e_mll:.2e} rel_grad={e_g:.2e} probe={res[p, 5]:.2e} flag={flags[p]:.0f}")
+        assert inside or flags[p] == 1.0, (p, e_mll, e_g)
+    print(f"flagged {int(flags.sum())} of {len(flags)} ({int((res[:, 4] != 0).sum())} broke "
+          f"down); {n_in} sampled problems inside the gates")
+    # the batch check: a flagged batch comes back as the float64 answer
+    if flags.any():
+        ref64 = gpx.batched.BatchedConjugateMLL(w["Xs"], w["ys"], **kw).mll().detach()
+        with pytest.warns(RuntimeWarning, match="recomputing the batch in float64"):
+            fb = gpx.batched.BatchedConjugateMLL(w["Xs"], w["ys"], precision="mixed", **kw).mll()
+        assert torch.equal(fb.detach(), ref64)
+
+
+def test_batched_mixed_equals_single_problem_mixed(cuda):
+    """A batch on the mixed path gives each problem the single-problem mixed
+    result (same kernels and arithmetic; the single path reads the inverse's
+    panels transposed, so agreement is to float32-product rounding, 1e-9)."""
+    d = 3
+    Xs, ys, ls, var, sd = _problems([40, 128, 300, 700], d, seed=21)
+    old = gpx.config.mixed_check
+    gpx.config.mixed_check = "off"
+    try:
+        m = gpx.batched.BatchedConjugateMLL(Xs, ys, kernel="matern52", lengthscale=ls,
+                                            variance=var, obs_stddev=sd, precision="mixed")
+        mll = m.mll().detach().cpu().numpy()
+        for p in range(len(Xs)):
+            k = gpx.kernels.Matern52(lengthscale=ls[p], variance=var[p])
+            post = gpx.gps.Prior(mean_function=gpx.mean_functions.Zero(), kernel=k,
+                                 jitter=1e-6) * \
+                gpx.likelihoods.Gaussian(num_datapoints=len(Xs[p]), obs_stddev=sd[p])
+            D = gpx.Dataset(X=torch.tensor(Xs[p], device=cuda),
+                            y=torch.tensor(ys[p][:, None], device=cuda))
+            single, res = gpx.objectives.conjugate_mll_with_info(post, D, precision="mixed")
+            assert abs(mll[p] - float(single)) <= 1e-9 * abs(float(single)), p
+            assert m.last_result[p, 7].item() == res[7].item()
+            assert abs(m.last_result[p, 5].item() - res[5].item()) <= 1e-6 * res[5].item() + 1e-15
+    finally:
+        gpx.config.mixed_check = old
+
+
+def test_autotuned_batch_equals_untuned(cuda):
+    """The autotuner times four launch variants per size group (refinement Sigma
+    products stored or recomputed; fused pass on X^T or X) and keeps the fastest;
+    every variant computes the same float64 quantities, so the tuned batch equals
+    the untuned one to 1e-12 in the MLL and the gradient."""
+    d = 4
+    Xs, ys, ls, var, sd = _problems([3, 100, 129, 300, 520, 1000, 1024], d, seed=17)
+    out = []
+    for tune in (False, True):
+        m = gpx.batched.BatchedConjugateMLL(Xs, ys, kernel="matern32", lengthscale=ls,
+                                            variance=var, obs_stddev=sd, autotune=tune)
+        mll = m.mll()
+        mll.sum().backward()
+        out.append((mll.detach().cpu().numpy(), m.lengthscale.value.grad.cpu().numpy(),
+                    m.tune_report))
+    (m0, g0, _), (m1, g1, rep) = out
+    assert [r["padded_size"] for r in rep] == [128, 256, 384, 640, 1024]
+    assert [r["members"] for r in rep] == [2, 1, 1, 1, 2]
+    for r in rep:
+        print(r)
+        assert all(t > 0 for t in r["ms"].values())
+    np.testing.assert_allclose(m1, m0, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(g1, g0, rtol=1e-12, atol=1e-300)
