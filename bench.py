@@ -111,6 +111,61 @@ def timed(fn, steps, warmup):
     return (time.perf_counter() - t0) / steps, out
 
 
+def batched_mixed_line(gpx, w, dev, args, t64, mll64) -> dict:
+    """The many-small-GPs batch on the mixed path (float32 storage, bf16 splits,
+    float64 panels, each problem's own probe check), timed with the check's host
+    read off; compared with the float64 MLLs of the same parameters."""
+    mm = gpx.batched.BatchedConjugateMLL(w["Xs"], w["ys"], kernel="rbf",
+                                         lengthscale=w["lengthscale"], variance=w["variance"],
+                                         obs_stddev=w["obs_stddev"], jitter=w["jitter"],
+                                         device=dev, precision="mixed")
+    params = [mm.lengthscale, mm.variance, mm.obs_stddev]
+
+    def step():
+        for p in params:
+            p.value.grad = None
+        mll = mm.mll()
+        mll.sum().backward()
+        return mll
+
+    old_check = gpx.config.mixed_check
+    gpx.config.mixed_check = "off"
+    try:
+        t_mx, mll_mx = timed(step, args.steps, args.warmup)
+    finally:
+        gpx.config.mixed_check = old_check
+    kept = mm.last_result[:, 7] == 0.0
+    mll_mx, mll64 = mll_mx.detach(), mll64.detach()
+    rel = ((mll_mx - mll64).abs() / mll64.abs())[kept]
+    return {"ms_per_step": round(t_mx * 1e3, 2), "speedup_vs_fp64": round(t64 / t_mx, 3),
+            "flagged_problems": int((~kept).sum()),
+            "max_rel_mll_diff_unflagged_vs_fp64": float(rel.max()) if rel.numel() else None,
+            "note": "with any problem flagged, mixed_check='fallback' recomputes the batch "
+                    "in float64"}
+
+
+def batched_fit_line(gpx, w, dev, args) -> dict:
+    """Training: Adam steps on the summed negative MLL of the batch, one step
+    (objective, backward, fused device Adam) captured in a HIP graph and replayed."""
+    import torch
+    fm = gpx.batched.BatchedConjugateMLL(w["Xs"], w["ys"], kernel="rbf",
+                                         lengthscale=w["lengthscale"], variance=w["variance"],
+                                         obs_stddev=w["obs_stddev"], jitter=w["jitter"],
+                                         device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, hist = gpx.fit(model=fm, objective=lambda mdl, data: -mdl.mll().sum(), train_data=None,
+                      num_iters=args.fit_steps, learning_rate=0.01, capture_graph=True,
+                      warmup=3)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+    hist = hist.cpu()
+    return {"adam_steps": args.fit_steps, "graph_captured": True, "seconds": round(t_fit, 3),
+            "ms_per_step": round(t_fit / args.fit_steps * 1e3, 2),
+            "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
+            "note": "includes 3 eager warm-up steps and the capture"}
+
+
 def other_workload(args) -> dict:
     """The other configurations, one JSON line each (not the headline):
     matern  -- exact MLL + gradient, N = 16,384, D = 8, Matern-5/2 ARD, mixed and FP64;
@@ -150,56 +205,7 @@ def other_workload(args) -> dict:
                              "transposed_pass": r["transposed_pass"],
                              "ms": {k: round(v, 3) for k, v in r["ms"].items()}}
                             for r in m.tune_report]
-        # training: 100 Adam steps on the summed negative MLL, one step captured in a
-        # HIP graph (device Adam, no host sync inside) and replayed
-        fm = gpx.batched.BatchedConjugateMLL(w["Xs"], w["ys"], kernel="rbf",
-                                             lengthscale=w["lengthscale"], variance=w["variance"],
-                                             obs_stddev=w["obs_stddev"], jitter=w["jitter"],
-                                             device=dev)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        _, hist = gpx.fit(model=fm, objective=lambda mdl, data: -mdl.mll().sum(),
-                          train_data=None, num_iters=args.fit_steps, learning_rate=0.01,
-                          capture_graph=True, warmup=3)
-        torch.cuda.synchronize()
-        t_fit = time.perf_counter() - t0
-        hist = hist.cpu()
-        # the same batch on the mixed path (float32 storage, bf16 splits, float64
-        # panels; each problem's probe check), timed with the check's host read off
-        mm = gpx.batched.BatchedConjugateMLL(w["Xs"], w["ys"], kernel="rbf",
-                                             lengthscale=w["lengthscale"], variance=w["variance"],
-                                             obs_stddev=w["obs_stddev"], jitter=w["jitter"],
-                                             device=dev, precision="mixed")
-        mparams = [mm.lengthscale, mm.variance, mm.obs_stddev]
-
-        def mstep():
-            for p in mparams:
-                p.value.grad = None
-            mll = mm.mll()
-            mll.sum().backward()
-            return mll
-
-        old_check = gpx.config.mixed_check
-        gpx.config.mixed_check = "off"
-        try:
-            t_mx, mll_mx = timed(mstep, args.steps, args.warmup)
-        finally:
-            gpx.config.mixed_check = old_check
-        flags = mm.last_result[:, 7]
-        kept = flags == 0.0
-        rel = ((mll_mx - mll).abs() / mll.abs())[kept]
-        line["mixed"] = {"ms_per_step": round(t_mx * 1e3, 2),
-                         "speedup_vs_fp64": round(t / t_mx, 3),
-                         "flagged_problems": int((~kept).sum()),
-                         "max_rel_mll_diff_unflagged_vs_fp64":
-                             float(rel.max()) if rel.numel() else None,
-                         "note": "with any problem flagged, mixed_check='fallback' recomputes "
-                                 "the batch in float64"}
-        line["fit"] = {"adam_steps": args.fit_steps, "graph_captured": True,
-                       "seconds": round(t_fit, 3),
-                       "ms_per_step": round(t_fit / args.fit_steps * 1e3, 2),
-                       "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
-                       "note": "includes 3 eager warm-up steps and the capture"}
+        cpu_n = 32
         line.update(metric="batched MLL+grad evals/s (4096 exact GPs, N_i in 32..1024, D=4)",
                     value=round(1.0 / t, 4), unit="evals/s", ms_per_step=round(t * 1e3, 2),
                     dtype="f64", config={"workload": "batched: 4096 RBF-ARD GPs "
@@ -207,7 +213,9 @@ def other_workload(args) -> dict:
                                          "problems": len(sizes), "sum_n": sum(sizes),
                                          "sum_n3": float(sum(s ** 3 for s in sizes))},
                     all_finite=bool(torch.isfinite(mll).all()))
-        cpu_n = 32
+        if not args.eval_only:
+            line["mixed"] = batched_mixed_line(gpx, w, dev, args, t, mll)
+            line["fit"] = batched_fit_line(gpx, w, dev, args)
     elif args.workload == "sparse":
         w = workloads.sparse()
         k = gpx.kernels.Matern32(lengthscale=w["lengthscale"], variance=w["variance"])
@@ -309,6 +317,9 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=4096)
     ap.add_argument("--fit-steps", type=int, default=100,
                     help="batched workload: graph-captured Adam steps timed after the evaluations")
+    ap.add_argument("--eval-only", action="store_true",
+                    help="batched workload: time the float64 evaluations only (no mixed-path "
+                         "line, no Adam steps), e.g. for a kernel trace")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed-arith", default="6,3,6", metavar="P,T,L",
                     help="bf16 products per element for the Cholesky updates, the "

@@ -6,7 +6,13 @@
  *   - Every pointer argument is a DEVICE pointer unless the name ends in _host.
  *     Buffers are caller-owned; nothing is freed or retained by the library.
  *   - Every call is stream-ordered on `stream` (a hipStream_t passed as void*;
- *     NULL = the legacy default stream) and returns without a host sync.
+ *     NULL = the legacy default stream) and returns without a host sync, except
+ *     the set-up calls gpk_batched_plan and gpk_batched_tune and
+ *     gpk_profile_collect, which synchronise and say so.
+ *   - Process-wide state: the batched plan registry and the autotune table
+ *     (both mutex-guarded host tables), the profiler's event list, and the
+ *     per-device scratch of gpk_mll_forward_noalloc.  Nothing else outlives a
+ *     call; the arithmetic is chosen per call through `prec`.
  *   - Matrices are row-major float64 with an explicit leading dimension.
  *   - Return value: GPK_OK or an error code; gpk_last_error() gives a
  *     thread-local message for the last failing call.
