@@ -577,7 +577,10 @@ int mll_forward_t(int kind, const Arith3& ar, const double* x, const double* y, 
   }
   {
     PhaseScope ps(GPK_PHASE_POTRF, s);
-    GPK_TRY(F.potrf(0, np));
+    if (!bt.gp && LOOKAHEAD_B > 0 && np >= 4 * LOOKAHEAD_B)
+      GPK_TRY(F.potrf_lookahead(np, LOOKAHEAD_B));
+    else
+      GPK_TRY(F.potrf(0, np));
   }
   {
     PhaseScope ps(GPK_PHASE_TRTRI, s);
@@ -1299,6 +1302,28 @@ int device_scratch(void** out) {
 }
 
 }  // namespace
+
+int lookahead_resources(hipStream_t* s2, hipEvent_t* ev3) {
+  struct Res {
+    hipStream_t s = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  };
+  thread_local std::map<int, Res> res;
+  int dev = 0;
+  GPK_HIP_TRY(hipGetDevice(&dev));
+  auto it = res.find(dev);
+  if (it == res.end()) {
+    Res r;
+    GPK_HIP_TRY(hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking));
+    for (int i = 0; i < 3; ++i)
+      GPK_HIP_TRY(hipEventCreateWithFlags(&r.ev[i], hipEventDisableTiming));
+    it = res.emplace(dev, r).first;
+  }
+  *s2 = it->second.s;
+  for (int i = 0; i < 3; ++i) ev3[i] = it->second.ev[i];
+  return GPK_OK;
+}
+
 }  // namespace gpk
 
 using namespace gpk;

@@ -23,6 +23,13 @@ namespace gpk {
 
 inline int64_t split_point(int64_t n) { return (n / NB / 2) * NB; }
 
+// Panel width of the look-ahead Cholesky (Factor::potrf_lookahead); 0 turns it
+// off.  Used for a single problem of at least 4 panels.
+#ifndef GPK_LOOKAHEAD_B
+#define GPK_LOOKAHEAD_B 2048
+#endif
+constexpr int64_t LOOKAHEAD_B = GPK_LOOKAHEAD_B;
+
 template <typename T>
 inline T* at(void* ws, size_t off) {
   return reinterpret_cast<T*>(static_cast<char*>(ws) + off);
@@ -50,6 +57,10 @@ struct Factor {
   // entries of L and the trailing matrix, sX for X = L^-1, sT for L21 X11; the
   // GEMMs read the (A, B) pair at fscale + 0 (POTRF), + 1 and + 2 (TRTRI).
   const float* fscale = nullptr;
+  // Diagonal blocks of this width (at multiples of it) already hold their
+  // inverse in X when trtri runs (potrf_lookahead inverts each panel's
+  // diagonal block on its second stream); NB = only the leaves.
+  int64_t tri_unit = NB;
 
   double* dblk_at(int64_t o) const { return dblk ? dblk + (o / NB) * NB * NB : nullptr; }
   int leaf(int64_t o) {
@@ -91,10 +102,62 @@ struct Factor {
                                     dblk_at(o + h), bt, fscale));
     return potrf(o + h, n - h);
   }
+  // Right-looking over panels of width B with a look-ahead of one panel: while
+  // this stream applies panel j's update to the trailing columns beyond panel
+  // j+1, a second stream factors panel j+1's diagonal block (the recursion's
+  // latency-bound bottom: leaves and small updates) and solves its column
+  // block, so that chain runs under the large update instead of after it.
+  //   stream s : (a) diag block j+1 -= P_j P_j^T   -> ev0
+  //              (b) column block j+1 -= P_j P_j^T -> ev1
+  //              (c) columns beyond j+1 -= ...      (overlaps s2)
+  //              wait ev2
+  //   stream s2: wait ev0; potrf(diag j+1); wait ev1; trsm(column block j+1); -> ev2
+  // The operations and their arithmetic are the recursion's; only the order of
+  // the trailing updates differs (rank-B steps instead of halving splits).
+  int potrf_lookahead(int64_t n, int64_t B) {
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev[3];
+    GPK_TRY(lookahead_resources(&s2, ev));
+    Factor<TS> G = *this;
+    G.s = s2;
+    GPK_TRY(potrf(0, B));
+    GPK_TRY(trtri(0, B));
+    GPK_TRY(trsm(B, n - B, 0, B));
+    for (int64_t o = 0; o + B < n; o += B) {
+      const int64_t o1 = o + B;
+      const int64_t b1 = n - o1 < B ? n - o1 : B;
+      const int64_t o2 = o1 + b1;
+      const int64_t rest = n - o2;
+      TS* P = L + o1 * ld + o;  // panel j below its diagonal block: (n - o1) x B
+      GPK_TRY(launch_gemm<TS, TS, TS>(ar_potrf, false, false, P, ld, P, ld, L + o1 * ld + o1, ld,
+                                      b1, b1, B, -1.0, true, true, KR_FULL, s, dblk_at(o1), bt,
+                                      fscale));
+      GPK_HIP_TRY(hipEventRecord(ev[0], s));
+      if (rest > 0)
+        GPK_TRY(launch_gemm<TS, TS, TS>(ar_potrf, false, false, P + b1 * ld, ld, P, ld,
+                                        L + o2 * ld + o1, ld, rest, b1, B, -1.0, true, false,
+                                        KR_FULL, s, nullptr, bt, fscale));
+      GPK_HIP_TRY(hipEventRecord(ev[1], s));
+      GPK_HIP_TRY(hipStreamWaitEvent(s2, ev[0], 0));
+      GPK_TRY(G.potrf(o1, b1));
+      GPK_TRY(G.trtri(o1, b1));
+      GPK_HIP_TRY(hipStreamWaitEvent(s2, ev[1], 0));
+      if (rest > 0) GPK_TRY(G.trsm(o2, rest, o1, b1));
+      GPK_HIP_TRY(hipEventRecord(ev[2], s2));
+      if (rest > 0)
+        GPK_TRY(launch_gemm<TS, TS, TS>(ar_potrf, false, false, P + b1 * ld, ld, P + b1 * ld,
+                                        ld, L + o2 * ld + o2, ld, rest, rest, B, -1.0, true, true,
+                                        KR_FULL, s, dblk_at(o2), bt, fscale));
+      GPK_HIP_TRY(hipStreamWaitEvent(s, ev[2], 0));
+    }
+    tri_unit = B;
+    return GPK_OK;
+  }
   // X = L^-1 below the leaves, whose diagonal blocks the leaves already wrote.
   int trtri(int64_t o, int64_t n) {
-    if (n == NB) return GPK_OK;
-    const int64_t h = split_point(n);
+    if (n <= tri_unit) return GPK_OK;
+    const int64_t hu = n / tri_unit / 2;
+    const int64_t h = (hu > 0 ? hu : 1) * tri_unit;
     GPK_TRY(trtri(o, h));
     GPK_TRY(trtri(o + h, n - h));
     // T = L21 X11   ((n-h) x h; X11 lower, so k >= column tile)

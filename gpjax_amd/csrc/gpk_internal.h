@@ -234,6 +234,10 @@ int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
                         const double* theta, int nls, double jitter, const double* a, double* v,
                         double* partial, hipStream_t s, const Batch& bt = Batch(), int nv = 1);
 
+// The look-ahead Cholesky's second stream and three events for the calling
+// thread on the current device (created on first use, kept for the process).
+int lookahead_resources(hipStream_t* s2, hipEvent_t* ev3);
+
 // v = S a for a symmetric S held as its packed lower NB-tiles in float64 (the
 // spk layout of launch_kernel_matrix), np x np; nv = 1 or 3 vectors np apart
 // share each read of S.  partial holds 2 * nv * T(T+1)/2 * NB doubles, T = np/NB.
