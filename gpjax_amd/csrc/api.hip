@@ -1046,6 +1046,11 @@ uint64_t hash_offsets(const int64_t* off, int64_t count) {
   return h;
 }
 
+#ifndef GPK_BATCH_STREAMS
+#define GPK_BATCH_STREAMS 4
+#endif
+constexpr int BATCH_STREAMS = GPK_BATCH_STREAMS;
+
 // ---- autotuning of the batched path ----
 // gpk_batched_tune times the launch variants (Variant: 4 combinations) of each
 // size group on the caller's own problems and keeps the fastest in a
@@ -1140,7 +1145,47 @@ int batched_mll_grad(int kind, int prec, const Arith3& ar, const double* x, cons
   const BatchPlan P = make_batch_plan(off, count, d, base);
   std::string arch;
   GPK_TRY(current_arch(&arch));
-  for (const BatchGroup& g : P.groups) {
+  const int ng = (int)P.groups.size();
+  // The size groups are independent (disjoint workspaces and outputs): they
+  // run on up to BATCH_STREAMS streams -- the caller's and library-owned ones,
+  // forked from and joined back to the caller's by events -- so one group's
+  // latency-bound launches overlap another's large ones.  Groups are dealt
+  // largest first (count * np^3) to the least loaded stream.
+  const int ns = std::min(std::min(BATCH_STREAMS, 1 + AUX_STREAMS), ng);
+  hipStream_t st[1 + AUX_STREAMS] = {s};
+  hipEvent_t ev[AUX_EVENTS] = {};
+  std::vector<int> on(ng, 0);
+  if (ns > 1) {
+    GPK_TRY(aux_streams(st + 1, ev));
+    std::vector<int> order(ng);
+    for (int i = 0; i < ng; ++i) order[i] = i;
+    auto cost = [&](int i) {
+      const double np = (double)P.groups[i].np;
+      return (double)P.groups[i].count * np * np * np;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    std::vector<double> load(ns, 0.0);
+    for (int i : order) {
+      const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      on[i] = k;
+      load[k] += cost(i);
+    }
+    GPK_HIP_TRY(hipEventRecord(ev[3], s));
+    for (int k = 1; k < ns; ++k) GPK_HIP_TRY(hipStreamWaitEvent(st[k], ev[3], 0));
+  }
+  // joins every stream used back into the caller's (an early error return included)
+  struct JoinAll {
+    hipStream_t* st;
+    hipEvent_t* ev;
+    int ns;
+    ~JoinAll() {
+      for (int k = 1; k < ns; ++k)
+        if (hipEventRecord(ev[3 + k], st[k]) == hipSuccess)
+          (void)hipStreamWaitEvent(st[0], ev[3 + k], 0);
+    }
+  } join{st, ev, ns};
+  for (int i = 0; i < ng; ++i) {
+    const BatchGroup& g = P.groups[i];
     Variant vr;
     vr.stored_sigma = false;
     vr.adj_transposed = false;
@@ -1150,7 +1195,7 @@ int batched_mll_grad(int kind, int prec, const Arith3& ar, const double* x, cons
       if (it != g_tune.end()) vr = it->second;
     }
     GPK_TRY(run_group<TS>(kind, ar, x, y, d, theta, nls, jitter, iters, ws, P, g, vr, result,
-                          grad, s));
+                          grad, st[on[i]]));
   }
   return GPK_OK;
 }
@@ -1303,24 +1348,42 @@ int device_scratch(void** out) {
 
 }  // namespace
 
-int lookahead_resources(hipStream_t* s2, hipEvent_t* ev3) {
-  struct Res {
-    hipStream_t s = nullptr;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-  };
-  thread_local std::map<int, Res> res;
+struct AuxRes {
+  hipStream_t s[AUX_STREAMS] = {};
+  hipEvent_t ev[AUX_EVENTS] = {};
+};
+
+int aux_resources(AuxRes** out) {
+  thread_local std::map<int, AuxRes> res;
   int dev = 0;
   GPK_HIP_TRY(hipGetDevice(&dev));
   auto it = res.find(dev);
   if (it == res.end()) {
-    Res r;
-    GPK_HIP_TRY(hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking));
-    for (int i = 0; i < 3; ++i)
+    AuxRes r;
+    for (int i = 0; i < AUX_STREAMS; ++i)
+      GPK_HIP_TRY(hipStreamCreateWithFlags(&r.s[i], hipStreamNonBlocking));
+    for (int i = 0; i < AUX_EVENTS; ++i)
       GPK_HIP_TRY(hipEventCreateWithFlags(&r.ev[i], hipEventDisableTiming));
     it = res.emplace(dev, r).first;
   }
-  *s2 = it->second.s;
-  for (int i = 0; i < 3; ++i) ev3[i] = it->second.ev[i];
+  *out = &it->second;
+  return GPK_OK;
+}
+
+int aux_streams(hipStream_t* streams, hipEvent_t* events) {
+  AuxRes* r = nullptr;
+  GPK_TRY(aux_resources(&r));
+  for (int i = 0; i < AUX_STREAMS; ++i) streams[i] = r->s[i];
+  for (int i = 0; i < AUX_EVENTS; ++i) events[i] = r->ev[i];
+  return GPK_OK;
+}
+
+// The look-ahead Cholesky uses aux stream 0 and events 0..2.
+int lookahead_resources(hipStream_t* s2, hipEvent_t* ev3) {
+  AuxRes* r = nullptr;
+  GPK_TRY(aux_resources(&r));
+  *s2 = r->s[0];
+  for (int i = 0; i < 3; ++i) ev3[i] = r->ev[i];
   return GPK_OK;
 }
 

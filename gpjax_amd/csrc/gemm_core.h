@@ -126,7 +126,12 @@ constexpr int BF_BK = 32;
 // one chunk for the fragment reads, four rows of four chunks for the staging
 // stores -- touch sixteen different 16-byte bank groups.
 constexpr bool BF_SWZ = GPK_BF_SWIZZLE != 0;
-constexpr int BF_LDK = BF_SWZ ? BF_BK : BF_BK + 16;  // row length (bf16 elements)
+// Padding of the unswizzled rows (bf16 elements): 16 gives 96-byte rows.
+#ifndef GPK_BF_PAD
+#define GPK_BF_PAD 16
+#endif
+constexpr int BF_LDK = BF_SWZ ? BF_BK : BF_BK + GPK_BF_PAD;  // row length (bf16 elements)
+static_assert(BF_LDK % 8 == 0, "bf16 LDS rows must stay 16-byte aligned");
 __device__ __forceinline__ int bf_off(int row, int chunk) {
   return row * BF_LDK + 8 * (BF_SWZ ? (chunk ^ ((row >> 2) & 3)) : chunk);
 }

@@ -234,8 +234,14 @@ int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
                         const double* theta, int nls, double jitter, const double* a, double* v,
                         double* partial, hipStream_t s, const Batch& bt = Batch(), int nv = 1);
 
-// The look-ahead Cholesky's second stream and three events for the calling
-// thread on the current device (created on first use, kept for the process).
+// Library-owned streams (default priority, non-blocking) and events for the
+// calling thread on the current device, created on first use and kept for the
+// process.  The look-ahead Cholesky takes stream 0 and events 0..2 (never
+// inside a batched call); the batched driver takes the streams and events
+// 3..3+AUX_STREAMS.
+constexpr int AUX_STREAMS = 3;
+constexpr int AUX_EVENTS = 8;
+int aux_streams(hipStream_t* streams, hipEvent_t* events);
 int lookahead_resources(hipStream_t* s2, hipEvent_t* ev3);
 
 // v = S a for a symmetric S held as its packed lower NB-tiles in float64 (the

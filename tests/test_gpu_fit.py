@@ -121,8 +121,10 @@ def test_mll_training_matches_oracle_driven_adam(cuda):
     assert h[-1] < h[0]
 
 
-@pytest.mark.parametrize("n", [256, 700])
+@pytest.mark.parametrize("n", [256, 700, 8192])
 def test_fit_graph_replay_matches_eager(cuda, n):
+    """n = 8,192 takes the look-ahead Cholesky (a second stream joined by events),
+    so the captured graph holds that fork and join."""
     w = workloads.exact_small(n=n)
     D = gpx.Dataset(X=torch.tensor(w["X"], device=cuda), y=torch.tensor(w["y"][:, None], device=cuda))
     steps = 12

@@ -297,3 +297,63 @@ def test_wide_inputs_parity(cuda, n, d, kind, ard, precision):
     rel_g = np.linalg.norm(g - gr) / np.linalg.norm(gr)
     print(f"wide d={d} {kind} {precision}: rel_mll={rel_m:.2e} rel_grad={rel_g:.2e}")
     assert rel_m <= tm and rel_g <= tg
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lookahead_path_graph_replay(cuda, monkeypatch, precision):
+    """N = 9,000 (padded 9,088 >= 4 panels of 2,048) runs the look-ahead Cholesky,
+    whose second stream forks from and joins the calling stream through events.
+    One forward + backward captured in a HIP graph and replayed three times, with
+    eager evaluations in between, equals the eager result every time, and the
+    eager result is within the gates of the oracle."""
+    monkeypatch.setattr(gpx.config, "mixed_check", "off")
+    rng = np.random.default_rng(5)
+    n, d = 9000, 3
+    X = rng.uniform(-3, 3, (n, d))
+    y = np.sin(X[:, 0]) * np.cos(X[:, 1]) + 0.1 * rng.standard_normal(n)
+    post, D, k = _model(cuda, X, y, "matern52", np.array([0.8, 1.1, 1.4]), 1.2, 0.2)
+
+    def step():
+        k.lengthscale.value.grad = None
+        mll = gpx.objectives.conjugate_mll(post, D, precision=precision)
+        mll.backward()
+        return mll.detach().clone(), k.lengthscale.value.grad.detach().clone()
+
+    eager = step()
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step()
+    torch.cuda.current_stream().wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        out = step()
+    for _ in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out[0], eager[0]), (out[0], eager[0])
+        assert torch.equal(out[1], eager[1])
+        again = step()
+        torch.cuda.synchronize()
+        assert torch.equal(again[0], eager[0])
+    _, res = gpx.objectives.conjugate_mll_with_info(post, D, precision=precision)
+    ref = _lookahead_oracle(X, y)
+    tm, tg = (1e-10, 1e-8) if precision == "fp64" else (1e-6, 1e-4)
+    gl = eager[1].cpu().numpy()
+    rel_m = abs(float(eager[0]) - ref["mll"]) / abs(ref["mll"])
+    rel_g = np.linalg.norm(gl - ref["lengthscale"]) / np.linalg.norm(ref["lengthscale"])
+    print(f"look-ahead {precision}: rel_mll={rel_m:.2e} rel_grad(l)={rel_g:.2e} "
+          f"flag={float(res[_lib.RES_FLAG]):.0f}")
+    assert (rel_m <= tm and rel_g <= tg) or (precision == "mixed" and res[_lib.RES_FLAG] == 1)
+
+
+_LA_ORACLE = {}
+
+
+def _lookahead_oracle(X, y):
+    """The float64 oracle at N = 9,000 takes ~20 s on the host: computed once."""
+    if "ref" not in _LA_ORACLE:
+        _LA_ORACLE["ref"] = O.conjugate_mll_grad("matern52", X, y, np.array([0.8, 1.1, 1.4]),
+                                                 1.2, 0.2)
+    return _LA_ORACLE["ref"]
