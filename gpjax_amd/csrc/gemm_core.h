@@ -48,20 +48,43 @@ constexpr int TILE = 128;
 // Output accumulators of every main loop are float64.  In the bf16 loops the
 // matrix cores accumulate in fp32 over short k-chunks only (see
 // mainloop_bf16); each chunk is then added into float64.
+// X3_WIDE: the BF16x3 loop (the triangular inverse) with 64 x 64 per wave and
+// four waves per tile, its three products in one fp32 accumulator set over
+// the whole k-range (no float64 set, so the wide wave still fits two waves per
+// SIMD): half the LDS fragment bytes per product of the 64 x 32 form.
+#ifndef GPK_X3_WIDE
+#define GPK_X3_WIDE 0
+#endif
 template <int AR>
+constexpr bool x3_wide() { return GPK_X3_WIDE != 0 && AR == AR_BF16X3; }
+
+template <int AR, bool F32 = x3_wide<AR>()>
 struct AccT {
   typedef f64x4 vec;
   typedef double scalar;
+};
+template <int AR>
+struct AccT<AR, true> {
+  typedef f32x4 vec;
+  typedef float scalar;
 };
 
 // Waves of one tile: NT threads, NW waves of which WN along n; each wave owns
 // rows wm*64 .. +64 (four 16-row blocks) and columns wn*WC .. +WC (NJ 16-column
 // blocks), wm = w / WN, wn = w % WN.
-template <int AR>
+template <int AR, bool WIDE = x3_wide<AR>()>
 struct WaveLayout {
   static constexpr int NT = 512;
   static constexpr int NW = NT / 64;
   static constexpr int WN = 4;
+  static constexpr int WC = TILE / WN;
+  static constexpr int NJ = WC / 16;
+};
+template <int AR>
+struct WaveLayout<AR, true> {
+  static constexpr int NT = 256;
+  static constexpr int NW = NT / 64;
+  static constexpr int WN = 2;
   static constexpr int WC = TILE / WN;
   static constexpr int NJ = WC / 16;
 };
@@ -117,23 +140,27 @@ constexpr int F64_BK = 16;
 constexpr int F64_LDP = TILE + 1;  // padded row length (doubles)
 constexpr size_t F64_SMEM = 2 /*A,B*/ * 2 /*buffers*/ * F64_BK * F64_LDP * sizeof(double);
 constexpr int BF_BK = 32;
+// LDS rows of the bf16 planes, chosen per arithmetic (measured on one box,
+// same tree, two runs each): padded 96-byte rows for BF16x6 and FP16x3, and for
+// the BF16x3 loop (the triangular inverse) unpadded 64-byte rows whose four
+// 16-byte chunks are permuted by chunk ^ ((row >> 2) & 3), so that sixteen lanes
+// reading (or writing) 16 bytes each -- sixteen rows of one chunk for the
+// fragment reads, four rows of four chunks for the staging stores -- touch
+// sixteen different 16-byte bank groups.  Swizzling the x3 loop took the inverse
+// from 338 to 330 ms; swizzling the x6 loops made the fused Sigma^-1 pass 16 ms
+// slower, and 80-byte padded rows were slower for both.
 #ifndef GPK_BF_SWIZZLE
-#define GPK_BF_SWIZZLE 0
+#define GPK_BF_SWIZZLE 1  // 1: the BF16x3 loop only; 2: every bf16 loop; 0: none
 #endif
-// LDS rows of the bf16 planes.  Padded: 96-byte rows.  Swizzled: unpadded
-// 64-byte rows whose four 16-byte chunks are permuted by chunk ^ ((row >> 2) & 3),
-// so that sixteen lanes reading (or writing) 16 bytes each -- sixteen rows of
-// one chunk for the fragment reads, four rows of four chunks for the staging
-// stores -- touch sixteen different 16-byte bank groups.
-constexpr bool BF_SWZ = GPK_BF_SWIZZLE != 0;
-// Padding of the unswizzled rows (bf16 elements): 16 gives 96-byte rows.
-#ifndef GPK_BF_PAD
-#define GPK_BF_PAD 16
-#endif
-constexpr int BF_LDK = BF_SWZ ? BF_BK : BF_BK + GPK_BF_PAD;  // row length (bf16 elements)
-static_assert(BF_LDK % 8 == 0, "bf16 LDS rows must stay 16-byte aligned");
+template <int AR>
+constexpr bool bf_swz() {
+  return GPK_BF_SWIZZLE == 2 || (GPK_BF_SWIZZLE == 1 && AR == AR_BF16X3);
+}
+template <int AR>
+constexpr int bf_ldk() { return bf_swz<AR>() ? BF_BK : BF_BK + 16; }  // row length (bf16)
+template <int AR>
 __device__ __forceinline__ int bf_off(int row, int chunk) {
-  return row * BF_LDK + 8 * (BF_SWZ ? (chunk ^ ((row >> 2) & 3)) : chunk);
+  return row * bf_ldk<AR>() + 8 * (bf_swz<AR>() ? (chunk ^ ((row >> 2) & 3)) : chunk);
 }
 constexpr int BF_CHUNK = 4;        // slabs per fp32 chunk: 128 k
 template <int AR>
@@ -153,7 +180,7 @@ __device__ __forceinline__ f32x4 mfma32(uint4 a, uint4 b, f32x4 c) {
 template <int AR>
 constexpr size_t smem_bytes() {
   return AR == AR_F64 ? F64_SMEM
-                      : (size_t)2 /*buffers*/ * 2 /*A,B*/ * bf_planes<AR>() * TILE * BF_LDK *
+                      : (size_t)2 /*buffers*/ * 2 /*A,B*/ * bf_planes<AR>() * TILE * bf_ldk<AR>() *
                             sizeof(unsigned short);
 }
 
@@ -272,44 +299,54 @@ __device__ __forceinline__ void mainloop_f64(const TA* __restrict__ A, int64_t l
 // over 256 k, the rounding of the running sums dominated the error of the
 // whole mixed-precision factorisation (float64 products of the same float32
 // operands are ~1000x closer to the float64 result).
-template <int AR, bool A_KM, bool B_KN, bool CFLUSH = false, bool DEEP = true>
+template <int AR, bool A_KM, bool B_KN, bool CFLUSH = false, bool DEEP = !x3_wide<AR>()>
 __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64_t lda,
                                               const float* __restrict__ B, int64_t ldb,
                                               int64_t m0, int64_t n0, int64_t kb, int64_t ke,
-                                              f64x4 (&acc64)[4][WaveLayout<AR>::NJ],
+                                              typename AccT<AR>::vec (&acc64)[4][WaveLayout<AR>::NJ],
                                               char* smem, float sa = 1.f, float sb = 1.f) {
   typedef WaveLayout<AR> WL;
+  typedef typename AccT<AR>::vec AV;
   constexpr int NJ = WL::NJ;
   constexpr int NP = bf_planes<AR>();
-  constexpr int OPS = NP * TILE * BF_LDK;  // bf16 elements of one operand in one buffer
+  constexpr int LDK = bf_ldk<AR>();
+  constexpr int OPS = NP * TILE * LDK;  // bf16 elements of one operand in one buffer
+  constexpr int R = 512 / WL::NT;       // staging units (8 k of one row) per thread
+  constexpr bool WIDE = x3_wide<AR>();
   unsigned short* S = reinterpret_cast<unsigned short*>(smem);  // [2][A, B][NP][TILE][LDK]
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w / WL::WN, wn = w % WL::WN;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) acc64[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < NJ; ++j) acc64[i][j] = AV{0, 0, 0, 0};
   if (kb >= ke) return;
 
-  // Every thread stages 8 consecutive k of one row of each operand per slab.
-  // Row-contiguous operand: row = t/4, k = (t%4)*8 .. +7 (two 16-byte loads).
-  // Column-contiguous operand: row = t%128, k = (t/128)*8 .. +7 (eight
+  // A slab (128 rows x 32 k of each operand) is 512 staging units of 8
+  // consecutive k of one row; thread t takes units t + q NT, q < R.
+  // Row-contiguous operand: unit e -> row e/4, k = (e%4)*8 .. +7 (two 16-byte
+  // loads).  Column-contiguous operand: row e%128, k = (e/128)*8 .. +7 (eight
   // 4-byte loads; consecutive lanes read consecutive rows, 256 bytes per load).
   constexpr int NSET = DEEP ? 2 : 1;
-  float ra[NSET][8], rb[NSET][8];
+  float ra[NSET][R][8], rb[NSET][R][8];
   auto gload_one = [&](const float* __restrict__ P, int64_t ld, int64_t r0, bool km,
-                       int64_t k0, float (&r)[8]) {
-    if (!km) {
-      const float* p = P + (r0 + (t >> 2)) * ld + k0 + (t & 3) * 8;
+                       int64_t k0, float (&rr)[R][8]) {
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const float4 v = *reinterpret_cast<const float4*>(p + 4 * q);
-        r[4 * q] = v.x; r[4 * q + 1] = v.y; r[4 * q + 2] = v.z; r[4 * q + 3] = v.w;
+    for (int q = 0; q < R; ++q) {
+      const int e = t + q * WL::NT;
+      float (&r)[8] = rr[q];
+      if (!km) {
+        const float* p = P + (r0 + (e >> 2)) * ld + k0 + (e & 3) * 8;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const float4 v = *reinterpret_cast<const float4*>(p + 4 * h);
+          r[4 * h] = v.x; r[4 * h + 1] = v.y; r[4 * h + 2] = v.z; r[4 * h + 3] = v.w;
+        }
+      } else {
+        const float* p = P + (k0 + (e >> 7) * 8) * ld + r0 + (e & 127);
+#pragma unroll
+        for (int h = 0; h < 8; ++h) r[h] = p[h * ld];
       }
-    } else {
-      const float* p = P + (k0 + (t >> 7) * 8) * ld + r0 + (t & 127);
-#pragma unroll
-      for (int q = 0; q < 8; ++q) r[q] = p[q * ld];
     }
   };
   auto gload = [&](int64_t k0, int set) {
@@ -318,15 +355,19 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
   };
   // Split into bf16 planes and store one 16-byte write per plane into the
   // k-contiguous layout the matrix-core operands read.
-  auto sstore_one = [&](unsigned short* base, const float (&r)[8], bool km, float sc) {
-    const int row = km ? (t & 127) : (t >> 2);
-    const int ko = km ? (t >> 7) * 8 : (t & 3) * 8;
-    uint4 pk[NP];
-    if constexpr (AR == AR_FP16X3) split_pack8_f16(r, sc, pk);
-    else split_pack8<NP>(r, pk);
+  auto sstore_one = [&](unsigned short* base, const float (&rr)[R][8], bool km, float sc) {
 #pragma unroll
-    for (int pl = 0; pl < NP; ++pl)
-      *reinterpret_cast<uint4*>(base + pl * TILE * BF_LDK + bf_off(row, ko / 8)) = pk[pl];
+    for (int q = 0; q < R; ++q) {
+      const int e = t + q * WL::NT;
+      const int row = km ? (e & 127) : (e >> 2);
+      const int ko = km ? (e >> 7) * 8 : (e & 3) * 8;
+      uint4 pk[NP];
+      if constexpr (AR == AR_FP16X3) split_pack8_f16(rr[q], sc, pk);
+      else split_pack8<NP>(rr[q], pk);
+#pragma unroll
+      for (int pl = 0; pl < NP; ++pl)
+        *reinterpret_cast<uint4*>(base + pl * TILE * LDK + bf_off<AR>(row, ko / 8)) = pk[pl];
+    }
   };
   // Slab s sits in register set s & 1 (DEEP) and is stored into LDS buffer s & 1.
   auto sstore = [&](int buf) {
@@ -356,7 +397,7 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
       uint4 a0[4], b0[NJ], a1[4], b1[NJ];
       auto frag = [&](const unsigned short* P, int pl, int base, int i) {
         return *reinterpret_cast<const uint4*>(
-            P + pl * TILE * BF_LDK + bf_off(base + i * 16 + (lane & 15), lane >> 4));
+            P + pl * TILE * LDK + bf_off<AR>(base + i * 16 + (lane & 15), lane >> 4));
       };
 #pragma unroll
       for (int i = 0; i < 4; ++i) a0[i] = frag(As, 0, wm * 64, i);
@@ -366,6 +407,19 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
       for (int i = 0; i < 4; ++i) a1[i] = frag(As, 1, wm * 64, i);
 #pragma unroll
       for (int j = 0; j < NJ; ++j) b1[j] = frag(Bs, 1, wn * WL::WC, j);
+      if constexpr (WIDE) {
+        // one fp32 chain per output over the whole k-range: hh, hl, lh
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < NJ; ++j) {
+            AV c = acc64[i][j];
+            c = mfma32<AR>(a0[i], b0[j], c);
+            c = mfma32<AR>(a1[i], b0[j], c);
+            c = mfma32<AR>(a0[i], b1[j], c);
+            acc64[i][j] = c;
+          }
+      } else {
       // Leading products first: they need only plane 0, and at the chunk's
       // last slab their float64 flush overlaps the correction products.
 #pragma unroll
@@ -416,6 +470,7 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
             }
         }
       }
+      }  // !WIDE
       // Stage slab s+1 (at the last slab a repeat of the final slab, stored and
       // never read) and refill its register set with slab s+3.
       sstore((u + 1) & 1);
@@ -427,12 +482,14 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
   // The correction products are at most 2^-8 of the leading ones, so their
   // single fp32 chain over the whole k-range adds little error (zero here
   // under CFLUSH).
+  if constexpr (!WIDE) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < NJ; ++j)
+      for (int j = 0; j < NJ; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)accc[i][j][r];
+        for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)accc[i][j][r];
+  }
   if constexpr (AR == AR_FP16X3) {
     const double inv = 1.0 / ((double)sa * (double)sb);  // a power of two: exact
 #pragma unroll

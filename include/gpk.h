@@ -10,9 +10,16 @@
  *     the set-up calls gpk_batched_plan and gpk_batched_tune and
  *     gpk_profile_collect, which synchronise and say so.
  *   - Process-wide state: the batched plan registry and the autotune table
- *     (both mutex-guarded host tables), the profiler's event list, and the
- *     per-device scratch of gpk_mll_forward_noalloc.  Nothing else outlives a
- *     call; the arithmetic is chosen per call through `prec`.
+ *     (both mutex-guarded host tables), the profiler's event list, the
+ *     per-device scratch of gpk_mll_forward_noalloc, and three library-owned
+ *     streams (default priority) with a few events per calling thread and
+ *     device.  Nothing else outlives a call; the arithmetic is chosen per call
+ *     through `prec`.
+ *   - Some calls fork work onto those library streams and join it back into
+ *     `stream` with events before they return (the look-ahead Cholesky for
+ *     n >= 8,192, the batched size buckets): to the caller the call stays
+ *     ordered on `stream`, and under HIP-graph capture the fork and join become
+ *     graph edges.
  *   - Matrices are row-major float64 with an explicit leading dimension.
  *   - Return value: GPK_OK or an error code; gpk_last_error() gives a
  *     thread-local message for the last failing call.
