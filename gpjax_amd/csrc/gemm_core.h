@@ -48,43 +48,20 @@ constexpr int TILE = 128;
 // Output accumulators of every main loop are float64.  In the bf16 loops the
 // matrix cores accumulate in fp32 over short k-chunks only (see
 // mainloop_bf16); each chunk is then added into float64.
-// X3_WIDE: the BF16x3 loop (the triangular inverse) with 64 x 64 per wave and
-// four waves per tile, its three products in one fp32 accumulator set over
-// the whole k-range (no float64 set, so the wide wave still fits two waves per
-// SIMD): half the LDS fragment bytes per product of the 64 x 32 form.
-#ifndef GPK_X3_WIDE
-#define GPK_X3_WIDE 0
-#endif
 template <int AR>
-constexpr bool x3_wide() { return GPK_X3_WIDE != 0 && AR == AR_BF16X3; }
-
-template <int AR, bool F32 = x3_wide<AR>()>
 struct AccT {
   typedef f64x4 vec;
   typedef double scalar;
-};
-template <int AR>
-struct AccT<AR, true> {
-  typedef f32x4 vec;
-  typedef float scalar;
 };
 
 // Waves of one tile: NT threads, NW waves of which WN along n; each wave owns
 // rows wm*64 .. +64 (four 16-row blocks) and columns wn*WC .. +WC (NJ 16-column
 // blocks), wm = w / WN, wn = w % WN.
-template <int AR, bool WIDE = x3_wide<AR>()>
+template <int AR>
 struct WaveLayout {
   static constexpr int NT = 512;
   static constexpr int NW = NT / 64;
   static constexpr int WN = 4;
-  static constexpr int WC = TILE / WN;
-  static constexpr int NJ = WC / 16;
-};
-template <int AR>
-struct WaveLayout<AR, true> {
-  static constexpr int NT = 256;
-  static constexpr int NW = NT / 64;
-  static constexpr int WN = 2;
   static constexpr int WC = TILE / WN;
   static constexpr int NJ = WC / 16;
 };
@@ -299,54 +276,45 @@ __device__ __forceinline__ void mainloop_f64(const TA* __restrict__ A, int64_t l
 // over 256 k, the rounding of the running sums dominated the error of the
 // whole mixed-precision factorisation (float64 products of the same float32
 // operands are ~1000x closer to the float64 result).
-template <int AR, bool A_KM, bool B_KN, bool CFLUSH = false, bool DEEP = !x3_wide<AR>()>
+template <int AR, bool A_KM, bool B_KN, bool CFLUSH = false, bool DEEP = true>
 __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64_t lda,
                                               const float* __restrict__ B, int64_t ldb,
                                               int64_t m0, int64_t n0, int64_t kb, int64_t ke,
-                                              typename AccT<AR>::vec (&acc64)[4][WaveLayout<AR>::NJ],
+                                              f64x4 (&acc64)[4][WaveLayout<AR>::NJ],
                                               char* smem, float sa = 1.f, float sb = 1.f) {
   typedef WaveLayout<AR> WL;
-  typedef typename AccT<AR>::vec AV;
   constexpr int NJ = WL::NJ;
   constexpr int NP = bf_planes<AR>();
   constexpr int LDK = bf_ldk<AR>();
   constexpr int OPS = NP * TILE * LDK;  // bf16 elements of one operand in one buffer
-  constexpr int R = 512 / WL::NT;       // staging units (8 k of one row) per thread
-  constexpr bool WIDE = x3_wide<AR>();
   unsigned short* S = reinterpret_cast<unsigned short*>(smem);  // [2][A, B][NP][TILE][LDK]
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w / WL::WN, wn = w % WL::WN;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) acc64[i][j] = AV{0, 0, 0, 0};
+    for (int j = 0; j < NJ; ++j) acc64[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
   if (kb >= ke) return;
 
-  // A slab (128 rows x 32 k of each operand) is 512 staging units of 8
-  // consecutive k of one row; thread t takes units t + q NT, q < R.
-  // Row-contiguous operand: unit e -> row e/4, k = (e%4)*8 .. +7 (two 16-byte
-  // loads).  Column-contiguous operand: row e%128, k = (e/128)*8 .. +7 (eight
+  // Every thread stages 8 consecutive k of one row of each operand per slab.
+  // Row-contiguous operand: row = t/4, k = (t%4)*8 .. +7 (two 16-byte loads).
+  // Column-contiguous operand: row = t%128, k = (t/128)*8 .. +7 (eight
   // 4-byte loads; consecutive lanes read consecutive rows, 256 bytes per load).
   constexpr int NSET = DEEP ? 2 : 1;
-  float ra[NSET][R][8], rb[NSET][R][8];
+  float ra[NSET][8], rb[NSET][8];
   auto gload_one = [&](const float* __restrict__ P, int64_t ld, int64_t r0, bool km,
-                       int64_t k0, float (&rr)[R][8]) {
+                       int64_t k0, float (&r)[8]) {
+    if (!km) {
+      const float* p = P + (r0 + (t >> 2)) * ld + k0 + (t & 3) * 8;
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int e = t + q * WL::NT;
-      float (&r)[8] = rr[q];
-      if (!km) {
-        const float* p = P + (r0 + (e >> 2)) * ld + k0 + (e & 3) * 8;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const float4 v = *reinterpret_cast<const float4*>(p + 4 * h);
-          r[4 * h] = v.x; r[4 * h + 1] = v.y; r[4 * h + 2] = v.z; r[4 * h + 3] = v.w;
-        }
-      } else {
-        const float* p = P + (k0 + (e >> 7) * 8) * ld + r0 + (e & 127);
-#pragma unroll
-        for (int h = 0; h < 8; ++h) r[h] = p[h * ld];
+      for (int q = 0; q < 2; ++q) {
+        const float4 v = *reinterpret_cast<const float4*>(p + 4 * q);
+        r[4 * q] = v.x; r[4 * q + 1] = v.y; r[4 * q + 2] = v.z; r[4 * q + 3] = v.w;
       }
+    } else {
+      const float* p = P + (k0 + (t >> 7) * 8) * ld + r0 + (t & 127);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) r[q] = p[q * ld];
     }
   };
   auto gload = [&](int64_t k0, int set) {
@@ -355,19 +323,15 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
   };
   // Split into bf16 planes and store one 16-byte write per plane into the
   // k-contiguous layout the matrix-core operands read.
-  auto sstore_one = [&](unsigned short* base, const float (&rr)[R][8], bool km, float sc) {
+  auto sstore_one = [&](unsigned short* base, const float (&r)[8], bool km, float sc) {
+    const int row = km ? (t & 127) : (t >> 2);
+    const int ko = km ? (t >> 7) * 8 : (t & 3) * 8;
+    uint4 pk[NP];
+    if constexpr (AR == AR_FP16X3) split_pack8_f16(r, sc, pk);
+    else split_pack8<NP>(r, pk);
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int e = t + q * WL::NT;
-      const int row = km ? (e & 127) : (e >> 2);
-      const int ko = km ? (e >> 7) * 8 : (e & 3) * 8;
-      uint4 pk[NP];
-      if constexpr (AR == AR_FP16X3) split_pack8_f16(rr[q], sc, pk);
-      else split_pack8<NP>(rr[q], pk);
-#pragma unroll
-      for (int pl = 0; pl < NP; ++pl)
-        *reinterpret_cast<uint4*>(base + pl * TILE * LDK + bf_off<AR>(row, ko / 8)) = pk[pl];
-    }
+    for (int pl = 0; pl < NP; ++pl)
+      *reinterpret_cast<uint4*>(base + pl * TILE * LDK + bf_off<AR>(row, ko / 8)) = pk[pl];
   };
   // Slab s sits in register set s & 1 (DEEP) and is stored into LDS buffer s & 1.
   auto sstore = [&](int buf) {
@@ -407,19 +371,6 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
       for (int i = 0; i < 4; ++i) a1[i] = frag(As, 1, wm * 64, i);
 #pragma unroll
       for (int j = 0; j < NJ; ++j) b1[j] = frag(Bs, 1, wn * WL::WC, j);
-      if constexpr (WIDE) {
-        // one fp32 chain per output over the whole k-range: hh, hl, lh
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < NJ; ++j) {
-            AV c = acc64[i][j];
-            c = mfma32<AR>(a0[i], b0[j], c);
-            c = mfma32<AR>(a1[i], b0[j], c);
-            c = mfma32<AR>(a0[i], b1[j], c);
-            acc64[i][j] = c;
-          }
-      } else {
       // Leading products first: they need only plane 0, and at the chunk's
       // last slab their float64 flush overlaps the correction products.
 #pragma unroll
@@ -470,7 +421,6 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
             }
         }
       }
-      }  // !WIDE
       // Stage slab s+1 (at the last slab a repeat of the final slab, stored and
       // never read) and refill its register set with slab s+3.
       sstore((u + 1) & 1);
@@ -482,14 +432,12 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
   // The correction products are at most 2^-8 of the leading ones, so their
   // single fp32 chain over the whole k-range adds little error (zero here
   // under CFLUSH).
-  if constexpr (!WIDE) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < NJ; ++j)
+    for (int j = 0; j < NJ; ++j)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)accc[i][j][r];
-  }
+      for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)accc[i][j][r];
   if constexpr (AR == AR_FP16X3) {
     const double inv = 1.0 / ((double)sa * (double)sb);  // a power of two: exact
 #pragma unroll
