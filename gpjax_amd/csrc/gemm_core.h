@@ -117,27 +117,21 @@ constexpr int F64_BK = 16;
 constexpr int F64_LDP = TILE + 1;  // padded row length (doubles)
 constexpr size_t F64_SMEM = 2 /*A,B*/ * 2 /*buffers*/ * F64_BK * F64_LDP * sizeof(double);
 constexpr int BF_BK = 32;
-// LDS rows of the bf16 planes, chosen per arithmetic (measured on one box,
-// same tree, two runs each): padded 96-byte rows for BF16x6 and FP16x3, and for
-// the BF16x3 loop (the triangular inverse) unpadded 64-byte rows whose four
-// 16-byte chunks are permuted by chunk ^ ((row >> 2) & 3), so that sixteen lanes
-// reading (or writing) 16 bytes each -- sixteen rows of one chunk for the
-// fragment reads, four rows of four chunks for the staging stores -- touch
-// sixteen different 16-byte bank groups.  Swizzling the x3 loop took the inverse
-// from 338 to 330 ms; swizzling the x6 loops made the fused Sigma^-1 pass 16 ms
-// slower, and 80-byte padded rows were slower for both.
-#ifndef GPK_BF_SWIZZLE
-#define GPK_BF_SWIZZLE 1  // 1: the BF16x3 loop only; 2: every bf16 loop; 0: none
-#endif
-template <int AR>
-constexpr bool bf_swz() {
-  return GPK_BF_SWIZZLE == 2 || (GPK_BF_SWIZZLE == 1 && AR == AR_BF16X3);
-}
-template <int AR>
-constexpr int bf_ldk() { return bf_swz<AR>() ? BF_BK : BF_BK + 16; }  // row length (bf16)
-template <int AR>
+// LDS rows of the bf16 planes.  Swizzled (SWZ): unpadded 64-byte rows whose
+// four 16-byte chunks are permuted by chunk ^ ((row >> 1) & 3).  Padded: 96-byte
+// rows.  Measured at N = 65,536 (SQ_LDS_BANK_CONFLICT, one evaluation; times on
+// one box, two runs each): the swizzled rows have no bank-conflict cycles in any
+// of the loops (the padded rows: 27-29 % of LDS-active cycles; the permutation
+// chunk ^ ((row >> 2) & 3): 36-44 %, and chunk ^ (row & 3), (chunk + row) & 3
+// were no better), and they take the Cholesky updates from 587 to 571 ms and
+// the triangular inverse from 335 to 321 ms; the fused Sigma^-1 pass, with its
+// single-set pipeline and float64 epilogue, ran 553 -> 561 ms with them and keeps
+// the padded rows.
+template <bool SWZ>
+constexpr int bf_ldk() { return SWZ ? BF_BK : BF_BK + 16; }  // row length (bf16 elements)
+template <bool SWZ>
 __device__ __forceinline__ int bf_off(int row, int chunk) {
-  return row * bf_ldk<AR>() + 8 * (bf_swz<AR>() ? (chunk ^ ((row >> 2) & 3)) : chunk);
+  return row * bf_ldk<SWZ>() + 8 * (SWZ ? chunk ^ ((row >> 1) & 3) : chunk);
 }
 constexpr int BF_CHUNK = 4;        // slabs per fp32 chunk: 128 k
 template <int AR>
@@ -154,10 +148,10 @@ __device__ __forceinline__ f32x4 mfma32(uint4 a, uint4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
                                                    __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
-template <int AR>
+template <int AR, bool SWZ = true>
 constexpr size_t smem_bytes() {
   return AR == AR_F64 ? F64_SMEM
-                      : (size_t)2 /*buffers*/ * 2 /*A,B*/ * bf_planes<AR>() * TILE * bf_ldk<AR>() *
+                      : (size_t)2 /*buffers*/ * 2 /*A,B*/ * bf_planes<AR>() * TILE * bf_ldk<SWZ>() *
                             sizeof(unsigned short);
 }
 
@@ -276,7 +270,7 @@ __device__ __forceinline__ void mainloop_f64(const TA* __restrict__ A, int64_t l
 // over 256 k, the rounding of the running sums dominated the error of the
 // whole mixed-precision factorisation (float64 products of the same float32
 // operands are ~1000x closer to the float64 result).
-template <int AR, bool A_KM, bool B_KN, bool CFLUSH = false, bool DEEP = true>
+template <int AR, bool A_KM, bool B_KN, bool CFLUSH = false, bool DEEP = true, bool SWZ = true>
 __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64_t lda,
                                               const float* __restrict__ B, int64_t ldb,
                                               int64_t m0, int64_t n0, int64_t kb, int64_t ke,
@@ -285,7 +279,7 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
   typedef WaveLayout<AR> WL;
   constexpr int NJ = WL::NJ;
   constexpr int NP = bf_planes<AR>();
-  constexpr int LDK = bf_ldk<AR>();
+  constexpr int LDK = bf_ldk<SWZ>();
   constexpr int OPS = NP * TILE * LDK;  // bf16 elements of one operand in one buffer
   unsigned short* S = reinterpret_cast<unsigned short*>(smem);  // [2][A, B][NP][TILE][LDK]
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -331,7 +325,7 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
     else split_pack8<NP>(r, pk);
 #pragma unroll
     for (int pl = 0; pl < NP; ++pl)
-      *reinterpret_cast<uint4*>(base + pl * TILE * LDK + bf_off<AR>(row, ko / 8)) = pk[pl];
+      *reinterpret_cast<uint4*>(base + pl * TILE * LDK + bf_off<SWZ>(row, ko / 8)) = pk[pl];
   };
   // Slab s sits in register set s & 1 (DEEP) and is stored into LDS buffer s & 1.
   auto sstore = [&](int buf) {
@@ -361,7 +355,7 @@ __device__ __forceinline__ void mainloop_bf16(const float* __restrict__ A, int64
       uint4 a0[4], b0[NJ], a1[4], b1[NJ];
       auto frag = [&](const unsigned short* P, int pl, int base, int i) {
         return *reinterpret_cast<const uint4*>(
-            P + pl * TILE * LDK + bf_off<AR>(base + i * 16 + (lane & 15), lane >> 4));
+            P + pl * TILE * LDK + bf_off<SWZ>(base + i * 16 + (lane & 15), lane >> 4));
       };
 #pragma unroll
       for (int i = 0; i < 4; ++i) a0[i] = frag(As, 0, wm * 64, i);
