@@ -1145,30 +1145,57 @@ int batched_mll_grad(int kind, int prec, const Arith3& ar, const double* x, cons
   const BatchPlan P = make_batch_plan(off, count, d, base);
   std::string arch;
   GPK_TRY(current_arch(&arch));
-  const int ng = (int)P.groups.size();
-  // The size groups are independent (disjoint workspaces and outputs): they
-  // run on up to BATCH_STREAMS streams -- the caller's and library-owned ones,
-  // forked from and joined back to the caller's by events -- so one group's
-  // latency-bound launches overlap another's large ones.  Groups are dealt
-  // largest first (count * np^3) to the least loaded stream.
-  const int ns = std::min(std::min(BATCH_STREAMS, 1 + AUX_STREAMS), ng);
+  // The size groups are independent (disjoint workspaces and outputs), and so
+  // are the members of one group: the work is cut into pieces -- whole groups,
+  // or contiguous member ranges of a group heavier than 1/BATCH_STREAMS of the
+  // batch (count * np^3) -- dealt largest first to the least loaded of up to
+  // BATCH_STREAMS streams, the caller's and library-owned ones, forked from and
+  // joined back to the caller's by events, so one piece's latency-bound launches
+  // overlap another's large ones.
+  struct Piece {
+    int group;
+    BatchGroup g;
+    double cost;
+  };
+  std::vector<Piece> pieces;
+  double total = 0.0;
+  for (const BatchGroup& g : P.groups) {
+    const double np = (double)g.np;
+    total += (double)g.count * np * np * np;
+  }
+  const int ns_max = std::min(BATCH_STREAMS, 1 + AUX_STREAMS);
+  for (int gi = 0; gi < (int)P.groups.size(); ++gi) {
+    const BatchGroup& g = P.groups[gi];
+    const double np = (double)g.np;
+    const double cost = (double)g.count * np * np * np;
+    int64_t k = ns_max > 1 ? (int64_t)std::ceil(cost / (total / ns_max) - 1e-9) : 1;
+    k = std::max<int64_t>(1, std::min<int64_t>(k, g.count));
+    for (int64_t q = 0; q < k; ++q) {
+      const int64_t a = g.count * q / k, b = g.count * (q + 1) / k;
+      Piece pc{gi, g, 0.0};
+      pc.g.first = g.first + a;
+      pc.g.count = b - a;
+      pc.g.off = g.off + (size_t)a * g.stride;
+      pc.cost = (double)pc.g.count * np * np * np;
+      pieces.push_back(pc);
+    }
+  }
+  const int np_ = (int)pieces.size();
+  const int ns = std::min(ns_max, np_);
   hipStream_t st[1 + AUX_STREAMS] = {s};
   hipEvent_t ev[AUX_EVENTS] = {};
-  std::vector<int> on(ng, 0);
+  std::vector<int> on(np_, 0);
   if (ns > 1) {
     GPK_TRY(aux_streams(st + 1, ev));
-    std::vector<int> order(ng);
-    for (int i = 0; i < ng; ++i) order[i] = i;
-    auto cost = [&](int i) {
-      const double np = (double)P.groups[i].np;
-      return (double)P.groups[i].count * np * np * np;
-    };
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    std::vector<int> order(np_);
+    for (int i = 0; i < np_; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return pieces[a].cost > pieces[b].cost; });
     std::vector<double> load(ns, 0.0);
     for (int i : order) {
       const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
       on[i] = k;
-      load[k] += cost(i);
+      load[k] += pieces[i].cost;
     }
     GPK_HIP_TRY(hipEventRecord(ev[3], s));
     for (int k = 1; k < ns; ++k) GPK_HIP_TRY(hipStreamWaitEvent(st[k], ev[3], 0));
@@ -1184,18 +1211,17 @@ int batched_mll_grad(int kind, int prec, const Arith3& ar, const double* x, cons
           (void)hipStreamWaitEvent(st[0], ev[3 + k], 0);
     }
   } join{st, ev, ns};
-  for (int i = 0; i < ng; ++i) {
-    const BatchGroup& g = P.groups[i];
+  for (int i = 0; i < np_; ++i) {
     Variant vr;
     vr.stored_sigma = false;
     vr.adj_transposed = false;
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
-      auto it = g_tune.find(tune_key(arch, kind, prec, g, d));
+      auto it = g_tune.find(tune_key(arch, kind, prec, P.groups[pieces[i].group], d));
       if (it != g_tune.end()) vr = it->second;
     }
-    GPK_TRY(run_group<TS>(kind, ar, x, y, d, theta, nls, jitter, iters, ws, P, g, vr, result,
-                          grad, st[on[i]]));
+    GPK_TRY(run_group<TS>(kind, ar, x, y, d, theta, nls, jitter, iters, ws, P, pieces[i].g, vr,
+                          result, grad, st[on[i]]));
   }
   return GPK_OK;
 }
