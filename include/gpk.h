@@ -17,7 +17,7 @@
  *     through `prec`.
  *   - Some calls fork work onto those library streams and join it back into
  *     `stream` with events before they return (the look-ahead Cholesky for
- *     n >= 8,192, the batched size buckets): to the caller the call stays
+ *     n > 8,064, the batched size buckets): to the caller the call stays
  *     ordered on `stream`, and under HIP-graph capture the fork and join become
  *     graph edges.
  *   - Matrices are row-major float64 with an explicit leading dimension.
@@ -136,7 +136,9 @@ size_t gpk_mll_workspace_bytes(int64_t n, int64_t d, int prec);
  * FP64, 6 for mixed; the steps stop once ||r - Sigma alpha|| <= 1e-12 ||r||,
  * decided on the device), and writes
  * result[GPK_RESULT_LEN] (slots GPK_RES_*).  The workspace then holds the state
- * that gpk_mll_backward consumes.
+ * that gpk_mll_backward consumes.  1 <= d <= 64 (the exact-GP entry points;
+ * GPJax takes any d, gpjax/kernels/stationary/base.py:120-146).  For n > 8,064
+ * the Cholesky forks onto a library stream and joins back (see Conventions).
  */
 int gpk_mll_forward(int kind, int prec, const double* x, const double* y, int64_t n, int64_t d,
                     const double* theta, int nls, double jitter, int refine_iters, void* ws,
