@@ -359,6 +359,12 @@ def main() -> int:
     arith = [int(v) for v in args.mixed_arith.split(",")]
     mixed = "mixed:%d,%d,%d" % tuple(arith)
     _lib.mixed_arith(*arith)  # validates the stages
+    # The a-posteriori check flags every result whose Cholesky or fused-pass
+    # stage is outside its calibration (fp16 or three-product splits); timing
+    # those stages means timing them unchecked (the flag is still reported).
+    calibrated = arith[0] in (0, 6) and arith[2] in (0, 6)
+    if not calibrated:
+        gpx.config.mixed_check = "off"
     t0 = time.time()
     w = workloads.exact_large(n=args.n)
     print(f"# workload generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
@@ -470,7 +476,8 @@ def main() -> int:
                             "refine_resid_mixed": resid_m,
                             "gate_mll_1e-6": rel_mll <= 1e-6, "gate_grad_1e-4": rel_g <= 1e-4}
     # the mixed path's own a-posteriori check (Rademacher probes of I - Sigma X^T X)
-    line["mixed_check"] = {"probe_frob_norm": probe_m * math.sqrt(args.n),
+    line["mixed_check"] = {"action": gpx.config.mixed_check,
+                           "probe_frob_norm": probe_m * math.sqrt(args.n),
                            "sinv_frob_est": float(res_m[_lib.RES_SINV]),
                            "mll_err_est": float(res_m[_lib.RES_MLL_ERR]),
                            "flag": flag_m, "flag_threshold": 0.75,

@@ -96,7 +96,10 @@ extern "C" {
                            /* 2^-25 max|Sigma| ||Sigma^-1||_F sqrt(max(1, n/1024)) */
                            /* (measured errors <= 8.1x it); flagged when 10x it   */
                            /* exceeds 1e-6 |MLL|, or ||E||_F > 0.75, or the       */
-                           /* refinement stopped above 1e-10                      */
+                           /* refinement stopped above 1e-10, and always when the */
+                           /* Cholesky updates or the fused Sigma^-1 pass use     */
+                           /* fp16 or three-product splits (the check is not      */
+                           /* calibrated for their product errors)                */
 #define GPK_RESULT_LEN 10
 
 const char* gpk_version(void);
