@@ -5,10 +5,13 @@
 //   2. Recursive Cholesky:  potrf(A11); L21 = A21 L11^-T (recursive TRSM whose
 //      leaves multiply by the float64 leaf inverses); A22 -= L21 L21^T (SYRK on
 //      the matrix cores); potrf(A22).  Leaves factor NB x NB blocks in float64
-//      and also emit their inverses and log-diagonal sums.
+//      and also emit their inverses and log-diagonal sums.  Single problems of
+//      at least four 2,048-wide panels take the look-ahead form (factor.h).
 //   3. Recursive triangular inverse X = L^-1:  X21 = -X22 (L21 X11).
-//   4. alpha = X^T X r, then float64 iterative refinement against Sigma
-//      recomputed from the inputs: alpha += X^T X (r - Sigma alpha).
+//   4. alpha = X^T X r, then float64 iterative refinement against Sigma in
+//      float64 (a stored copy in the mixed path, recomputed from the inputs in
+//      the FP64 path): alpha += X^T X (r - Sigma alpha), stopped on the device
+//      at 1e-12; the mixed path's a-posteriori probes ride along.
 //   5. logdet = sum of leaf log-diagonals, quad = r . alpha, mll.
 // Backward (gpk_mll_backward): the fused LAUUM + adjoint tile pass and a
 // deterministic reduction to the gradient.
