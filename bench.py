@@ -172,7 +172,7 @@ def other_workload(args) -> dict:
     batched -- 4,096 independent exact GPs (N_i in 32..1024, D = 4, RBF ARD), MLL +
                gradient of all of them per step;
     sparse  -- collapsed ELBO + gradient (theta and the 2,048 x 8 inducing inputs),
-               N = 1e6, D = 8, Matern-3/2 ARD, FP64;
+               N = 1e6, D = 8, Matern-3/2 ARD, mixed and FP64;
     predict -- ConjugatePosterior.predict with the diagonal variance at large T:
                the matern workload's N = 16,384 training points, T = 1,000,000 test
                points ~ N(0, I_8), FP64 (factorisation included in every step)."""
@@ -227,19 +227,42 @@ def other_workload(args) -> dict:
                         y=torch.tensor(w["y"][:, None], device=dev))
         params = [k.lengthscale, k.variance, post.likelihood.obs_stddev, q.inducing_inputs]
 
-        def step():
-            for p in params:
-                p.value.grad = None
-            e = gpx.objectives.collapsed_elbo(q, D)
-            e.backward()
-            return e
+        def make(prec):
+            def step():
+                for p in params:
+                    p.value.grad = None
+                e, res = gpx.objectives.collapsed_elbo_with_info(q, D, precision=prec)
+                e.backward()
+                return float(e.detach()), grads(params), res
+            return step
 
-        t, e = timed(step, args.steps, args.warmup)
-        line.update(metric="collapsed ELBO+grad evals/s at N=1e6, M=2048", value=round(1.0 / t, 4),
-                    unit="evals/s", ms_per_step=round(t * 1e3, 2), dtype="f64",
+        t_m, (e_m, g_m, res_m) = timed(make("mixed"), args.steps, args.warmup)
+        t_d, (e_d, g_d, _) = timed(make("fp64"), args.steps, args.warmup)
+        nth = g_d.numel() - w["Z"].size  # theta entries first, then the 2,048 x 8 Z gradient
+        gt_m, gt_d, gz_m, gz_d = g_m[:nth], g_d[:nth], g_m[nth:], g_d[nth:]
+        res_m = res_m.cpu()
+        flag = float(res_m[gpx._lib.RES_FLAG])
+        line.update(metric="collapsed ELBO+grad evals/s at N=1e6, M=2048, mixed",
+                    value=round(1.0 / t_m, 4), unit="evals/s", ms_per_step=round(t_m * 1e3, 2),
+                    dtype="bf16x6-split products of f32 Kzx, f64 sums and m x m algebra",
                     config={"workload": "sparse: N=1e6, D=8, M=2048, Matern-3/2 ARD "
                                         "(gpjax_amd.workloads.sparse)", "n": len(w["X"]),
-                            "m": 2048, "d": 8}, elbo=float(e.detach()))
+                            "m": 2048, "d": 8}, elbo=e_m,
+                    fp64_gpu={"evals_per_s": round(1.0 / t_d, 4),
+                              "ms_per_step": round(t_d * 1e3, 2), "elbo": e_d},
+                    speedup_vs_fp64=round(t_d / t_m, 3),
+                    accuracy={"rel_elbo_vs_fp64": abs(e_m - e_d) / abs(e_d),
+                              "rel_grad_theta_vs_fp64":
+                                  float(torch.linalg.norm(gt_m - gt_d) / torch.linalg.norm(gt_d)),
+                              "rel_grad_z_vs_fp64":
+                                  float(torch.linalg.norm(gz_m - gz_d) / torch.linalg.norm(gz_d)),
+                              "max_grad_z_err_of_max_entry":
+                                  float((gz_m - gz_d).abs().max() / gz_d.abs().max())},
+                    mixed_check={"flag": flag,
+                                 "elbo_err_est": float(res_m[gpx._lib.RES_MLL_ERR]),
+                                 "meaning": "flag 0: mixed result kept; 1: flagged; 2: "
+                                            "recomputed in float64 (then the timed steps are "
+                                            "not the mixed path)"})
         cpu_n = 16384
     elif args.workload == "predict":
         import numpy as np

@@ -20,7 +20,11 @@
 // The backward evaluates Kzx once per chunk of 65,536 points, forms W for the
 // chunk with one GEMM on the matrix cores and reduces it against
 // dKzx / d(lengthscale, variance, Z) elementwise.
-// Everything here runs in float64.
+// GPK_PREC_FP64 runs everything in float64.  GPK_PREC_MIXED keeps the Kzx
+// chunks in float32 (each entry evaluated in float64 and rounded once) and
+// runs the two N-streamed products, P += Kc Kc^T and Kc^T G_P, as bf16-split
+// products whose fp32 chunk sums are flushed to float64; P, b, q, the M x M
+// factorisations and inverses and every reduction stay float64.
 #include <cmath>
 
 #include "factor.h"
@@ -52,17 +56,18 @@ __device__ __forceinline__ double d2_of(const double* a, const double* b) {
 // Kc[i * ldk + (g - c0)] = k(z_i, x_g) for the chunk's points; one block per
 // inducing row, fixed-order reduction; the first chunk writes, later ones add.
 // (P = Kzx Kxz is accumulated from the same chunk by one FP64 GEMM.)
+template <typename TK>
 __global__ __launch_bounds__(256) void chunk_rowsum_kernel(
-    const double* __restrict__ Kc, int64_t ldk, int64_t c0, int64_t ncol,
+    const TK* __restrict__ Kc, int64_t ldk, int64_t c0, int64_t ncol,
     const double* __restrict__ y, const double* __restrict__ theta, int nls, int accumulate,
     double* __restrict__ b, double* __restrict__ q) {
   __shared__ double red[2][4];
   const int64_t i = blockIdx.x;
   const double cmean = theta[nls + 2];
-  const double* row = Kc + i * ldk;
+  const TK* row = Kc + i * ldk;
   double sb = 0.0, sq = 0.0;
   for (int64_t j = threadIdx.x; j < ncol; j += 256) {
-    const double k = row[j];
+    const double k = (double)row[j];
     sb += k * (y[c0 + j] - cmean);
     sq += k;
   }
@@ -146,15 +151,19 @@ __global__ void mirror_kernel(double* __restrict__ A, int64_t ld, int64_t mp) {
   if (i / NB > j / NB) A[j * ld + i] = A[i * ld + j];
 }
 
-// G_P = (Kinv - Cinv) / s - c c^T / s^3
-__global__ void gp_kernel(double* __restrict__ G, const double* __restrict__ Kinv,
-                          const double* __restrict__ Cinv, const double* __restrict__ c,
-                          int64_t ld, int64_t mp, const double* __restrict__ theta, int nls) {
+// G_P = (Kinv - Cinv) / s - c c^T / s^3 (and its float32 rounding in Gf, the
+// operand of the mixed data pass, when Gf is given)
+__global__ void gp_kernel(double* __restrict__ G, float* __restrict__ Gf,
+                          const double* __restrict__ Kinv, const double* __restrict__ Cinv,
+                          const double* __restrict__ c, int64_t ld, int64_t mp,
+                          const double* __restrict__ theta, int nls) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= mp * mp) return;
   const int64_t i = e / mp, j = e % mp;
   const double sd = theta[nls + 1], s = sd * sd;
-  G[i * ld + j] = (Kinv[i * ld + j] - Cinv[i * ld + j]) / s - c[i] * c[j] / (s * s * s);
+  const double g = (Kinv[i * ld + j] - Cinv[i * ld + j]) / s - c[i] * c[j] / (s * s * s);
+  G[i * ld + j] = g;
+  if (Gf) Gf[i * ld + j] = (float)g;
 }
 
 // G_K = Kinv - Cinv - c c^T / s^2 - T2 / s,  T2 = Kinv P Kinv
@@ -182,6 +191,31 @@ __global__ __launch_bounds__(256) void frob_kernel(const double* __restrict__ A,
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Per-block partials of sum_ij (G_P,ij P_ij)^2, G_P = (Kinv - Cinv) / s - c c^T / s^3
+// formed on the fly (the a-posteriori error model of the mixed statistics).
+__global__ __launch_bounds__(256) void gpfrob_kernel(const double* __restrict__ Kinv,
+                                                     const double* __restrict__ Cinv,
+                                                     const double* __restrict__ c,
+                                                     const double* __restrict__ P, int64_t mp,
+                                                     const double* __restrict__ theta, int nls,
+                                                     double* __restrict__ part) {
+  __shared__ double red[4];
+  const double sd = theta[nls + 1], s = sd * sd;
+  double acc = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < mp * mp;
+       e += (int64_t)gridDim.x * 256) {
+    const int64_t i = e / mp, j = e % mp;
+    const double g = (Kinv[e] - Cinv[e]) / s - c[i] * c[j] / (s * s * s);
+    const double v = g * P[e];
+    acc += v * v;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
   if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
@@ -353,8 +387,8 @@ __global__ __launch_bounds__(256) void elbo_fwd_finalize(
     const double* __restrict__ leafK, const int* __restrict__ infoK,
     const double* __restrict__ leafC, const int* __restrict__ infoC, int64_t nleaf,
     const double* __restrict__ b, const double* __restrict__ c, int64_t mp, int64_t n,
-    const double* __restrict__ theta, int nls, double* __restrict__ scal,
-    double* __restrict__ result) {
+    const double* __restrict__ theta, int nls, const double* __restrict__ epart,
+    int uncalibrated, double* __restrict__ scal, double* __restrict__ result) {
   __shared__ double red[4];
   double bc = 0.0;
   for (int64_t i = threadIdx.x; i < mp; i += 256) bc += b[i] * c[i];
@@ -362,6 +396,11 @@ __global__ __launch_bounds__(256) void elbo_fwd_finalize(
   double tr = 0.0;
   for (int k = threadIdx.x; k < nf; k += 256) tr += fpart[k];
   tr = block_sum256(tr, red);
+  double gp2 = 0.0;
+  if (epart) {
+    for (int k = threadIdx.x; k < nf; k += 256) gp2 += epart[k];
+    gp2 = block_sum256(gp2, red);
+  }
   if (threadIdx.x == 0) {
     double rr = 0.0, sr = 0.0;
     for (int k = 0; k < nr; ++k) {
@@ -398,6 +437,18 @@ __global__ __launch_bounds__(256) void elbo_fwd_finalize(
     result[GPK_RES_RESID] = trace;
     result[GPK_RES_INFO] = (double)info;
     for (int k = 5; k < GPK_RESULT_LEN; ++k) result[k] = 0.0;
+    if (epart) {
+      // Mixed statistics: the bf16-split products and their fp32 chunk sums
+      // leave P with entry errors R_ij of relative size ~ 2^-24 and random
+      // sign, whose first-order effect on the ELBO is tr(G_P R) / 2.  Modelled
+      // as 2^-24 |P_ij| with independent signs its spread is
+      // 2^-24 sqrt(2) ||G_P o P||_F / 2 (measured |errors| <= 1.2x it).
+      const double err = 0.5 * 0x1p-24 * sqrt(2.0 * gp2);
+      result[GPK_RES_MLL_ERR] = err;
+      const bool bad = info != 0 || !isfinite(elbo) || !(5.0 * err <= 1e-6 * fabs(elbo)) ||
+                       uncalibrated;
+      result[GPK_RES_FLAG] = bad ? 1.0 : 0.0;
+    }
   }
 }
 
@@ -483,7 +534,8 @@ struct ElboLayout {
   size_t off_liK = 0, off_ldK = 0, off_ifK = 0, off_liC = 0, off_ldC = 0, off_ifC = 0;
   size_t off_b = 0, off_q = 0, off_c = 0, off_u = 0, off_Pc = 0;
   size_t off_gzpart = 0, off_tpart = 0;
-  size_t off_tpart2 = 0, off_gzkz = 0, off_rpart = 0, off_fpart = 0, off_vpart = 0, off_scal = 0;
+  size_t off_tpart2 = 0, off_gzkz = 0, off_rpart = 0, off_fpart = 0, off_epart = 0;
+  size_t off_vpart = 0, off_scal = 0;
   // one chunk of Kzx [mp][nbw]; (G_P Kzx)^T [nbw][mp] in the backward, the
   // split-K partials of P in the forward
   size_t off_kc = 0, off_wt = 0;
@@ -536,6 +588,7 @@ ElboLayout make_elbo_layout(int64_t n, int64_t m) {
   L.off_gzkz = o; o += al((size_t)mp * EDMAX * sizeof(double));
   L.off_rpart = o; o += al((size_t)L.nr * 2 * sizeof(double));
   L.off_fpart = o; o += al((size_t)L.nf * sizeof(double));
+  L.off_epart = o; o += al((size_t)L.nf * sizeof(double));
   L.off_vpart = o; o += al((size_t)((mp + 1023) / 1024) * mp * sizeof(double));
   L.off_scal = o; o += al(32 * sizeof(double));
   L.off_kc = o; o += al((size_t)mp * L.nbw * sizeof(double));
@@ -545,6 +598,42 @@ ElboLayout make_elbo_layout(int64_t n, int64_t m) {
 }
 
 inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Storage and product arithmetic of one ELBO call.  GPK_PREC_FP64: Kzx chunks
+// in float64, FP64 matrix-core products.  GPK_PREC_MIXED: float32 chunks,
+// P += Kc Kc^T with six-product and Kc^T G_P with six-product bf16 splits;
+// GPK_PREC_MIXED_ARITH(p, t, l) names the two products' arithmetic instead
+// (p: statistics product, t: data-pass product, each 0, 3 or 6; l is not used
+// by the ELBO).  The fp16 split needs operand scales and is not offered here.
+struct ElboArith {
+  bool f32 = false;
+  int fwd = AR_F64, bwd = AR_F64;
+};
+
+int decode_elbo_prec(int prec, ElboArith* ea) {
+  ElboArith a;
+  if (prec == GPK_PREC_FP64) {
+    *ea = a;
+    return GPK_OK;
+  }
+  if ((prec & 0xFF) != GPK_PREC_MIXED) return GPK_ERR_BAD_ARG;
+  a.f32 = true;
+  a.fwd = AR_BF16X6;
+  a.bwd = AR_BF16X6;
+  if (prec & GPK_PREC_ARITH_FLAG) {
+    a.fwd = (prec >> 12) & 0xF;
+    a.bwd = (prec >> 16) & 0xF;
+    const int l = (prec >> 20) & 0xF;
+    auto ok = [](int v) { return v == AR_F64 || v == AR_BF16X3 || v == AR_BF16X6; };
+    if ((prec & ~0xFFF1FF) != 0 || !ok(a.fwd) || !ok(a.bwd) ||
+        !(ok(l) || l == AR_FP16X3))
+      return GPK_ERR_BAD_ARG;
+  } else if (prec != GPK_PREC_MIXED) {
+    return GPK_ERR_BAD_ARG;
+  }
+  *ea = a;
+  return GPK_OK;
+}
 
 #define GPK_DMAX_DISPATCH(d, MACRO) \
   do {                              \
@@ -556,9 +645,10 @@ inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b);
     else MACRO(32);                 \
   } while (0)
 
-int elbo_forward_impl(int kind, const double* x, const double* y, int64_t n, const double* z,
-                      int64_t m, int d, const double* theta, int nls, double jitter, void* ws,
-                      const ElboLayout& Lo, double* result, hipStream_t s) {
+int elbo_forward_impl(int kind, const ElboArith& ea, const double* x, const double* y,
+                      int64_t n, const double* z, int64_t m, int d, const double* theta, int nls,
+                      double jitter, void* ws, const ElboLayout& Lo, double* result,
+                      hipStream_t s) {
   const int64_t mp = Lo.mp;
   double* P = at<double>(ws, Lo.off_P);
   double* b = at<double>(ws, Lo.off_b);
@@ -567,34 +657,49 @@ int elbo_forward_impl(int kind, const double* x, const double* y, int64_t n, con
   double* u = at<double>(ws, Lo.off_u);
   double* vpart = at<double>(ws, Lo.off_vpart);
   // 1. streamed statistics, chunk by chunk: Kc = Kzx for the chunk's points
-  //    (evaluated once, zero outside m x ncol), P (+)= Kc Kc^T on the FP64
-  //    matrix cores (lower tiles), b (+)= Kc r, q (+)= Kc 1.
+  //    (evaluated once, zero outside m x ncol), P (+)= Kc Kc^T on the matrix
+  //    cores (lower tiles), b (+)= Kc r, q (+)= Kc 1.
   //    The lower tiles alone (136 at M = 2,048) would leave most of the chip
   //    idle, so the GEMM is split over k into S members of one launch (the
-  //    batch descriptor shifts every operand by z * kz doubles): member z reads
-  //    columns z*kz .. +kz of Kc and accumulates its partial of P at column
-  //    z*kz of a [mp][nbw] buffer (the one that holds Wt in the backward).
-  double* Kc = at<double>(ws, Lo.off_kc);
+  //    batch descriptor shifts every operand by the same z * kz * sizeof(Kc
+  //    entry) bytes): member z reads columns z*kz .. +kz of Kc and accumulates
+  //    its float64 partial of P at column z*kp (kp = kz for float64 chunks,
+  //    kz / 2 for float32 ones; kp >= mp keeps the partials apart) of a
+  //    [mp][nbw] float64 buffer (the one that holds Wt in the backward).
+  const size_t ke = ea.f32 ? sizeof(float) : sizeof(double);
   double* Pp = at<double>(ws, Lo.off_wt);
   const int64_t nbw = Lo.nbw;
+  const int64_t kdiv = ea.f32 ? 2 : 1;
   int64_t S = 1;
-  while (S < 16 && (nbw / (2 * S)) % TILE == 0 && nbw / (2 * S) >= mp) S *= 2;
-  const int64_t kz = nbw / S;
+  while (S < 16 && (nbw / (2 * S)) % TILE == 0 && nbw / (2 * S) >= kdiv * mp) S *= 2;
+  const int64_t kz = nbw / S, kp = kz / kdiv;
   Batch sk;
-  sk.ws_stride = kz * (int64_t)sizeof(double);
+  sk.ws_stride = kz * (int64_t)ke;
   sk.count = (unsigned)S;
   for (int64_t c0 = 0; c0 < n; c0 += BW_CHUNK) {
     const int64_t ncol = (n - c0 < BW_CHUNK) ? n - c0 : BW_CHUNK;
-    GPK_TRY(launch_kernel_matrix<double>(kind, z, m, x + c0 * d, ncol, d, theta, nls, 0.0,
-                                         KM_ZERO_PAD, Kc, nbw, mp, nbw, false, s));
-    GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, false, Kc, nbw, Kc, nbw, Pp, nbw,
+    if (ea.f32) {
+      float* Kc = at<float>(ws, Lo.off_kc);
+      GPK_TRY(launch_kernel_matrix<float>(kind, z, m, x + c0 * d, ncol, d, theta, nls, 0.0,
+                                          KM_ZERO_PAD, Kc, nbw, mp, nbw, false, s));
+      GPK_TRY(launch_gemm<float, float, double>(ea.fwd, false, false, Kc, nbw, Kc, nbw, Pp, nbw,
                                                 mp, mp, kz, 1.0, c0 > 0, true, KR_FULL, s,
                                                 nullptr, sk));
-    hipLaunchKernelGGL(chunk_rowsum_kernel, dim3((unsigned)mp), dim3(256), 0, s, Kc, nbw, c0,
-                       ncol, y, theta, nls, c0 > 0 ? 1 : 0, b, qv);
+      hipLaunchKernelGGL(chunk_rowsum_kernel<float>, dim3((unsigned)mp), dim3(256), 0, s, Kc,
+                         nbw, c0, ncol, y, theta, nls, c0 > 0 ? 1 : 0, b, qv);
+    } else {
+      double* Kc = at<double>(ws, Lo.off_kc);
+      GPK_TRY(launch_kernel_matrix<double>(kind, z, m, x + c0 * d, ncol, d, theta, nls, 0.0,
+                                           KM_ZERO_PAD, Kc, nbw, mp, nbw, false, s));
+      GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, false, Kc, nbw, Kc, nbw, Pp,
+                                                  nbw, mp, mp, kz, 1.0, c0 > 0, true, KR_FULL, s,
+                                                  nullptr, sk));
+      hipLaunchKernelGGL(chunk_rowsum_kernel<double>, dim3((unsigned)mp), dim3(256), 0, s, Kc,
+                         nbw, c0, ncol, y, theta, nls, c0 > 0 ? 1 : 0, b, qv);
+    }
     GPK_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(psum_kernel, dim3(cdiv(mp * mp, 256)), dim3(256), 0, s, Pp, nbw, kz, S, P,
+  hipLaunchKernelGGL(psum_kernel, dim3(cdiv(mp * mp, 256)), dim3(256), 0, s, Pp, nbw, kp, S, P,
                      mp);
   GPK_LAUNCH_CHECK();
   hipLaunchKernelGGL(rstats_kernel, dim3((unsigned)Lo.nr), dim3(256), 0, s, y, n, theta, nls,
@@ -632,22 +737,33 @@ int elbo_forward_impl(int kind, const double* x, const double* y, int64_t n, con
   hipLaunchKernelGGL(mirror_kernel, dim3(cdiv(mp * mp, 256)), dim3(256), 0, s, Kinv, mp, mp);
   hipLaunchKernelGGL(mirror_kernel, dim3(cdiv(mp * mp, 256)), dim3(256), 0, s, Cinv, mp, mp);
   GPK_LAUNCH_CHECK();
-  // 5. tr(Kzz^-1 P) and the objective
+  // 5. tr(Kzz^-1 P) and the objective (mixed: with the error model's ||G_P o P||_F)
   double* fpart = at<double>(ws, Lo.off_fpart);
   hipLaunchKernelGGL(frob_kernel, dim3((unsigned)Lo.nf), dim3(256), 0, s, Kinv, P, mp * mp,
                      fpart);
   GPK_LAUNCH_CHECK();
+  double* epart = nullptr;
+  if (ea.f32) {
+    epart = at<double>(ws, Lo.off_epart);
+    hipLaunchKernelGGL(gpfrob_kernel, dim3((unsigned)Lo.nf), dim3(256), 0, s, Kinv, Cinv, cvec,
+                       P, mp, theta, nls, epart);
+    GPK_LAUNCH_CHECK();
+  }
+  // The error model is calibrated for six-product (or float64) products only.
+  const int uncal = ea.f32 && (ea.fwd == AR_BF16X3 || ea.bwd == AR_BF16X3) ? 1 : 0;
   hipLaunchKernelGGL(elbo_fwd_finalize, dim3(1), dim3(256), 0, s, at<double>(ws, Lo.off_rpart),
                      (int)Lo.nr, fpart, (int)Lo.nf, at<double>(ws, Lo.off_ldK),
                      at<int>(ws, Lo.off_ifK), at<double>(ws, Lo.off_ldC), at<int>(ws, Lo.off_ifC),
-                     mp / NB, b, cvec, mp, n, theta, nls, at<double>(ws, Lo.off_scal), result);
+                     mp / NB, b, cvec, mp, n, theta, nls, epart, uncal,
+                     at<double>(ws, Lo.off_scal), result);
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }
 
-int elbo_backward_impl(int kind, const double* x, const double* y, int64_t n, const double* z,
-                       int64_t m, int d, const double* theta, int nls, double gbar, void* ws,
-                       const ElboLayout& Lo, double* grad, double* grad_z, hipStream_t s) {
+int elbo_backward_impl(int kind, const ElboArith& ea, const double* x, const double* y,
+                       int64_t n, const double* z, int64_t m, int d, const double* theta, int nls,
+                       double gbar, void* ws, const ElboLayout& Lo, double* grad, double* grad_z,
+                       hipStream_t s) {
   const int64_t mp = Lo.mp;
   double* P = at<double>(ws, Lo.off_P);
   double* Kinv = at<double>(ws, Lo.off_Kinv);
@@ -668,21 +784,31 @@ int elbo_backward_impl(int kind, const double* x, const double* y, int64_t n, co
                      scal);
   GPK_LAUNCH_CHECK();
   // G_P and the data pass
-  hipLaunchKernelGGL(gp_kernel, dim3(cdiv(mp * mp, 256)), dim3(256), 0, s, G, Kinv, Cinv, cvec,
-                     mp, mp, theta, nls);
+  // the float32 G_P of the mixed data pass lives in T1 (free until G_K)
+  float* Gf = ea.f32 ? at<float>(ws, Lo.off_T1) : nullptr;
+  hipLaunchKernelGGL(gp_kernel, dim3(cdiv(mp * mp, 256)), dim3(256), 0, s, G, Gf, Kinv, Cinv,
+                     cvec, mp, mp, theta, nls);
   GPK_LAUNCH_CHECK();
   // Data pass, chunk by chunk: Kc = Kzx for the chunk's points (zero outside
-  // m x ncol), Wt = Kc^T G_P on the FP64 matrix cores, then the elementwise
-  // contraction against dKzx.
-  double* Kc = at<double>(ws, Lo.off_kc);
+  // m x ncol), Wt = Kc^T G_P (float64) on the matrix cores, then the
+  // elementwise contraction against dKzx.
   double* Wt = at<double>(ws, Lo.off_wt);
   for (int64_t c0 = 0; c0 < n; c0 += BW_CHUNK) {
     const int64_t ncol = (n - c0 < BW_CHUNK) ? n - c0 : BW_CHUNK;
     const int64_t ncp = round_up(ncol, TILE);
-    GPK_TRY(launch_kernel_matrix<double>(kind, z, m, x + c0 * d, ncol, d, theta, nls, 0.0,
-                                         KM_ZERO_PAD, Kc, ncp, mp, ncp, false, s));
-    GPK_TRY(launch_gemm<double, double, double>(AR_F64, true, false, Kc, ncp, G, mp, Wt, mp, ncp,
-                                                mp, mp, 1.0, false, false, KR_FULL, s));
+    if (ea.f32) {
+      float* Kc = at<float>(ws, Lo.off_kc);
+      GPK_TRY(launch_kernel_matrix<float>(kind, z, m, x + c0 * d, ncol, d, theta, nls, 0.0,
+                                          KM_ZERO_PAD, Kc, ncp, mp, ncp, false, s));
+      GPK_TRY(launch_gemm<float, float, double>(ea.bwd, true, false, Kc, ncp, Gf, mp, Wt, mp,
+                                                ncp, mp, mp, 1.0, false, false, KR_FULL, s));
+    } else {
+      double* Kc = at<double>(ws, Lo.off_kc);
+      GPK_TRY(launch_kernel_matrix<double>(kind, z, m, x + c0 * d, ncol, d, theta, nls, 0.0,
+                                           KM_ZERO_PAD, Kc, ncp, mp, ncp, false, s));
+      GPK_TRY(launch_gemm<double, double, double>(AR_F64, true, false, Kc, ncp, G, mp, Wt, mp,
+                                                  ncp, mp, mp, 1.0, false, false, KR_FULL, s));
+    }
     dim3 grid((unsigned)Lo.mt, (unsigned)((ncol + EW - 1) / EW));
 #define GPK_E3(DM)                                                                              \
   hipLaunchKernelGGL(elbo_wgrad_kernel<DM>, grid, dim3(256), 0, s, kind, x, y, z, m, d, theta, \
@@ -725,14 +851,20 @@ int elbo_backward_impl(int kind, const double* x, const double* y, int64_t n, co
   return GPK_OK;
 }
 
+// ea == nullptr: the caller runs float64 only (the collapsed predictive).
 int check_elbo(int kind, int prec, const double* x, const double* y, int64_t n, const double* z,
-               int64_t m, int64_t d, const double* theta, int nls) {
+               int64_t m, int64_t d, const double* theta, int nls, ElboArith* ea) {
   if (kind < GPK_KERNEL_RBF || kind > GPK_KERNEL_MATERN52) {
     set_error("unknown kernel kind %d", kind);
     return GPK_ERR_BAD_ARG;
   }
-  if (prec != GPK_PREC_FP64) {
-    set_error("collapsed ELBO: only GPK_PREC_FP64 is implemented");
+  if (!ea && prec != GPK_PREC_FP64) {
+    set_error("collapsed predict: only GPK_PREC_FP64 is implemented");
+    return GPK_ERR_BAD_ARG;
+  }
+  if (ea && decode_elbo_prec(prec, ea) != GPK_OK) {
+    set_error("collapsed ELBO: invalid prec 0x%x (GPK_PREC_FP64, GPK_PREC_MIXED or "
+              "GPK_PREC_MIXED_ARITH(p, t, l) with p, t in {0, 3, 6})", prec);
     return GPK_ERR_BAD_ARG;
   }
   if (!x || !y || !z || !theta) {
@@ -813,7 +945,7 @@ int elbo_predict_impl(int kind, const double* x, const double* y, int64_t n, con
                       double jitter, int full_cov, void* ws, const ElboLayout& Lo,
                       const ElboPredLayout& P, double* mean, double* cov, hipStream_t s) {
   const int64_t mp = Lo.mp, tp = P.tp;
-  GPK_TRY(elbo_forward_impl(kind, x, y, n, z, m, d, theta, nls, jitter, ws, Lo,
+  GPK_TRY(elbo_forward_impl(kind, ElboArith(), x, y, n, z, m, d, theta, nls, jitter, ws, Lo,
                             at<double>(ws, P.off_res), s));
   double* kzt = at<double>(ws, P.off_kzt);
   double* a1 = at<double>(ws, P.off_a1);
@@ -890,7 +1022,8 @@ size_t gpk_elbo_workspace_bytes(int64_t n, int64_t m, int64_t d, int prec) {
 int gpk_elbo_forward(int kind, int prec, const double* x, const double* y, int64_t n,
                      const double* z, int64_t m, int64_t d, const double* theta, int nls,
                      double jitter, void* ws, size_t ws_bytes, double* result, void* stream) {
-  int rc = check_elbo(kind, prec, x, y, n, z, m, d, theta, nls);
+  ElboArith ea;
+  int rc = check_elbo(kind, prec, x, y, n, z, m, d, theta, nls, &ea);
   if (rc) return rc;
   if (!ws || !result) {
     set_error("collapsed ELBO: null workspace/result");
@@ -905,7 +1038,7 @@ int gpk_elbo_forward(int kind, int prec, const double* x, const double* y, int64
     set_error("collapsed ELBO: workspace must be 256-byte aligned");
     return GPK_ERR_BAD_ARG;
   }
-  return elbo_forward_impl(kind, x, y, n, z, m, (int)d, theta, nls, jitter, ws, Lo, result,
+  return elbo_forward_impl(kind, ea, x, y, n, z, m, (int)d, theta, nls, jitter, ws, Lo, result,
                            (hipStream_t)stream);
 }
 
@@ -913,7 +1046,8 @@ int gpk_elbo_backward(int kind, int prec, const double* x, const double* y, int6
                       const double* z, int64_t m, int64_t d, const double* theta, int nls,
                       double gbar_host, void* ws, size_t ws_bytes, double* grad, double* grad_z,
                       void* stream) {
-  int rc = check_elbo(kind, prec, x, y, n, z, m, d, theta, nls);
+  ElboArith ea;
+  int rc = check_elbo(kind, prec, x, y, n, z, m, d, theta, nls, &ea);
   if (rc) return rc;
   if (!ws || !grad || !grad_z) {
     set_error("collapsed ELBO: null workspace/grad");
@@ -928,8 +1062,8 @@ int gpk_elbo_backward(int kind, int prec, const double* x, const double* y, int6
     set_error("collapsed ELBO: workspace must be 256-byte aligned");
     return GPK_ERR_BAD_ARG;
   }
-  return elbo_backward_impl(kind, x, y, n, z, m, (int)d, theta, nls, gbar_host, ws, Lo, grad,
-                            grad_z, (hipStream_t)stream);
+  return elbo_backward_impl(kind, ea, x, y, n, z, m, (int)d, theta, nls, gbar_host, ws, Lo,
+                            grad, grad_z, (hipStream_t)stream);
 }
 
 size_t gpk_elbo_predict_workspace_bytes(int64_t n, int64_t m, int64_t d, int64_t t,
@@ -943,7 +1077,7 @@ int gpk_elbo_predict(int kind, int prec, const double* x, const double* y, int64
                      const double* z, int64_t m, int64_t d, const double* xt, int64_t t,
                      const double* theta, int nls, double jitter, int full_cov, void* ws,
                      size_t ws_bytes, double* mean, double* cov, void* stream) {
-  int rc = check_elbo(kind, prec, x, y, n, z, m, d, theta, nls);
+  int rc = check_elbo(kind, prec, x, y, n, z, m, d, theta, nls, nullptr);
   if (rc) return rc;
   if (!xt || t < 1 || !ws || !mean || !cov) {
     set_error("collapsed predict: null xt/ws/mean/cov or t < 1");

@@ -34,20 +34,28 @@ def _is_mixed(prec_id: int) -> bool:
     return (prec_id & 0xFF) == _lib.PRECISIONS["mixed"]
 
 
-def mixed_check_says_recompute(res: torch.Tensor, prec_id: int, n: int, what: str) -> bool:
+def mixed_check_says_recompute(res: torch.Tensor, prec_id: int, n: int, what: str,
+                               elbo: bool = False) -> bool:
     """Apply ``config.mixed_check`` to a mixed-precision result buffer: True when
-    the caller should recompute in float64.  Reads two result slots (a host sync)
-    unless the check is off, the path is float64, or a graph is being captured."""
+    the caller should recompute in float64.  Reads the check's result slots (a
+    host sync) unless the check is off, the path is float64, or a graph is being
+    captured.  ``elbo``: the buffer is gpk_elbo_forward's."""
     if not _is_mixed(prec_id) or config.mixed_check == "off":
         return False
     if torch.cuda.is_current_stream_capturing():
         return False
-    info, flag, probe = res[[_lib.RES_INFO, _lib.RES_FLAG, _lib.RES_PROBE]].tolist()
+    info, flag, probe, err, val = res[[_lib.RES_INFO, _lib.RES_FLAG, _lib.RES_PROBE,
+                                       _lib.RES_MLL_ERR, _lib.RES_MLL]].tolist()
     if info != 0.0:
-        # float32 storage moves Sigma by ~1e-7 relative; a pivot that then turns
-        # non-positive is a precision failure (float64 may still factor it)
+        # float32 storage moves Sigma (or C = Kzz + P / s) by ~1e-7 relative; a
+        # pivot that then turns non-positive is a precision failure (float64 may
+        # still factor it)
         msg = (f"{what}: the mixed-precision Cholesky broke down at pivot {int(info) - 1} "
                f"(n = {n})")
+    elif flag == 1.0 and elbo:
+        msg = (f"{what}: the mixed-precision a-posteriori check flagged this problem "
+               f"(estimated ELBO error {err:.3g} of |ELBO| = {abs(val):.6g}, n = {n}); its "
+               "ELBO / gradient may be outside 1e-6 / 1e-4 of the float64 answer")
     elif flag == 1.0:
         msg = (f"{what}: the mixed-precision a-posteriori check flagged this problem "
                f"(||I - Sigma X^T X||_F ~ {probe * math.sqrt(n):.3g}, n = {n}); its MLL / "
@@ -192,21 +200,30 @@ class _CollapsedELBO(torch.autograd.Function):
     """Forward: gpk_elbo_forward.  Backward: gpk_elbo_backward (Kzx tiles recomputed)."""
 
     @staticmethod
-    def forward(ctx, ls, var, sd, c, Z, X, y, kind_id, jitter, nls, info_out):
+    def forward(ctx, ls, var, sd, c, Z, X, y, kind_id, prec_id, jitter, nls, info_out):
         n, d = X.shape
         m = Z.shape[0]
         dev = X.device
         theta = torch.cat([ls.reshape(-1)[:nls], var.reshape(1), sd.reshape(1),
                            c.reshape(1)]).to(dev, torch.float64).contiguous()
         Zc = Z.detach().to(dev, torch.float64).contiguous()
-        nbytes = _lib.lib().gpk_elbo_workspace_bytes(n, m, d, 0)
-        ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
-        res = torch.empty(_lib.RESULT_LEN, dtype=torch.float64, device=dev)
-        _lib.call("gpk_elbo_forward", kind_id, 0, _lib.ptr(X), _lib.ptr(y), n, _lib.ptr(Zc), m, d,
-                  _lib.ptr(theta), nls, float(jitter), _lib.ptr(ws), nbytes, _lib.ptr(res),
-                  _lib.stream_ptr(dev))
+
+        def run(prec):
+            nbytes = _lib.lib().gpk_elbo_workspace_bytes(n, m, d, prec)
+            ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+            res = torch.empty(_lib.RESULT_LEN, dtype=torch.float64, device=dev)
+            _lib.call("gpk_elbo_forward", kind_id, prec, _lib.ptr(X), _lib.ptr(y), n,
+                      _lib.ptr(Zc), m, d, _lib.ptr(theta), nls, float(jitter), _lib.ptr(ws),
+                      nbytes, _lib.ptr(res), _lib.stream_ptr(dev))
+            return ws, nbytes, res
+
+        ws, nbytes, res = run(prec_id)
+        if mixed_check_says_recompute(res, prec_id, n, "collapsed_elbo", elbo=True):
+            prec_id = _lib.PRECISIONS["fp64"]
+            ws, nbytes, res = run(prec_id)
+            res[_lib.RES_FLAG] = 2.0
         ctx.ws, ctx.nbytes, ctx.theta, ctx.Z = ws, nbytes, theta, Zc
-        ctx.X, ctx.y, ctx.kind_id, ctx.nls = X, y, kind_id, nls
+        ctx.X, ctx.y, ctx.kind_id, ctx.prec_id, ctx.nls = X, y, kind_id, prec_id, nls
         ctx.shapes = (ls.shape, var.shape, sd.shape, c.shape, Z.shape)
         if info_out is not None:
             info_out["result"] = res
@@ -220,19 +237,22 @@ class _CollapsedELBO(torch.autograd.Function):
         nls = ctx.nls
         grad = torch.empty(nls + 3, dtype=torch.float64, device=X.device)
         gz = torch.empty((m, d), dtype=torch.float64, device=X.device)
-        _lib.call("gpk_elbo_backward", ctx.kind_id, 0, _lib.ptr(X), _lib.ptr(y), n, _lib.ptr(Zc),
-                  m, d, _lib.ptr(ctx.theta), nls, 1.0, _lib.ptr(ctx.ws), ctx.nbytes,
+        _lib.call("gpk_elbo_backward", ctx.kind_id, ctx.prec_id, _lib.ptr(X), _lib.ptr(y), n,
+                  _lib.ptr(Zc), m, d, _lib.ptr(ctx.theta), nls, 1.0, _lib.ptr(ctx.ws), ctx.nbytes,
                   _lib.ptr(grad), _lib.ptr(gz), _lib.stream_ptr(X.device))
         g = grad * gbar
         ls_s, var_s, sd_s, c_s, z_s = ctx.shapes
         ls_g = g[:nls].reshape(ls_s) if math.prod(ls_s) == nls else g[:nls].sum().expand(ls_s)
         ctx.ws = None
         return (ls_g, g[nls].reshape(var_s), g[nls + 1].reshape(sd_s), g[nls + 2].reshape(c_s),
-                (gz * gbar).reshape(z_s), None, None, None, None, None, None)
+                (gz * gbar).reshape(z_s), None, None, None, None, None, None, None)
 
 
-def collapsed_elbo_with_info(variational_family, data):
-    """(elbo, result) -- result is the device buffer of ``gpk_elbo_forward``."""
+def collapsed_elbo_with_info(variational_family, data, precision: str | None = None):
+    """(elbo, result) -- result is the device buffer of ``gpk_elbo_forward``.
+    ``precision``: "fp64", "mixed" (float32 Kzx chunks, six-product bf16 splits
+    for the two N-streamed products) or "mixed:P,T,L" (P: the product
+    Kzx Kxz, T: the data pass Kxz G; L unused); default ``config.precision``."""
     post = variational_family.posterior
     kernel = post.prior.kernel
     X = kernel.slice_input(data.X)
@@ -248,11 +268,12 @@ def collapsed_elbo_with_info(variational_family, data):
     info: dict = {}
     elbo = _CollapsedELBO.apply(kernel.lengthscale.value, kernel.variance.value,
                                 post.likelihood.obs_stddev.value, c, Z, X, y, kernel.kind_id,
-                                float(variational_family.jitter), nls, info)
+                                _precision_id(precision), float(variational_family.jitter), nls,
+                                info)
     return elbo, info["result"]
 
 
-def collapsed_elbo(variational_family, data):
+def collapsed_elbo(variational_family, data, precision: str | None = None):
     """Collapsed ELBO (``gpjax/objectives.py:321-416``), differentiable in the kernel
     hyperparameters, obs_stddev, a trainable constant mean and the inducing inputs."""
-    return collapsed_elbo_with_info(variational_family, data)[0]
+    return collapsed_elbo_with_info(variational_family, data, precision)[0]

@@ -190,9 +190,19 @@ size_t gpk_elbo_workspace_bytes(int64_t n, int64_t m, int64_t d, int prec);
  * jitter `jitter` (CollapsedVariationalGaussian, variational_families.py:767-784).
  * The data enter through P = Kzx Kxz, b = Kzx (y - c) and q = Kzx 1, which are
  * accumulated on the matrix cores chunk by chunk (Kzx evaluated once per chunk
- * of 65,536 points into the workspace): no n x m matrix is stored.  result slots: [GPK_RES_MLL] = ELBO, [1] = log|B|,
- * [2] = the quadratic term, [3] = the trace term, [GPK_RES_INFO] = pivot status.
- * d <= 32; prec must be GPK_PREC_FP64.
+ * of 65,536 points into the workspace): no n x m matrix is stored.  result
+ * slots: [GPK_RES_MLL] = ELBO, [1] = log|B|, [2] = the quadratic term, [3] = the
+ * trace term, [GPK_RES_INFO] = pivot status.  d <= 32.
+ * prec: GPK_PREC_FP64, or GPK_PREC_MIXED: the Kzx chunks in float32 (each entry
+ * evaluated in float64, rounded once) and the two n-streamed products (P and
+ * the backward's Kxz G_P) as six-product bf16 splits with float64 sums; the
+ * m x m algebra stays float64.  GPK_PREC_MIXED_ARITH(p, t, l) names the
+ * products instead: p for P, t for the backward's, each 0, 3 or 6 (l unused).
+ * Mixed only: [GPK_RES_MLL_ERR] = 2^-24 ||G_P o P||_F / sqrt(2), a model of
+ * the ELBO's error from the split products (G_P = dELBO / dP, measured errors
+ * <= 1.2x it), and [GPK_RES_FLAG] = 1 when 5x it exceeds 1e-6 |ELBO|, the
+ * factorisation broke down, or either product uses three-product splits (not
+ * calibrated): recompute in GPK_PREC_FP64.  The same workspace size serves both.
  */
 int gpk_elbo_forward(int kind, int prec, const double* x, const double* y, int64_t n,
                      const double* z, int64_t m, int64_t d, const double* theta, int nls,

@@ -208,3 +208,102 @@ def test_collapsed_predict_diagonal_streams_large_t(cuda):
     np.testing.assert_allclose(full.mean.cpu().numpy(), mu[:200], rtol=0, atol=1e-12)
     np.testing.assert_allclose(np.diag(full.covariance().cpu().numpy()), var[:200], rtol=0,
                                atol=1e-11)
+
+
+def _elbo_and_grads(q, D, k, post, precision):
+    """ELBO, [d lengthscale.., d variance, d obs_stddev, d mean] and d Z on one path."""
+    params = [k.lengthscale, k.variance, post.likelihood.obs_stddev,
+              post.prior.mean_function.constant, q.inducing_inputs]
+    for p in params:
+        p.value.grad = None
+    e, res = gpx.objectives.collapsed_elbo_with_info(q, D, precision=precision)
+    e.backward()
+    g = np.concatenate([np.atleast_1d(k.lengthscale.value.grad.cpu().numpy()),
+                        [k.variance.value.grad.item(), post.likelihood.obs_stddev.value.grad.item(),
+                         post.prior.mean_function.constant.value.grad.item()]])
+    return e.item(), g, q.inducing_inputs.value.grad.cpu().numpy().copy(), res.cpu().numpy()
+
+
+# (kind, n, m, d, ard, lengthscale range, obs_stddev, the check must pass it)
+MIXED_CASES = [("rbf", 500, 40, 2, True, (0.05, 0.15), 0.3, False),
+               ("matern32", 3000, 300, 8, True, (0.2, 0.6), 0.3, False),
+               ("matern52", 1234, 129, 3, False, (0.2, 0.6), 0.3, False),
+               ("matern12", 700, 64, 1, True, (0.2, 0.6), 0.3, False),
+               ("matern52", 900, 96, 24, True, (0.6, 1.6), 0.3, False),
+               ("rbf", 640, 80, 32, True, (0.6, 1.6), 0.3, True),
+               ("matern32", 20000, 1000, 4, True, (0.1, 0.5), 0.3, False),
+               ("rbf", 20000, 512, 3, True, (0.2, 0.5), 0.3, False),
+               ("matern32", 40000, 1024, 8, True, (0.1, 0.5), 0.2, True)]
+
+
+@pytest.mark.parametrize("kind,n,m,d,ard,lsr,sd,passes", MIXED_CASES)
+def test_mixed_elbo_in_gate_or_flagged(cuda, monkeypatch, kind, n, m, d, ard, lsr, sd, passes):
+    """Mixed ELBO (float32 Kzx chunks, six-product bf16 splits): every result the
+    a-posteriori check lets through is within 1e-6 of the float64 ELBO, 1e-4
+    (relative, norm-wise) of its theta gradient and 1e-4 of the largest entry of
+    its inducing-input gradient; the float64 path stands in for the oracle here
+    (test_elbo_and_grad_parity pins it to 1e-9).  Ill-conditioned Kzz (low
+    dimensions, long lengthscales, many inducing points) makes the float32
+    statistics too coarse: those cases must come back flagged, and the 1-D and
+    3-D RBF ones break down outright.  The well-conditioned cases (C4's recipe
+    at N = 40,000, M = 1,024; RBF in 32 dimensions) must pass the check, so the
+    test cannot pass by flagging everything."""
+    monkeypatch.setattr(gpx.config, "mixed_check", "off")
+    rng = np.random.default_rng(n + m)
+    X = rng.uniform(0, 1, (n, d))
+    y = np.sin(2 * np.pi * X).sum(1) + 0.2 * rng.standard_normal(n)
+    Z = X[rng.choice(n, m, replace=False)] + 0.003
+    ls = rng.uniform(lsr[0], lsr[1], d) if ard else 0.4
+    k = KCLS[kind](lengthscale=ls, variance=1.2)
+    mf = gpx.mean_functions.Constant(gpx.parameters.Real(0.15))
+    post = gpx.gps.Prior(mean_function=mf, kernel=k) * \
+        gpx.likelihoods.Gaussian(num_datapoints=n, obs_stddev=sd)
+    q = gpx.variational_families.CollapsedVariationalGaussian(posterior=post, inducing_inputs=Z)
+    D = gpx.Dataset(X=torch.tensor(X, device=cuda), y=torch.tensor(y[:, None], device=cuda))
+    e0, g0, z0, r0 = _elbo_and_grads(q, D, k, post, "fp64")
+    e1, g1, z1, r1 = _elbo_and_grads(q, D, k, post, "mixed")
+    assert r0[gpx._lib.RES_FLAG] == 0.0 and r0[gpx._lib.RES_MLL_ERR] == 0.0
+    flag, est = r1[gpx._lib.RES_FLAG], r1[gpx._lib.RES_MLL_ERR]
+    rel_e = abs(e1 - e0) / abs(e0)
+    rel_g = np.linalg.norm(g1 - g0) / np.linalg.norm(g0)
+    z_err = np.max(np.abs(z1 - z0)) / np.max(np.abs(z0))
+    print(f"{kind} n={n} m={m} d={d}: flag={flag} info={r1[gpx._lib.RES_INFO]} "
+          f"est_rel={est / abs(e0):.2e} rel_elbo={rel_e:.2e} rel_g={rel_g:.2e} z_err={z_err:.2e}")
+    if passes:
+        assert flag == 0.0
+    if flag == 0.0:
+        assert rel_e <= 1e-6
+        assert rel_g <= 1e-4
+        assert z_err <= 1e-4
+        assert abs(e1 - e0) <= 2.0 * est
+    else:
+        assert flag == 1.0
+
+
+def test_mixed_elbo_falls_back_to_fp64(cuda, monkeypatch):
+    """Under mixed_check="fallback" a flagged mixed ELBO (here the mixed
+    factorisation of C = Kzz + P / s breaks down: RBF in three dimensions,
+    512 inducing points) is recomputed in float64: FLAG slot 2 and the float64
+    ELBO and gradients."""
+    rng = np.random.default_rng(20512)
+    n, m, d = 20000, 512, 3
+    X = rng.uniform(0, 1, (n, d))
+    y = np.sin(2 * np.pi * X).sum(1) + 0.2 * rng.standard_normal(n)
+    Z = X[rng.choice(n, m, replace=False)] + 0.003
+    k = gpx.kernels.RBF(lengthscale=rng.uniform(0.2, 0.5, d), variance=1.2)
+    mf = gpx.mean_functions.Constant(gpx.parameters.Real(0.15))
+    post = gpx.gps.Prior(mean_function=mf, kernel=k) * \
+        gpx.likelihoods.Gaussian(num_datapoints=n, obs_stddev=0.3)
+    q = gpx.variational_families.CollapsedVariationalGaussian(posterior=post, inducing_inputs=Z)
+    D = gpx.Dataset(X=torch.tensor(X, device=cuda), y=torch.tensor(y[:, None], device=cuda))
+    monkeypatch.setattr(gpx.config, "mixed_check", "fallback")
+    e0, g0, z0, _ = _elbo_and_grads(q, D, k, post, "fp64")
+    with pytest.warns(RuntimeWarning, match="recomputing in float64"):
+        e1, g1, z1, r1 = _elbo_and_grads(q, D, k, post, "mixed")
+    assert r1[gpx._lib.RES_FLAG] == 2.0
+    assert e1 == e0
+    np.testing.assert_array_equal(g1, g0)
+    np.testing.assert_array_equal(z1, z0)
+    monkeypatch.setattr(gpx.config, "mixed_check", "raise")
+    with pytest.raises(gpx.objectives.MixedPrecisionError):
+        _elbo_and_grads(q, D, k, post, "mixed")
