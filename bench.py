@@ -244,7 +244,8 @@ def other_workload(args) -> dict:
         flag = float(res_m[gpx._lib.RES_FLAG])
         line.update(metric="collapsed ELBO+grad evals/s at N=1e6, M=2048, mixed",
                     value=round(1.0 / t_m, 4), unit="evals/s", ms_per_step=round(t_m * 1e3, 2),
-                    dtype="bf16x6-split products of f32 Kzx, f64 sums and m x m algebra",
+                    dtype="f32 Kzx; P as bf16x6-split, gradient pass as fp16x3-split "
+                          "products with f64 sums; f64 m x m algebra",
                     config={"workload": "sparse: N=1e6, D=8, M=2048, Matern-3/2 ARD "
                                         "(gpjax_amd.workloads.sparse)", "n": len(w["X"]),
                             "m": 2048, "d": 8}, elbo=e_m,

@@ -21,10 +21,11 @@
 // chunk with one GEMM on the matrix cores and reduces it against
 // dKzx / d(lengthscale, variance, Z) elementwise.
 // GPK_PREC_FP64 runs everything in float64.  GPK_PREC_MIXED keeps the Kzx
-// chunks in float32 (each entry evaluated in float64 and rounded once) and
-// runs the two N-streamed products, P += Kc Kc^T and Kc^T G_P, as bf16-split
-// products whose fp32 chunk sums are flushed to float64; P, b, q, the M x M
-// factorisations and inverses and every reduction stay float64.
+// chunks in float32 (squared distances in float64, the kernel function in
+// float32 arithmetic) and runs the two N-streamed products, P += Kc Kc^T and
+// Kc^T G_P, as bf16-split products whose fp32 chunk sums are flushed to
+// float64; P, b, q, the M x M factorisations and inverses and every reduction
+// stay float64.
 #include <cmath>
 
 #include "factor.h"
@@ -151,19 +152,57 @@ __global__ void mirror_kernel(double* __restrict__ A, int64_t ld, int64_t mp) {
   if (i / NB > j / NB) A[j * ld + i] = A[i * ld + j];
 }
 
-// G_P = (Kinv - Cinv) / s - c c^T / s^3 (and its float32 rounding in Gf, the
-// operand of the mixed data pass, when Gf is given)
-__global__ void gp_kernel(double* __restrict__ G, float* __restrict__ Gf,
-                          const double* __restrict__ Kinv, const double* __restrict__ Cinv,
-                          const double* __restrict__ c, int64_t ld, int64_t mp,
-                          const double* __restrict__ theta, int nls) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= mp * mp) return;
-  const int64_t i = e / mp, j = e % mp;
+// G_P = (Kinv - Cinv) / s - c c^T / s^3 over the dense mp x mp (ld = mp), and
+// its float32 rounding in Gf (the operand of the mixed data pass) when Gf is
+// given; gmax (optional) gets per-block partials of max |G_P| (the fp16 split's
+// operand scale).  Grid-stride over 256-thread blocks.
+__global__ __launch_bounds__(256) void gp_kernel(double* __restrict__ G, float* __restrict__ Gf,
+                                                 const double* __restrict__ Kinv,
+                                                 const double* __restrict__ Cinv,
+                                                 const double* __restrict__ c, int64_t mp,
+                                                 const double* __restrict__ theta, int nls,
+                                                 double* __restrict__ gmax) {
+  __shared__ double red[4];
   const double sd = theta[nls + 1], s = sd * sd;
-  const double g = (Kinv[i * ld + j] - Cinv[i * ld + j]) / s - c[i] * c[j] / (s * s * s);
-  G[i * ld + j] = g;
-  if (Gf) Gf[i * ld + j] = (float)g;
+  double mx = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < mp * mp;
+       e += (int64_t)gridDim.x * 256) {
+    const int64_t i = e / mp, j = e % mp;
+    const double g = (Kinv[e] - Cinv[e]) / s - c[i] * c[j] / (s * s * s);
+    G[e] = g;
+    if (Gf) Gf[e] = (float)g;
+    mx = fmax(mx, fabs(g));
+  }
+  if (!gmax) return;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) gmax[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// Power-of-two fp16 operand scales of the mixed ELBO's products: each maps its
+// bound b to at most 2^14 (b clamped to [2^-60, 2^60]).  |Kzx| <= var (every
+// kernel here peaks at var); gmax: nf partials of max |G_P| (or null).
+// fs[0] = fs[1] = scale of Kzx (statistics product), fs[2] = scale of Kzx and
+// fs[3] = scale of G_P (data-pass product).
+__global__ void elbo_fp16_scale_kernel(const double* __restrict__ theta, int nls,
+                                       const double* __restrict__ gmax, int nf,
+                                       float* __restrict__ fs) {
+  if (threadIdx.x != 0) return;
+  auto sc = [](double b) {
+    b = fmin(fmax(b, 0x1p-60), 0x1p60);
+    return (float)ldexp(1.0, (int)floor(log2(16384.0 / b)));
+  };
+  const float sk = sc(theta[nls]);
+  fs[0] = sk;
+  fs[1] = sk;
+  fs[2] = sk;
+  if (gmax) {
+    double g = 0.0;
+    for (int k = 0; k < nf; ++k) g = fmax(g, gmax[k]);
+    fs[3] = sc(g);
+  }
 }
 
 // G_K = Kinv - Cinv - c c^T / s^2 - T2 / s,  T2 = Kinv P Kinv
@@ -230,11 +269,13 @@ __global__ __launch_bounds__(256) void gpfrob_kernel(const double* __restrict__ 
 // recomputed from the inputs per element.  Partials, split = c0 / EW + cb:
 // gzpart[(split * mp + i) * DMAX + q] and
 // tpart[(split * mt_count + mt) * (DMAX + 1) + q] (lengthscales, then variance).
-template <int DMAX>
+// Mixed path (TW = float): Wt in float32 and the kernel value and factor in
+// float32 arithmetic; distances, products and sums stay float64.
+template <int DMAX, typename TW>
 __global__ __launch_bounds__(256) void elbo_wgrad_kernel(
     int kind, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, int64_t m, int d, const double* __restrict__ theta, int nls,
-    const double* __restrict__ Wt, int64_t mp, int64_t c0, int64_t ncol,
+    const TW* __restrict__ Wt, int64_t mp, int64_t c0, int64_t ncol,
     const double* __restrict__ cvec, double* __restrict__ gzpart, double* __restrict__ tpart) {
   __shared__ double xc[TILE][DMAX + 1];
   __shared__ double rc[TILE];
@@ -268,12 +309,19 @@ __global__ __launch_bounds__(256) void elbo_wgrad_kernel(
     __syncthreads();
     const int jend = (b0 + TILE < p1) ? TILE : (int)(p1 - b0);
     if (rowok) {
-      const double* wcol = Wt + (b0 - c0) * mp + gi;
+      const TW* wcol = Wt + (b0 - c0) * mp + gi;
       for (int jj = half; jj < jend; jj += 2) {
-        const double W = wcol[(int64_t)jj * mp] + gb * rc[jj];
+        const double W = (double)wcol[(int64_t)jj * mp] + gb * rc[jj];
         const double d2 = d2_of<DMAX>(zi, xc[jj]);
         double kv, fv;
-        kval_fac(kind, d2, kv, fv);
+        if constexpr (std::is_same<TW, float>::value) {
+          float k32, f32;
+          kval_facf(kind, (float)d2, k32, f32);
+          kv = k32;
+          fv = f32;
+        } else {
+          kval_fac(kind, d2, kv, fv);
+        }
         s_var += W * kv;
         const double wf = W * fv;
 #pragma unroll
@@ -534,7 +582,7 @@ struct ElboLayout {
   size_t off_liK = 0, off_ldK = 0, off_ifK = 0, off_liC = 0, off_ldC = 0, off_ifC = 0;
   size_t off_b = 0, off_q = 0, off_c = 0, off_u = 0, off_Pc = 0;
   size_t off_gzpart = 0, off_tpart = 0;
-  size_t off_tpart2 = 0, off_gzkz = 0, off_rpart = 0, off_fpart = 0, off_epart = 0;
+  size_t off_tpart2 = 0, off_gzkz = 0, off_rpart = 0, off_fpart = 0, off_epart = 0, off_fs = 0;
   size_t off_vpart = 0, off_scal = 0;
   // one chunk of Kzx [mp][nbw]; (G_P Kzx)^T [nbw][mp] in the backward, the
   // split-K partials of P in the forward
@@ -589,6 +637,7 @@ ElboLayout make_elbo_layout(int64_t n, int64_t m) {
   L.off_rpart = o; o += al((size_t)L.nr * 2 * sizeof(double));
   L.off_fpart = o; o += al((size_t)L.nf * sizeof(double));
   L.off_epart = o; o += al((size_t)L.nf * sizeof(double));
+  L.off_fs = o; o += al(4 * sizeof(float));
   L.off_vpart = o; o += al((size_t)((mp + 1023) / 1024) * mp * sizeof(double));
   L.off_scal = o; o += al(32 * sizeof(double));
   L.off_kc = o; o += al((size_t)mp * L.nbw * sizeof(double));
@@ -601,10 +650,13 @@ inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b);
 
 // Storage and product arithmetic of one ELBO call.  GPK_PREC_FP64: Kzx chunks
 // in float64, FP64 matrix-core products.  GPK_PREC_MIXED: float32 chunks,
-// P += Kc Kc^T with six-product and Kc^T G_P with six-product bf16 splits;
+// P += Kc Kc^T with six-product bf16 splits (the ELBO's accuracy rests on P)
+// and Kc^T G_P with the fp16 split (it feeds only the gradient; measured
+// within 1.5x of the six-product gradient error at twice the speed);
 // GPK_PREC_MIXED_ARITH(p, t, l) names the two products' arithmetic instead
-// (p: statistics product, t: data-pass product, each 0, 3 or 6; l is not used
-// by the ELBO).  The fp16 split needs operand scales and is not offered here.
+// (p: statistics product, t: data-pass product, each 0, 3, 4 or 6; l is not
+// used by the ELBO).  The fp16 split (4) scales Kzx by its bound var and G_P by
+// its largest entry, found on the device.
 struct ElboArith {
   bool f32 = false;
   int fwd = AR_F64, bwd = AR_F64;
@@ -619,15 +671,15 @@ int decode_elbo_prec(int prec, ElboArith* ea) {
   if ((prec & 0xFF) != GPK_PREC_MIXED) return GPK_ERR_BAD_ARG;
   a.f32 = true;
   a.fwd = AR_BF16X6;
-  a.bwd = AR_BF16X6;
+  a.bwd = AR_FP16X3;
   if (prec & GPK_PREC_ARITH_FLAG) {
     a.fwd = (prec >> 12) & 0xF;
     a.bwd = (prec >> 16) & 0xF;
     const int l = (prec >> 20) & 0xF;
-    auto ok = [](int v) { return v == AR_F64 || v == AR_BF16X3 || v == AR_BF16X6; };
-    if ((prec & ~0xFFF1FF) != 0 || !ok(a.fwd) || !ok(a.bwd) ||
-        !(ok(l) || l == AR_FP16X3))
-      return GPK_ERR_BAD_ARG;
+    auto ok = [](int v) {
+      return v == AR_F64 || v == AR_BF16X3 || v == AR_BF16X6 || v == AR_FP16X3;
+    };
+    if ((prec & ~0xFFF1FF) != 0 || !ok(a.fwd) || !ok(a.bwd) || !ok(l)) return GPK_ERR_BAD_ARG;
   } else if (prec != GPK_PREC_MIXED) {
     return GPK_ERR_BAD_ARG;
   }
@@ -676,65 +728,95 @@ int elbo_forward_impl(int kind, const ElboArith& ea, const double* x, const doub
   Batch sk;
   sk.ws_stride = kz * (int64_t)ke;
   sk.count = (unsigned)S;
-  for (int64_t c0 = 0; c0 < n; c0 += BW_CHUNK) {
-    const int64_t ncol = (n - c0 < BW_CHUNK) ? n - c0 : BW_CHUNK;
-    if (ea.f32) {
-      float* Kc = at<float>(ws, Lo.off_kc);
-      GPK_TRY(launch_kernel_matrix<float>(kind, z, m, x + c0 * d, ncol, d, theta, nls, 0.0,
-                                          KM_ZERO_PAD, Kc, nbw, mp, nbw, false, s));
-      GPK_TRY(launch_gemm<float, float, double>(ea.fwd, false, false, Kc, nbw, Kc, nbw, Pp, nbw,
-                                                mp, mp, kz, 1.0, c0 > 0, true, KR_FULL, s,
-                                                nullptr, sk));
-      hipLaunchKernelGGL(chunk_rowsum_kernel<float>, dim3((unsigned)mp), dim3(256), 0, s, Kc,
-                         nbw, c0, ncol, y, theta, nls, c0 > 0 ? 1 : 0, b, qv);
-    } else {
-      double* Kc = at<double>(ws, Lo.off_kc);
-      GPK_TRY(launch_kernel_matrix<double>(kind, z, m, x + c0 * d, ncol, d, theta, nls, 0.0,
-                                           KM_ZERO_PAD, Kc, nbw, mp, nbw, false, s));
-      GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, false, Kc, nbw, Kc, nbw, Pp,
-                                                  nbw, mp, mp, kz, 1.0, c0 > 0, true, KR_FULL, s,
-                                                  nullptr, sk));
-      hipLaunchKernelGGL(chunk_rowsum_kernel<double>, dim3((unsigned)mp), dim3(256), 0, s, Kc,
-                         nbw, c0, ncol, y, theta, nls, c0 > 0 ? 1 : 0, b, qv);
-    }
+  float* fs = at<float>(ws, Lo.off_fs);
+  if (ea.f32 && ea.fwd == AR_FP16X3) {
+    hipLaunchKernelGGL(elbo_fp16_scale_kernel, dim3(1), dim3(64), 0, s, theta, nls, nullptr, 0,
+                       fs);
     GPK_LAUNCH_CHECK();
   }
+  double* T = at<double>(ws, Lo.off_T);
+  double* Kb = at<double>(ws, Lo.off_K);
+  double* Kinv = at<double>(ws, Lo.off_Kinv);
+  Factor<double> FK{Kb, at<double>(ws, Lo.off_XK), mp, T, at<double>(ws, Lo.off_liK),
+                    at<double>(ws, Lo.off_ldK), at<int>(ws, Lo.off_ifK), AR_F64, AR_F64, s};
+  // The Kzz branch (Kzz + jitter I with identity pad, its factor, inverse factor
+  // and Kzz^-1 = X^T X, lower tiles then mirrored) needs nothing from the data:
+  // it runs on a library stream (aux stream 0, events 6 and 7) while the chunks
+  // stream through, and is joined back into s at the end of this scope (an
+  // early error return included), before C is factored: the two factorisations
+  // share the scratch T.
+  {
+    hipStream_t st[AUX_STREAMS];
+    hipEvent_t ev[AUX_EVENTS];
+    GPK_TRY(aux_streams(st, ev));
+    hipStream_t s2 = st[0];
+    GPK_HIP_TRY(hipEventRecord(ev[6], s));
+    GPK_HIP_TRY(hipStreamWaitEvent(s2, ev[6], 0));
+    struct Join {
+      hipStream_t s, s2;
+      hipEvent_t e;
+      ~Join() {
+        if (hipEventRecord(e, s2) == hipSuccess) (void)hipStreamWaitEvent(s, e, 0);
+      }
+    } join{s, s2, ev[7]};
+    FK.s = s2;
+    GPK_TRY(launch_kernel_matrix<double>(kind, z, m, z, m, d, theta, nls, jitter, 1, Kb, mp, mp,
+                                         mp, true, s2));
+    GPK_TRY(FK.potrf(0, mp));
+    GPK_TRY(FK.trtri(0, mp));
+    GPK_TRY(launch_gemm<double, double, double>(AR_F64, true, true, FK.X, mp, FK.X, mp, Kinv, mp,
+                                                mp, mp, mp, 1.0, false, true, KR_GE_M, s2));
+    hipLaunchKernelGGL(mirror_kernel, dim3(cdiv(mp * mp, 256)), dim3(256), 0, s2, Kinv, mp, mp);
+    GPK_LAUNCH_CHECK();
+    FK.s = s;
+    for (int64_t c0 = 0; c0 < n; c0 += BW_CHUNK) {
+      const int64_t ncol = (n - c0 < BW_CHUNK) ? n - c0 : BW_CHUNK;
+      if (ea.f32) {
+        float* Kc = at<float>(ws, Lo.off_kc);
+        GPK_TRY(launch_cross_f32(kind, z, m, x + c0 * d, ncol, d, theta, nls, Kc, nbw, mp, nbw,
+                                 s));
+        GPK_TRY(launch_gemm<float, float, double>(ea.fwd, false, false, Kc, nbw, Kc, nbw, Pp, nbw,
+                                                  mp, mp, kz, 1.0, c0 > 0, true, KR_FULL, s,
+                                                  nullptr, sk, fs));
+        hipLaunchKernelGGL(chunk_rowsum_kernel<float>, dim3((unsigned)mp), dim3(256), 0, s, Kc,
+                           nbw, c0, ncol, y, theta, nls, c0 > 0 ? 1 : 0, b, qv);
+      } else {
+        double* Kc = at<double>(ws, Lo.off_kc);
+        GPK_TRY(launch_kernel_matrix<double>(kind, z, m, x + c0 * d, ncol, d, theta, nls, 0.0,
+                                             KM_ZERO_PAD, Kc, nbw, mp, nbw, false, s));
+        GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, false, Kc, nbw, Kc, nbw, Pp,
+                                                    nbw, mp, mp, kz, 1.0, c0 > 0, true, KR_FULL, s,
+                                                    nullptr, sk));
+        hipLaunchKernelGGL(chunk_rowsum_kernel<double>, dim3((unsigned)mp), dim3(256), 0, s, Kc,
+                           nbw, c0, ncol, y, theta, nls, c0 > 0 ? 1 : 0, b, qv);
+      }
+      GPK_LAUNCH_CHECK();
+    }
+  }  // joins the Kzz branch
   hipLaunchKernelGGL(psum_kernel, dim3(cdiv(mp * mp, 256)), dim3(256), 0, s, Pp, nbw, kp, S, P,
                      mp);
   GPK_LAUNCH_CHECK();
   hipLaunchKernelGGL(rstats_kernel, dim3((unsigned)Lo.nr), dim3(256), 0, s, y, n, theta, nls,
                      at<double>(ws, Lo.off_rpart));
   GPK_LAUNCH_CHECK();
-  // 2. Kzz + jitter I and C = Kzz + P / s, both with identity pad; factor + invert
-  double* Kb = at<double>(ws, Lo.off_K);
+  // 2. C = Kzz + jitter I + P / s with identity pad; factor + invert
   double* Cb = at<double>(ws, Lo.off_C);
-  GPK_TRY(launch_kernel_matrix<double>(kind, z, m, z, m, d, theta, nls, jitter, 1, Kb, mp, mp,
-                                       mp, true, s));
   GPK_TRY(launch_kernel_matrix<double>(kind, z, m, z, m, d, theta, nls, jitter, 1, Cb, mp, mp,
                                        mp, true, s));
   hipLaunchKernelGGL(add_p_kernel, dim3(cdiv(mp * mp, 256)), dim3(256), 0, s, Cb, P, mp, mp,
                      theta, nls);
   GPK_LAUNCH_CHECK();
-  double* T = at<double>(ws, Lo.off_T);
-  Factor<double> FK{Kb, at<double>(ws, Lo.off_XK), mp, T, at<double>(ws, Lo.off_liK),
-                    at<double>(ws, Lo.off_ldK), at<int>(ws, Lo.off_ifK), AR_F64, AR_F64, s};
   Factor<double> FC{Cb, at<double>(ws, Lo.off_XC), mp, T, at<double>(ws, Lo.off_liC),
                     at<double>(ws, Lo.off_ldC), at<int>(ws, Lo.off_ifC), AR_F64, AR_F64, s};
-  GPK_TRY(FK.potrf(0, mp));
-  GPK_TRY(FK.trtri(0, mp));
   GPK_TRY(FC.potrf(0, mp));
   GPK_TRY(FC.trtri(0, mp));
   // 3. c = C^-1 b = X_C^T X_C b
   GPK_TRY(launch_trmv<double>(FC.X, mp, mp, b, u, false, vpart, s));
   GPK_TRY(launch_trmv<double>(FC.X, mp, mp, u, cvec, true, vpart, s));
-  // 4. Kzz^-1 and C^-1 = X^T X (lower tiles, then mirrored)
-  double* Kinv = at<double>(ws, Lo.off_Kinv);
+  // 4. C^-1 = X^T X (lower tiles, then mirrored; Kzz^-1 came with the Kzz branch)
   double* Cinv = at<double>(ws, Lo.off_Cinv);
-  GPK_TRY(launch_gemm<double, double, double>(AR_F64, true, true, FK.X, mp, FK.X, mp, Kinv, mp,
-                                              mp, mp, mp, 1.0, false, true, KR_GE_M, s));
   GPK_TRY(launch_gemm<double, double, double>(AR_F64, true, true, FC.X, mp, FC.X, mp, Cinv, mp,
                                               mp, mp, mp, 1.0, false, true, KR_GE_M, s));
-  hipLaunchKernelGGL(mirror_kernel, dim3(cdiv(mp * mp, 256)), dim3(256), 0, s, Kinv, mp, mp);
   hipLaunchKernelGGL(mirror_kernel, dim3(cdiv(mp * mp, 256)), dim3(256), 0, s, Cinv, mp, mp);
   GPK_LAUNCH_CHECK();
   // 5. tr(Kzz^-1 P) and the objective (mixed: with the error model's ||G_P o P||_F)
@@ -749,8 +831,11 @@ int elbo_forward_impl(int kind, const ElboArith& ea, const double* x, const doub
                        P, mp, theta, nls, epart);
     GPK_LAUNCH_CHECK();
   }
-  // The error model is calibrated for six-product (or float64) products only.
-  const int uncal = ea.f32 && (ea.fwd == AR_BF16X3 || ea.bwd == AR_BF16X3) ? 1 : 0;
+  // The error model is calibrated for six-product (or float64) statistics
+  // products and six-product, fp16-split (or float64) data-pass products.
+  const bool calf = ea.fwd == AR_BF16X6 || ea.fwd == AR_F64;
+  const bool calb = ea.bwd == AR_BF16X6 || ea.bwd == AR_FP16X3 || ea.bwd == AR_F64;
+  const int uncal = ea.f32 && !(calf && calb) ? 1 : 0;
   hipLaunchKernelGGL(elbo_fwd_finalize, dim3(1), dim3(256), 0, s, at<double>(ws, Lo.off_rpart),
                      (int)Lo.nr, fpart, (int)Lo.nf, at<double>(ws, Lo.off_ldK),
                      at<int>(ws, Lo.off_ifK), at<double>(ws, Lo.off_ldC), at<int>(ws, Lo.off_ifC),
@@ -783,58 +868,96 @@ int elbo_backward_impl(int kind, const ElboArith& ea, const double* x, const dou
   hipLaunchKernelGGL(elbo_bwd_scalars, dim3(1), dim3(256), 0, s, fpart, (int)Lo.nf, cvec, Pc, mp,
                      scal);
   GPK_LAUNCH_CHECK();
-  // G_P and the data pass
-  // the float32 G_P of the mixed data pass lives in T1 (free until G_K)
-  float* Gf = ea.f32 ? at<float>(ws, Lo.off_T1) : nullptr;
-  hipLaunchKernelGGL(gp_kernel, dim3(cdiv(mp * mp, 256)), dim3(256), 0, s, G, Gf, Kinv, Cinv,
-                     cvec, mp, mp, theta, nls);
-  GPK_LAUNCH_CHECK();
-  // Data pass, chunk by chunk: Kc = Kzx for the chunk's points (zero outside
-  // m x ncol), Wt = Kc^T G_P (float64) on the matrix cores, then the
-  // elementwise contraction against dKzx.
-  double* Wt = at<double>(ws, Lo.off_wt);
-  for (int64_t c0 = 0; c0 < n; c0 += BW_CHUNK) {
-    const int64_t ncol = (n - c0 < BW_CHUNK) ? n - c0 : BW_CHUNK;
-    const int64_t ncp = round_up(ncol, TILE);
-    if (ea.f32) {
-      float* Kc = at<float>(ws, Lo.off_kc);
-      GPK_TRY(launch_kernel_matrix<float>(kind, z, m, x + c0 * d, ncol, d, theta, nls, 0.0,
-                                          KM_ZERO_PAD, Kc, ncp, mp, ncp, false, s));
-      GPK_TRY(launch_gemm<float, float, double>(ea.bwd, true, false, Kc, ncp, Gf, mp, Wt, mp,
-                                                ncp, mp, mp, 1.0, false, false, KR_FULL, s));
-    } else {
-      double* Kc = at<double>(ws, Lo.off_kc);
-      GPK_TRY(launch_kernel_matrix<double>(kind, z, m, x + c0 * d, ncol, d, theta, nls, 0.0,
-                                           KM_ZERO_PAD, Kc, ncp, mp, ncp, false, s));
-      GPK_TRY(launch_gemm<double, double, double>(AR_F64, true, false, Kc, ncp, G, mp, Wt, mp,
-                                                  ncp, mp, mp, 1.0, false, false, KR_FULL, s));
+  // The Kzz-side terms (G_K = Kinv - Cinv - c c^T / s^2 - Kinv P Kinv / s into
+  // T1, then tr(G_K dKzz) for every theta and Z) need only the forward's
+  // matrices: they run on a library stream (aux stream 0, events 6 and 7)
+  // during the data pass and are joined back (an early error return included)
+  // before the gradients are assembled.
+  hipStream_t st[AUX_STREAMS];
+  hipEvent_t ev[AUX_EVENTS];
+  GPK_TRY(aux_streams(st, ev));
+  hipStream_t s2 = st[0];
+  GPK_HIP_TRY(hipEventRecord(ev[6], s));
+  GPK_HIP_TRY(hipStreamWaitEvent(s2, ev[6], 0));
+  struct Join {
+    hipStream_t s, s2;
+    hipEvent_t e;
+    ~Join() {
+      if (hipEventRecord(e, s2) == hipSuccess) (void)hipStreamWaitEvent(s, e, 0);
     }
-    dim3 grid((unsigned)Lo.mt, (unsigned)((ncol + EW - 1) / EW));
-#define GPK_E3(DM)                                                                              \
-  hipLaunchKernelGGL(elbo_wgrad_kernel<DM>, grid, dim3(256), 0, s, kind, x, y, z, m, d, theta, \
-                     nls, Wt, mp, c0, ncol, cvec, at<double>(ws, Lo.off_gzpart),                \
-                     at<double>(ws, Lo.off_tpart))
-    GPK_DMAX_DISPATCH(d, GPK_E3);
-#undef GPK_E3
-    GPK_LAUNCH_CHECK();
-  }
-  // G_K = Kinv - Cinv - c c^T / s^2 - Kinv P Kinv / s (G is reused: the data pass is done)
-  GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, false, Kinv, mp, P, mp, T1, mp, mp,
-                                              mp, mp, 1.0, false, false, KR_FULL, s));
-  GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, false, T1, mp, Kinv, mp, T2, mp, mp,
-                                              mp, mp, 1.0, false, false, KR_FULL, s));
-  hipLaunchKernelGGL(gk_kernel, dim3(cdiv(mp * mp, 256)), dim3(256), 0, s, G, Kinv, Cinv, T2,
-                     cvec, mp, mp, theta, nls);
-  GPK_LAUNCH_CHECK();
+  };
   {
-#define GPK_E4(DM)                                                                            \
-  hipLaunchKernelGGL(elbo_kzz_adj_kernel<DM>, dim3((unsigned)(mp / 256)), dim3(256), 0, s, kind, \
-                     z, m, d, theta, nls, G, mp, at<double>(ws, Lo.off_gzkz),                  \
+    Join join{s, s2, ev[7]};
+    GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, false, Kinv, mp, P, mp, T1, mp,
+                                                mp, mp, mp, 1.0, false, false, KR_FULL, s2));
+    GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, false, T1, mp, Kinv, mp, T2, mp,
+                                                mp, mp, mp, 1.0, false, false, KR_FULL, s2));
+    hipLaunchKernelGGL(gk_kernel, dim3(cdiv(mp * mp, 256)), dim3(256), 0, s2, T1, Kinv, Cinv, T2,
+                       cvec, mp, mp, theta, nls);
+    GPK_LAUNCH_CHECK();
+#define GPK_E4(DM)                                                                              \
+  hipLaunchKernelGGL(elbo_kzz_adj_kernel<DM>, dim3((unsigned)(mp / 256)), dim3(256), 0, s2, kind, \
+                     z, m, d, theta, nls, T1, mp, at<double>(ws, Lo.off_gzkz),                   \
                      at<double>(ws, Lo.off_tpart2))
     GPK_DMAX_DISPATCH(d, GPK_E4);
 #undef GPK_E4
     GPK_LAUNCH_CHECK();
-  }
+    // G_P and the data pass
+    // the float32 G_P of the mixed data pass lives in XK (the inverse factor of
+    // Kzz, which no backward step reads); the fp16 split's max |G_P| partials in
+    // epart (free after the forward)
+    float* Gf = ea.f32 ? at<float>(ws, Lo.off_XK) : nullptr;
+    const bool f16 = ea.f32 && ea.bwd == AR_FP16X3;
+    double* gmax = f16 ? at<double>(ws, Lo.off_epart) : nullptr;
+    float* fs = at<float>(ws, Lo.off_fs);
+    hipLaunchKernelGGL(gp_kernel, dim3((unsigned)Lo.nf), dim3(256), 0, s, G, Gf, Kinv, Cinv, cvec,
+                       mp, theta, nls, gmax);
+    GPK_LAUNCH_CHECK();
+    if (f16) {
+      hipLaunchKernelGGL(elbo_fp16_scale_kernel, dim3(1), dim3(64), 0, s, theta, nls, gmax,
+                         (int)Lo.nf, fs);
+      GPK_LAUNCH_CHECK();
+    }
+    // Data pass, chunk by chunk: Kc = Kzx for the chunk's points (zero outside
+    // m x ncol), Wt = Kc^T G_P on the matrix cores (float64 sums, stored in
+    // float64, or float32 in the mixed path), then the elementwise contraction
+    // against dKzx.
+    double* Wt = at<double>(ws, Lo.off_wt);
+    float* Wtf = at<float>(ws, Lo.off_wt);
+    for (int64_t c0 = 0; c0 < n; c0 += BW_CHUNK) {
+      const int64_t ncol = (n - c0 < BW_CHUNK) ? n - c0 : BW_CHUNK;
+      const int64_t ncp = round_up(ncol, TILE);
+      if (ea.f32) {
+        float* Kc = at<float>(ws, Lo.off_kc);
+        GPK_TRY(launch_cross_f32(kind, z, m, x + c0 * d, ncol, d, theta, nls, Kc, ncp, mp, ncp,
+                                 s));
+        GPK_TRY(launch_gemm<float, float, float>(ea.bwd, true, false, Kc, ncp, Gf, mp, Wtf, mp,
+                                                 ncp, mp, mp, 1.0, false, false, KR_FULL, s,
+                                                 nullptr, Batch(), fs + 2));
+      } else {
+        double* Kc = at<double>(ws, Lo.off_kc);
+        GPK_TRY(launch_kernel_matrix<double>(kind, z, m, x + c0 * d, ncol, d, theta, nls, 0.0,
+                                             KM_ZERO_PAD, Kc, ncp, mp, ncp, false, s));
+        GPK_TRY(launch_gemm<double, double, double>(AR_F64, true, false, Kc, ncp, G, mp, Wt, mp,
+                                                    ncp, mp, mp, 1.0, false, false, KR_FULL, s));
+      }
+      dim3 grid((unsigned)Lo.mt, (unsigned)((ncol + EW - 1) / EW));
+#define GPK_E3(DM)                                                                         \
+  do {                                                                                     \
+    if (ea.f32)                                                                            \
+      hipLaunchKernelGGL((elbo_wgrad_kernel<DM, float>), grid, dim3(256), 0, s, kind, x, y, \
+                         z, m, d, theta, nls, Wtf, mp, c0, ncol, cvec,                     \
+                         at<double>(ws, Lo.off_gzpart), at<double>(ws, Lo.off_tpart));     \
+    else                                                                                   \
+      hipLaunchKernelGGL((elbo_wgrad_kernel<DM, double>), grid, dim3(256), 0, s, kind, x,  \
+                         y, z, m, d, theta, nls, Wt, mp, c0, ncol, cvec,                   \
+                         at<double>(ws, Lo.off_gzpart), at<double>(ws, Lo.off_tpart));     \
+  } while (0)
+      GPK_DMAX_DISPATCH(d, GPK_E3);
+#undef GPK_E3
+      GPK_LAUNCH_CHECK();
+    }
+  }  // joins the Kzz-side terms
   int dmax = 32;
   if (d <= 1) dmax = 1;
   else if (d <= 2) dmax = 2;

@@ -123,6 +123,21 @@ __device__ __forceinline__ double kval(int kind, double d2) {
   return (1.0 + s5 * t + (5.0 / 3.0) * t * t) * exp(-s5 * t);
 }
 
+// kval in float32 arithmetic (rho^2 formed in float64 by the caller), for
+// entries that are stored in float32 anyway: a few float32 ulps of relative
+// error instead of the half ulp of rounding a float64 value.
+__device__ __forceinline__ float kvalf(int kind, float d2) {
+  if (kind == GPK_KERNEL_RBF) return expf(-0.5f * d2);
+  const float t = sqrtf(fmaxf(d2, 1e-36f));
+  if (kind == GPK_KERNEL_MATERN12) return expf(-t);
+  if (kind == GPK_KERNEL_MATERN32) {
+    const float s3 = 1.7320508f;
+    return (1.0f + s3 * t) * expf(-s3 * t);
+  }
+  const float s5 = 2.2360680f;
+  return (1.0f + s5 * t + (5.0f / 3.0f) * t * t) * expf(-s5 * t);
+}
+
 // Both k(rho) and F(rho) with one exponential.
 __device__ __forceinline__ void kval_fac(int kind, double d2, double& k, double& f) {
   if (kind == GPK_KERNEL_RBF) {
@@ -150,6 +165,33 @@ __device__ __forceinline__ void kval_fac(int kind, double d2, double& k, double&
   f = (5.0 / 3.0) * (1.0 + s5 * t) * e;
 }
 
+// kval_fac in float32 arithmetic (the mixed collapsed ELBO's gradient pass).
+__device__ __forceinline__ void kval_facf(int kind, float d2, float& k, float& f) {
+  if (kind == GPK_KERNEL_RBF) {
+    k = expf(-0.5f * d2);
+    f = k;
+    return;
+  }
+  const float t = sqrtf(fmaxf(d2, 1e-36f));
+  if (kind == GPK_KERNEL_MATERN12) {
+    const float e = expf(-t);
+    k = e;
+    f = d2 > 1e-36f ? e / t : 0.0f;
+    return;
+  }
+  if (kind == GPK_KERNEL_MATERN32) {
+    const float s3 = 1.7320508f;
+    const float e = expf(-s3 * t);
+    k = (1.0f + s3 * t) * e;
+    f = 3.0f * e;
+    return;
+  }
+  const float s5 = 2.2360680f;
+  const float e = expf(-s5 * t);
+  k = (1.0f + s5 * t + (5.0f / 3.0f) * t * t) * e;
+  f = (5.0f / 3.0f) * (1.0f + s5 * t) * e;
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers (each defined next to its kernels).
 // ---------------------------------------------------------------------------
@@ -171,6 +213,14 @@ int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, 
                          TS* out, int64_t ldo, int64_t rows_total, int64_t cols_total,
                          bool lower_only, hipStream_t s, double* dblk = nullptr,
                          const Batch& bt = Batch(), double* spk = nullptr);
+
+// Cross-covariance K[i][j] = k(x_i, z_j) (i < n, j < m; zero elsewhere up to
+// rows_total x cols_total) in float32, the kernel function evaluated in float32
+// arithmetic from float64 squared distances (kvalf): the mixed collapsed ELBO's
+// Kzx chunks (x = inducing inputs, z = the chunk's points).
+int launch_cross_f32(int kind, const double* x, int64_t n, const double* z, int64_t m, int d,
+                     const double* theta, int nls, float* out, int64_t ldo, int64_t rows_total,
+                     int64_t cols_total, hipStream_t s);
 
 // Leaf: factor the NB x NB diagonal block at (o, o) of A in float64 in LDS,
 // write L (lower, zero upper) to A, L^-1 to X (lower, zero upper) and to

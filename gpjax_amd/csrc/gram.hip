@@ -25,7 +25,8 @@ namespace {
 constexpr int KT = 64;    // output tile edge
 constexpr int DCH = 32;   // input dimensions staged per LDS pass
 
-template <typename TS>
+// F32K (float32 output only): the kernel function in float32 arithmetic (kvalf).
+template <typename TS, bool F32K = false>
 __global__ __launch_bounds__(256) void kmat_kernel(int kind, const double* __restrict__ x,
                                                    int64_t n, const double* __restrict__ z,
                                                    int64_t m, int d,
@@ -104,6 +105,10 @@ __global__ __launch_bounds__(256) void kmat_kernel(int kind, const double* __res
     const int64_t i = i0 + rg * 16 + r;
     if (i >= rows_total) continue;
     double v;
+    if constexpr (F32K) {
+      out[i * ldo + j] = (i < n && j < m) ? (float)var * kvalf(kind, (float)d2[r]) : 0.0f;
+      continue;
+    }
     if (i < n && j < m) {
       v = var * kval(kind, d2[r]);
       const int dm = diag_mode & KM_DIAG_MASK;
@@ -155,6 +160,23 @@ int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, 
   hipLaunchKernelGGL(kmat_kernel<TS>, grid, dim3(256), 0, s, kind, x, n, z, m, d, theta, nls,
                      jitter, diag_mode, out, ldo, rows_total, cols_total, lower_only ? 1 : 0,
                      dblk, lower_only ? spk : nullptr, bt);
+  GPK_LAUNCH_CHECK();
+  return GPK_OK;
+}
+
+int launch_cross_f32(int kind, const double* x, int64_t n, const double* z, int64_t m, int d,
+                     const double* theta, int nls, float* out, int64_t ldo, int64_t rows_total,
+                     int64_t cols_total, hipStream_t s) {
+  if (rows_total <= 0 || cols_total <= 0) return GPK_OK;
+  const int64_t gy = (rows_total + KT - 1) / KT;
+  if (gy > 65535) {
+    set_error("kernel matrix: %lld rows exceed one launch", (long long)rows_total);
+    return GPK_ERR_BAD_ARG;
+  }
+  dim3 grid((unsigned)((cols_total + KT - 1) / KT), (unsigned)gy, 1);
+  hipLaunchKernelGGL((kmat_kernel<float, true>), grid, dim3(256), 0, s, kind, x, n, z, m, d,
+                     theta, nls, 0.0, KM_ZERO_PAD, out, ldo, rows_total, cols_total, 0, nullptr,
+                     nullptr, Batch());
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }

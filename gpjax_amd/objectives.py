@@ -250,9 +250,10 @@ class _CollapsedELBO(torch.autograd.Function):
 
 def collapsed_elbo_with_info(variational_family, data, precision: str | None = None):
     """(elbo, result) -- result is the device buffer of ``gpk_elbo_forward``.
-    ``precision``: "fp64", "mixed" (float32 Kzx chunks, six-product bf16 splits
-    for the two N-streamed products) or "mixed:P,T,L" (P: the product
-    Kzx Kxz, T: the data pass Kxz G; L unused); default ``config.precision``."""
+    ``precision``: "fp64", "mixed" (float32 Kzx chunks, P = Kzx Kxz as
+    six-product bf16 splits, the gradient pass's Kxz G_P as the fp16 split) or
+    "mixed:P,T,L" (P: the product Kzx Kxz, T: the gradient pass's; L unused);
+    default ``config.precision``."""
     post = variational_family.posterior
     kernel = post.prior.kernel
     X = kernel.slice_input(data.X)

@@ -193,16 +193,18 @@ size_t gpk_elbo_workspace_bytes(int64_t n, int64_t m, int64_t d, int prec);
  * of 65,536 points into the workspace): no n x m matrix is stored.  result
  * slots: [GPK_RES_MLL] = ELBO, [1] = log|B|, [2] = the quadratic term, [3] = the
  * trace term, [GPK_RES_INFO] = pivot status.  d <= 32.
- * prec: GPK_PREC_FP64, or GPK_PREC_MIXED: the Kzx chunks in float32 (each entry
- * evaluated in float64, rounded once) and the two n-streamed products (P and
- * the backward's Kxz G_P) as six-product bf16 splits with float64 sums; the
- * m x m algebra stays float64.  GPK_PREC_MIXED_ARITH(p, t, l) names the
- * products instead: p for P, t for the backward's, each 0, 3 or 6 (l unused).
+ * prec: GPK_PREC_FP64, or GPK_PREC_MIXED: the Kzx chunks in float32 (squared
+ * distances in float64, the kernel function in float32), P = Kzx Kxz as
+ * six-product bf16 splits and the backward's Kxz G_P as the fp16 split, both
+ * with float64 sums; the m x m algebra stays float64.
+ * GPK_PREC_MIXED_ARITH(p, t, l) names the two products instead: p for P, t for
+ * the backward's, each 0, 3, 4 or 6 (l unused).
  * Mixed only: [GPK_RES_MLL_ERR] = 2^-24 ||G_P o P||_F / sqrt(2), a model of
  * the ELBO's error from the split products (G_P = dELBO / dP, measured errors
  * <= 1.2x it), and [GPK_RES_FLAG] = 1 when 5x it exceeds 1e-6 |ELBO|, the
- * factorisation broke down, or either product uses three-product splits (not
- * calibrated): recompute in GPK_PREC_FP64.  The same workspace size serves both.
+ * factorisation broke down, or the products are ones the model is not
+ * calibrated for (p other than 6 or 0, t = 3): recompute in GPK_PREC_FP64.
+ * The same workspace size serves both.
  */
 int gpk_elbo_forward(int kind, int prec, const double* x, const double* y, int64_t n,
                      const double* z, int64_t m, int64_t d, const double* theta, int nls,

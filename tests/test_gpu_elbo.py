@@ -236,9 +236,13 @@ MIXED_CASES = [("rbf", 500, 40, 2, True, (0.05, 0.15), 0.3, False),
                ("matern32", 40000, 1024, 8, True, (0.1, 0.5), 0.2, True)]
 
 
+@pytest.mark.parametrize("precision", ["mixed", "mixed:6,6,6"])
 @pytest.mark.parametrize("kind,n,m,d,ard,lsr,sd,passes", MIXED_CASES)
-def test_mixed_elbo_in_gate_or_flagged(cuda, monkeypatch, kind, n, m, d, ard, lsr, sd, passes):
-    """Mixed ELBO (float32 Kzx chunks, six-product bf16 splits): every result the
+def test_mixed_elbo_in_gate_or_flagged(cuda, monkeypatch, kind, n, m, d, ard, lsr, sd, passes,
+                                       precision):
+    """Mixed ELBO (float32 Kzx chunks, P = Kzx Kxz as six-product bf16 splits,
+    the gradient pass's product as the fp16 split or, "mixed:6,6,6", six-product
+    bf16 splits): every result the
     a-posteriori check lets through is within 1e-6 of the float64 ELBO, 1e-4
     (relative, norm-wise) of its theta gradient and 1e-4 of the largest entry of
     its inducing-input gradient; the float64 path stands in for the oracle here
@@ -261,7 +265,7 @@ def test_mixed_elbo_in_gate_or_flagged(cuda, monkeypatch, kind, n, m, d, ard, ls
     q = gpx.variational_families.CollapsedVariationalGaussian(posterior=post, inducing_inputs=Z)
     D = gpx.Dataset(X=torch.tensor(X, device=cuda), y=torch.tensor(y[:, None], device=cuda))
     e0, g0, z0, r0 = _elbo_and_grads(q, D, k, post, "fp64")
-    e1, g1, z1, r1 = _elbo_and_grads(q, D, k, post, "mixed")
+    e1, g1, z1, r1 = _elbo_and_grads(q, D, k, post, precision)
     assert r0[gpx._lib.RES_FLAG] == 0.0 and r0[gpx._lib.RES_MLL_ERR] == 0.0
     flag, est = r1[gpx._lib.RES_FLAG], r1[gpx._lib.RES_MLL_ERR]
     rel_e = abs(e1 - e0) / abs(e0)
