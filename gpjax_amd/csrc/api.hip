@@ -757,8 +757,8 @@ int predict_t(int kind, const Arith3& ar, const double* x, const double* y, int6
     for (int64_t t0 = 0; t0 < t; t0 += P.tc) {
       const int64_t tn = (t - t0 < P.tc) ? t - t0 : P.tc;
       const int64_t tnp = round_up(tn, NB);
-      GPK_TRY(launch_kernel_matrix<double>(kind, x, n, xt + t0 * d, tn, d, theta, nls, 0.0,
-                                           KM_ZERO_PAD, kxt, tnp, np, tnp, false, s));
+      GPK_TRY(launch_cross<double>(kind, x, n, xt + t0 * d, tn, d, theta, nls, false, kxt, tnp,
+                                   np, tnp, s));
       GPK_TRY(launch_colreduce(kxt, tnp, np, tnp, alpha, vec, part, s));
       hipLaunchKernelGGL(pred_mean_kernel, dim3(nblk(tn)), dim3(256), 0, s, vec, theta, nls, tn,
                          mean + t0);
@@ -768,8 +768,8 @@ int predict_t(int kind, const Arith3& ar, const double* x, const double* y, int6
                                                 tnp, np, 1.0, false, false, KR_LE_M, s));
         GPK_TRY(launch_colreduce(A, tnp, np, tnp, nullptr, vec, part, s));
       } else {
-        GPK_TRY(launch_kernel_matrix<float>(kind, x, n, xt + t0 * d, tn, d, theta, nls, 0.0,
-                                            KM_ZERO_PAD, kxtf, tnp, np, tnp, false, s));
+        GPK_TRY(launch_cross<float>(kind, x, n, xt + t0 * d, tn, d, theta, nls, false, kxtf,
+                                    tnp, np, tnp, s));
         GPK_TRY(launch_gemm<float, float, double>(AR_BF16X6, false, true,
                                                   reinterpret_cast<const float*>(X), np, kxtf,
                                                   tnp, A, tnp, np, tnp, np, 1.0, false, false,
@@ -783,8 +783,7 @@ int predict_t(int kind, const Arith3& ar, const double* x, const double* y, int6
     return GPK_OK;
   }
   // Kxt (gps.py:515), zero outside n x t
-  GPK_TRY(launch_kernel_matrix<double>(kind, x, n, xt, t, d, theta, nls, 0.0, KM_ZERO_PAD, kxt,
-                                       tp, np, tp, false, s));
+  GPK_TRY(launch_cross<double>(kind, x, n, xt, t, d, theta, nls, false, kxt, tp, np, tp, s));
   // mean = m(t) + Kxt^T alpha  (= m(t) + (L^-1 Kxt)^T L^-1 (y - m), gps.py:517-520)
   GPK_TRY(launch_colreduce(kxt, tp, np, tp, alpha, vec, part, s));
   hipLaunchKernelGGL(pred_mean_kernel, dim3(nblk(t)), dim3(256), 0, s, vec, theta, nls, t, mean);

@@ -186,23 +186,25 @@ __global__ __launch_bounds__(256) void gp_kernel(double* __restrict__ G, float* 
 // kernel here peaks at var); gmax: nf partials of max |G_P| (or null).
 // fs[0] = fs[1] = scale of Kzx (statistics product), fs[2] = scale of Kzx and
 // fs[3] = scale of G_P (data-pass product).
+// One wave of 64 threads.
 __global__ void elbo_fp16_scale_kernel(const double* __restrict__ theta, int nls,
                                        const double* __restrict__ gmax, int nf,
                                        float* __restrict__ fs) {
-  if (threadIdx.x != 0) return;
   auto sc = [](double b) {
     b = fmin(fmax(b, 0x1p-60), 0x1p60);
     return (float)ldexp(1.0, (int)floor(log2(16384.0 / b)));
   };
+  double g = 0.0;
+  if (gmax)
+    for (int k = threadIdx.x; k < nf; k += 64) g = fmax(g, gmax[k]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) g = fmax(g, __shfl_xor(g, off));
+  if (threadIdx.x != 0) return;
   const float sk = sc(theta[nls]);
   fs[0] = sk;
   fs[1] = sk;
   fs[2] = sk;
-  if (gmax) {
-    double g = 0.0;
-    for (int k = 0; k < nf; ++k) g = fmax(g, gmax[k]);
-    fs[3] = sc(g);
-  }
+  if (gmax) fs[3] = sc(g);
 }
 
 // G_K = Kinv - Cinv - c c^T / s^2 - T2 / s,  T2 = Kinv P Kinv
@@ -773,8 +775,8 @@ int elbo_forward_impl(int kind, const ElboArith& ea, const double* x, const doub
       const int64_t ncol = (n - c0 < BW_CHUNK) ? n - c0 : BW_CHUNK;
       if (ea.f32) {
         float* Kc = at<float>(ws, Lo.off_kc);
-        GPK_TRY(launch_cross_f32(kind, z, m, x + c0 * d, ncol, d, theta, nls, Kc, nbw, mp, nbw,
-                                 s));
+        GPK_TRY(launch_cross<float>(kind, z, m, x + c0 * d, ncol, d, theta, nls, true, Kc, nbw,
+                                    mp, nbw, s));
         GPK_TRY(launch_gemm<float, float, double>(ea.fwd, false, false, Kc, nbw, Kc, nbw, Pp, nbw,
                                                   mp, mp, kz, 1.0, c0 > 0, true, KR_FULL, s,
                                                   nullptr, sk, fs));
@@ -782,8 +784,8 @@ int elbo_forward_impl(int kind, const ElboArith& ea, const double* x, const doub
                            nbw, c0, ncol, y, theta, nls, c0 > 0 ? 1 : 0, b, qv);
       } else {
         double* Kc = at<double>(ws, Lo.off_kc);
-        GPK_TRY(launch_kernel_matrix<double>(kind, z, m, x + c0 * d, ncol, d, theta, nls, 0.0,
-                                             KM_ZERO_PAD, Kc, nbw, mp, nbw, false, s));
+        GPK_TRY(launch_cross<double>(kind, z, m, x + c0 * d, ncol, d, theta, nls, false, Kc, nbw,
+                                     mp, nbw, s));
         GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, false, Kc, nbw, Kc, nbw, Pp,
                                                     nbw, mp, mp, kz, 1.0, c0 > 0, true, KR_FULL, s,
                                                     nullptr, sk));
@@ -929,15 +931,15 @@ int elbo_backward_impl(int kind, const ElboArith& ea, const double* x, const dou
       const int64_t ncp = round_up(ncol, TILE);
       if (ea.f32) {
         float* Kc = at<float>(ws, Lo.off_kc);
-        GPK_TRY(launch_cross_f32(kind, z, m, x + c0 * d, ncol, d, theta, nls, Kc, ncp, mp, ncp,
-                                 s));
+        GPK_TRY(launch_cross<float>(kind, z, m, x + c0 * d, ncol, d, theta, nls, true, Kc, ncp,
+                                    mp, ncp, s));
         GPK_TRY(launch_gemm<float, float, float>(ea.bwd, true, false, Kc, ncp, Gf, mp, Wtf, mp,
                                                  ncp, mp, mp, 1.0, false, false, KR_FULL, s,
                                                  nullptr, Batch(), fs + 2));
       } else {
         double* Kc = at<double>(ws, Lo.off_kc);
-        GPK_TRY(launch_kernel_matrix<double>(kind, z, m, x + c0 * d, ncol, d, theta, nls, 0.0,
-                                             KM_ZERO_PAD, Kc, ncp, mp, ncp, false, s));
+        GPK_TRY(launch_cross<double>(kind, z, m, x + c0 * d, ncol, d, theta, nls, false, Kc, ncp,
+                                     mp, ncp, s));
         GPK_TRY(launch_gemm<double, double, double>(AR_F64, true, false, Kc, ncp, G, mp, Wt, mp,
                                                     ncp, mp, mp, 1.0, false, false, KR_FULL, s));
       }
@@ -1083,8 +1085,8 @@ int elbo_predict_impl(int kind, const double* x, const double* y, int64_t n, con
     for (int64_t t0 = 0; t0 < t; t0 += P.tc) {
       const int64_t tn = (t - t0 < P.tc) ? t - t0 : P.tc;
       const int64_t tnp = round_up(tn, NB);
-      GPK_TRY(launch_kernel_matrix<double>(kind, z, m, xt + t0 * d, tn, d, theta, nls, 0.0,
-                                           KM_ZERO_PAD, kzt, tnp, mp, tnp, false, s));
+      GPK_TRY(launch_cross<double>(kind, z, m, xt + t0 * d, tn, d, theta, nls, false, kzt, tnp,
+                                   mp, tnp, s));
       GPK_TRY(launch_colreduce(kzt, tnp, mp, tnp, at<double>(ws, Lo.off_c), v1, part, s));
       hipLaunchKernelGGL(sgpr_mean_kernel, dim3(cdiv(tn, 256)), dim3(256), 0, s, v1, theta, nls,
                          tn, mean + t0);
@@ -1101,8 +1103,7 @@ int elbo_predict_impl(int kind, const double* x, const double* y, int64_t n, con
     }
     return GPK_OK;
   }
-  GPK_TRY(launch_kernel_matrix<double>(kind, z, m, xt, t, d, theta, nls, 0.0, KM_ZERO_PAD, kzt,
-                                       tp, mp, tp, false, s));
+  GPK_TRY(launch_cross<double>(kind, z, m, xt, t, d, theta, nls, false, kzt, tp, mp, tp, s));
   // mean: c + Kzt^T (C^-1 b) / s
   GPK_TRY(launch_colreduce(kzt, tp, mp, tp, at<double>(ws, Lo.off_c), v1, part, s));
   hipLaunchKernelGGL(sgpr_mean_kernel, dim3(cdiv(t, 256)), dim3(256), 0, s, v1, theta, nls, t,

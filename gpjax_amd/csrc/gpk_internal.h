@@ -214,13 +214,16 @@ int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, 
                          bool lower_only, hipStream_t s, double* dblk = nullptr,
                          const Batch& bt = Batch(), double* spk = nullptr);
 
-// Cross-covariance K[i][j] = k(x_i, z_j) (i < n, j < m; zero elsewhere up to
-// rows_total x cols_total) in float32, the kernel function evaluated in float32
-// arithmetic from float64 squared distances (kvalf): the mixed collapsed ELBO's
-// Kzx chunks (x = inducing inputs, z = the chunk's points).
-int launch_cross_f32(int kind, const double* x, int64_t n, const double* z, int64_t m, int d,
-                     const double* theta, int nls, float* out, int64_t ldo, int64_t rows_total,
-                     int64_t cols_total, hipStream_t s);
+// Cross-covariance K[i][j] = var k(x_i, z_j) (i < n, j < m; zero elsewhere up
+// to rows_total x cols_total) for long point streams (the collapsed ELBO's Kzx
+// chunks: x = inducing inputs, z = the chunk's points; the predictive's test
+// points).  Entries equal launch_kernel_matrix's (d > 32 runs it); f32k
+// (float32 output only, d <= 32) evaluates the kernel function in float32
+// arithmetic from the float64 squared distances (kvalf).
+template <typename TS>
+int launch_cross(int kind, const double* x, int64_t n, const double* z, int64_t m, int d,
+                 const double* theta, int nls, bool f32k, TS* out, int64_t ldo,
+                 int64_t rows_total, int64_t cols_total, hipStream_t s);
 
 // Leaf: factor the NB x NB diagonal block at (o, o) of A in float64 in LDS,
 // write L (lower, zero upper) to A, L^-1 to X (lower, zero upper) and to

@@ -25,8 +25,7 @@ namespace {
 constexpr int KT = 64;    // output tile edge
 constexpr int DCH = 32;   // input dimensions staged per LDS pass
 
-// F32K (float32 output only): the kernel function in float32 arithmetic (kvalf).
-template <typename TS, bool F32K = false>
+template <typename TS>
 __global__ __launch_bounds__(256) void kmat_kernel(int kind, const double* __restrict__ x,
                                                    int64_t n, const double* __restrict__ z,
                                                    int64_t m, int d,
@@ -105,10 +104,6 @@ __global__ __launch_bounds__(256) void kmat_kernel(int kind, const double* __res
     const int64_t i = i0 + rg * 16 + r;
     if (i >= rows_total) continue;
     double v;
-    if constexpr (F32K) {
-      out[i * ldo + j] = (i < n && j < m) ? (float)var * kvalf(kind, (float)d2[r]) : 0.0f;
-      continue;
-    }
     if (i < n && j < m) {
       v = var * kval(kind, d2[r]);
       const int dm = diag_mode & KM_DIAG_MASK;
@@ -132,7 +127,144 @@ __global__ __launch_bounds__(256) void kmat_kernel(int kind, const double* __res
   }
 }
 
+// Cross-covariance over a long stream of points (the collapsed ELBO's Kzx
+// chunks): out[i][j] = var k(x_i / l, z_j / l) for i < n, j < m, zero elsewhere
+// up to rows_total x cols_total.  One 256-thread block owns 64 rows (x, scaled
+// and staged once in LDS, read as wave-wide broadcasts) and walks CT column
+// tiles of 64 (z) through two LDS buffers: the next tile's scaled inputs are
+// loaded into registers before the current tile's arithmetic and stored after
+// it, so their global-load latency hides behind a tile of work.  Thread
+// (rg, jj) = (wave, lane) computes rows rg*16 .. +15 of column jj.  kmat_kernel's
+// short-lived blocks (one tile, inputs staged per tile) ran at a third of the
+// store and ALU bound on these shapes.  The dimension loop stays a runtime loop
+// (unrolled, the compiler issues all 16 x d row reads at once and runs out of
+// registers).  The distances are the same arithmetic in the same order as
+// kmat_kernel's, so the float64 entries are identical; F32K evaluates the
+// kernel function in float32 (kvalf).
+constexpr int CT = 8;
+
+template <typename TS, bool F32K, int DMAX>
+__global__ __launch_bounds__(256) void cross_kernel(int kind, const double* __restrict__ x,
+                                                    int64_t n, const double* __restrict__ z,
+                                                    int64_t m, int d,
+                                                    const double* __restrict__ theta, int nls,
+                                                    TS* __restrict__ out, int64_t ldo,
+                                                    int64_t rows_total, int64_t cols_total) {
+  constexpr int NPRE = KT * DMAX / 256;  // staged inputs per thread and tile (DMAX >= 4)
+  __shared__ double xs[KT][DMAX];
+  __shared__ double zs[2][DMAX][KT];
+  const int tid = threadIdx.x, jj = tid & 63, rg = tid >> 6;
+  const int64_t i0 = (int64_t)blockIdx.y * KT;
+  for (int e = tid; e < KT * DMAX; e += 256) {
+    const int r = e / DMAX, q = e % DMAX;
+    const int64_t gi = i0 + r;
+    xs[r][q] = (q < d && gi < n) ? x[gi * d + q] / theta[nls == 1 ? 0 : q] : 0.0;
+  }
+  // staged element k of a tile: point r = (tid + 256 k) / DMAX, dimension
+  // q = (tid + 256 k) % DMAX
+  double pre[NPRE];
+  auto load = [&](int64_t jt) {
+#pragma unroll
+    for (int k = 0; k < NPRE; ++k) {
+      const int e = tid + 256 * k, r = e / DMAX, q = e % DMAX;
+      const int64_t gj = jt * KT + r;
+      pre[k] = (q < d && gj < m) ? z[gj * d + q] / theta[nls == 1 ? 0 : q] : 0.0;
+    }
+  };
+  auto stage = [&](int b) {
+#pragma unroll
+    for (int k = 0; k < NPRE; ++k) {
+      const int e = tid + 256 * k;
+      zs[b][e % DMAX][e / DMAX] = pre[k];
+    }
+  };
+  const int64_t ntiles = (cols_total + KT - 1) / KT;
+  int64_t jt = (int64_t)blockIdx.x * CT;
+  load(jt);
+  stage(0);
+  const double var = theta[nls];
+  __syncthreads();
+  for (int c = 0; c < CT && jt < ntiles; ++c, ++jt) {
+    const bool more = c + 1 < CT && jt + 1 < ntiles;
+    if (more) load(jt + 1);
+    const int b = c & 1;
+    double d2[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) d2[r] = 0.0;
+    for (int q = 0; q < d; ++q) {
+      const double zv = zs[b][q][jj];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const double df = xs[rg * 16 + r][q] - zv;
+        d2[r] += df * df;
+      }
+    }
+    const int64_t j = jt * KT + jj;
+    if (j < cols_total) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t i = i0 + rg * 16 + r;
+        if (i >= rows_total) break;
+        const bool in = i < n && j < m;
+        if constexpr (F32K) {
+          out[i * ldo + j] = in ? (float)var * kvalf(kind, (float)d2[r]) : 0.0f;
+        } else {
+          out[i * ldo + j] = (TS)(in ? var * kval(kind, d2[r]) : 0.0);
+        }
+      }
+    }
+    if (more) stage(b ^ 1);
+    __syncthreads();
+  }
+}
+
 }  // namespace
+
+template <typename TS>
+int launch_cross(int kind, const double* x, int64_t n, const double* z, int64_t m, int d,
+                 const double* theta, int nls, bool f32k, TS* out, int64_t ldo,
+                 int64_t rows_total, int64_t cols_total, hipStream_t s) {
+  if (rows_total <= 0 || cols_total <= 0) return GPK_OK;
+  if (f32k && !std::is_same<TS, float>::value) {
+    set_error("cross-covariance: the float32 kernel function needs float32 output");
+    return GPK_ERR_BAD_ARG;
+  }
+  if (d > 32 && !f32k)  // wider inputs: the general builder (same entries)
+    return launch_kernel_matrix<TS>(kind, x, n, z, m, d, theta, nls, 0.0, KM_ZERO_PAD, out, ldo,
+                                    rows_total, cols_total, false, s);
+  const int64_t gy = (rows_total + KT - 1) / KT;
+  if (gy > 65535 || d < 1 || d > 32) {
+    set_error("cross-covariance: %lld rows / d = %d outside one launch's range",
+              (long long)rows_total, d);
+    return GPK_ERR_BAD_ARG;
+  }
+  const int64_t ntiles = (cols_total + KT - 1) / KT;
+  dim3 grid((unsigned)((ntiles + CT - 1) / CT), (unsigned)gy, 1);
+#define GPK_CROSS(DM)                                                                          \
+  do {                                                                                         \
+    if (f32k)                                                                                  \
+      hipLaunchKernelGGL((cross_kernel<TS, std::is_same<TS, float>::value, DM>), grid,         \
+                         dim3(256), 0, s, kind, x, n, z, m, d, theta, nls, out, ldo,           \
+                         rows_total, cols_total);                                              \
+    else                                                                                       \
+      hipLaunchKernelGGL((cross_kernel<TS, false, DM>), grid, dim3(256), 0, s, kind, x, n, z, \
+                         m, d, theta, nls, out, ldo, rows_total, cols_total);                  \
+  } while (0)
+  if (d <= 4) GPK_CROSS(4);
+  else if (d <= 8) GPK_CROSS(8);
+  else if (d <= 16) GPK_CROSS(16);
+  else GPK_CROSS(32);
+#undef GPK_CROSS
+  GPK_LAUNCH_CHECK();
+  return GPK_OK;
+}
+
+template int launch_cross<double>(int, const double*, int64_t, const double*, int64_t, int,
+                                  const double*, int, bool, double*, int64_t, int64_t, int64_t,
+                                  hipStream_t);
+template int launch_cross<float>(int, const double*, int64_t, const double*, int64_t, int,
+                                 const double*, int, bool, float*, int64_t, int64_t, int64_t,
+                                 hipStream_t);
 
 template <typename TS>
 int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, int64_t m,
@@ -160,23 +292,6 @@ int launch_kernel_matrix(int kind, const double* x, int64_t n, const double* z, 
   hipLaunchKernelGGL(kmat_kernel<TS>, grid, dim3(256), 0, s, kind, x, n, z, m, d, theta, nls,
                      jitter, diag_mode, out, ldo, rows_total, cols_total, lower_only ? 1 : 0,
                      dblk, lower_only ? spk : nullptr, bt);
-  GPK_LAUNCH_CHECK();
-  return GPK_OK;
-}
-
-int launch_cross_f32(int kind, const double* x, int64_t n, const double* z, int64_t m, int d,
-                     const double* theta, int nls, float* out, int64_t ldo, int64_t rows_total,
-                     int64_t cols_total, hipStream_t s) {
-  if (rows_total <= 0 || cols_total <= 0) return GPK_OK;
-  const int64_t gy = (rows_total + KT - 1) / KT;
-  if (gy > 65535) {
-    set_error("kernel matrix: %lld rows exceed one launch", (long long)rows_total);
-    return GPK_ERR_BAD_ARG;
-  }
-  dim3 grid((unsigned)((cols_total + KT - 1) / KT), (unsigned)gy, 1);
-  hipLaunchKernelGGL((kmat_kernel<float, true>), grid, dim3(256), 0, s, kind, x, n, z, m, d,
-                     theta, nls, 0.0, KM_ZERO_PAD, out, ldo, rows_total, cols_total, 0, nullptr,
-                     nullptr, Batch());
   GPK_LAUNCH_CHECK();
   return GPK_OK;
 }
