@@ -291,7 +291,8 @@ int launch_sigma_matvec(int kind, const double* x, int64_t n, int64_t np, int d,
 // calling thread on the current device, created on first use and kept for the
 // process.  The look-ahead Cholesky takes stream 0 and events 0..2 (never
 // inside a batched call); the batched driver takes the streams and events
-// 3..3+AUX_STREAMS.
+// 3..3+AUX_STREAMS; the collapsed ELBO takes stream 0 and events 6 and 7 (it
+// never runs the look-ahead Cholesky).
 constexpr int AUX_STREAMS = 3;
 constexpr int AUX_EVENTS = 8;
 int aux_streams(hipStream_t* streams, hipEvent_t* events);
