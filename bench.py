@@ -175,7 +175,8 @@ def other_workload(args) -> dict:
                N = 1e6, D = 8, Matern-3/2 ARD, mixed and FP64;
     predict -- ConjugatePosterior.predict with the diagonal variance at large T:
                the matern workload's N = 16,384 training points, T = 1,000,000 test
-               points ~ N(0, I_8), FP64 (factorisation included in every step)."""
+               points ~ N(0, I_8), mixed and FP64 (factorisation included in every
+               step)."""
     import torch
 
     import gpjax_amd as gpx
@@ -277,16 +278,26 @@ def other_workload(args) -> dict:
                         y=torch.tensor(w["y"][:, None], device=dev))
         T = torch.tensor(np.random.default_rng(7).standard_normal((t, 8)), device=dev)
 
-        def step():
-            return post.predict(T, D, full_covariance=False, precision="fp64")
+        def make(prec):
+            def step():
+                return post.predict(T, D, full_covariance=False, precision=prec)
+            return step
 
-        tm, dist = timed(step, args.steps, args.warmup)
-        line.update(metric="test points/s, diagonal predictive, N=16,384, T=1e6 (FP64)",
+        tm, dist = timed(make("mixed"), args.steps, args.warmup)
+        td, dist64 = timed(make("fp64"), args.steps, args.warmup)
+        rel = lambda a, b: float((a - b).abs().max() / b.abs().max())  # noqa: E731
+        line.update(metric="test points/s, diagonal predictive, N=16,384, T=1e6, mixed",
                     value=round(t / tm, 1), unit="test points/s", ms_per_step=round(tm * 1e3, 2),
-                    dtype="f64",
+                    dtype="f32 storage, bf16x6-split products, f64 panels and refinement",
                     config={"workload": "predict: matern training data, T=1e6 test points "
                                         "(gpjax_amd.workloads.exact_matern)", "n": n, "t": t,
                             "d": 8},
+                    fp64_gpu={"test_points_per_s": round(t / td, 1),
+                              "ms_per_step": round(td * 1e3, 2)},
+                    speedup_vs_fp64=round(td / tm, 3),
+                    accuracy={"mean_max_err_of_max_vs_fp64": rel(dist.mean, dist64.mean),
+                              "variance_max_err_of_max_vs_fp64":
+                                  rel(dist.variance, dist64.variance)},
                     all_finite=bool(torch.isfinite(dist.mean).all()
                                     and torch.isfinite(dist.variance).all()))
         cpu_n = None

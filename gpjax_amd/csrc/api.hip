@@ -1565,6 +1565,11 @@ int gpk_predict(int kind, int prec, const double* x, const double* y, int64_t n,
   }
   Arith3 ar;
   const int base = decode_prec(prec, &ar);
+  // The predictive variance is k(t, t) - ||X k_t||^2 with X ~ L^-1 itself, so
+  // plain GPK_PREC_MIXED takes six-product splits for the triangular inverse
+  // here (three-product ones left 2e-5 of the largest variance at N = 16,384,
+  // T = 1e6; the inverse is a small share of a predict call).
+  if (prec == GPK_PREC_MIXED) ar.trtri = AR_BF16X6;
   const Layout Lo = make_layout(n, d, base);
   const PredLayout P = make_pred_layout(Lo, t, full_cov);
   if ((rc = check_ws("predict", ws, ws_bytes, P.total))) return rc;

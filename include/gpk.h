@@ -250,9 +250,13 @@ size_t gpk_predict_workspace_bytes(int64_t n, int64_t d, int64_t t, int prec, in
  *   cov  = Ktt - (L^-1 Kxt)^T (L^-1 Kxt) + jitter_prior I.
  * full_cov != 0 writes the dense t x t covariance (row-major, ld = t) to cov;
  * full_cov == 0 writes only its diagonal (t entries).  The factorisation runs
- * in `prec`; everything after it runs in float64.  result (optional, device,
- * GPK_RESULT_LEN): the factorisation's slots as gpk_mll_forward writes them
- * (pivot status, refinement residual, the mixed path's accuracy check).
+ * in `prec` (plain GPK_PREC_MIXED takes six-product splits for the triangular
+ * inverse here too: X ~ L^-1 enters the variance directly); in the mixed
+ * path's diagonal mode L^-1 Kxt is a six-product bf16-split product of float32
+ * operands with float64 sums; everything else runs in float64.  result
+ * (optional, device, GPK_RESULT_LEN): the factorisation's slots as
+ * gpk_mll_forward writes them (pivot status, refinement residual, the mixed
+ * path's accuracy check).
  */
 int gpk_predict(int kind, int prec, const double* x, const double* y, int64_t n, int64_t d,
                 const double* xt, int64_t t, const double* theta, int nls, double jitter_post,
