@@ -492,7 +492,7 @@ __global__ __launch_bounds__(256) void elbo_fwd_finalize(
       // leave P with entry errors R_ij of relative size ~ 2^-24 and random
       // sign, whose first-order effect on the ELBO is tr(G_P R) / 2.  Modelled
       // as 2^-24 |P_ij| with independent signs its spread is
-      // 2^-24 sqrt(2) ||G_P o P||_F / 2 (measured |errors| <= 1.4x it).
+      // 2^-24 sqrt(2) ||G_P o P||_F / 2 (measured |errors| <= 1.6x it).
       const double err = 0.5 * 0x1p-24 * sqrt(2.0 * gp2);
       result[GPK_RES_MLL_ERR] = err;
       const bool bad = info != 0 || !isfinite(elbo) || !(5.0 * err <= 1e-6 * fabs(elbo)) ||

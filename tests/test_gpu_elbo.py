@@ -233,7 +233,10 @@ MIXED_CASES = [("rbf", 500, 40, 2, True, (0.05, 0.15), 0.3, False),
                ("rbf", 640, 80, 32, True, (0.6, 1.6), 0.3, True),
                ("matern32", 20000, 1000, 4, True, (0.1, 0.5), 0.3, False),
                ("rbf", 20000, 512, 3, True, (0.2, 0.5), 0.3, False),
-               ("matern32", 40000, 1024, 8, True, (0.1, 0.5), 0.2, True)]
+               ("matern32", 40000, 1024, 8, True, (0.1, 0.5), 0.2, True),
+               ("matern52", 5, 3, 2, True, (0.3, 0.6), 0.3, False),
+               ("rbf", 1, 1, 1, True, (0.3, 0.6), 0.3, False),
+               ("matern32", 70000, 300, 5, True, (0.1, 0.3), 0.2, False)]
 
 
 @pytest.mark.parametrize("precision", ["mixed", "mixed:6,6,6"])
@@ -248,10 +251,11 @@ def test_mixed_elbo_in_gate_or_flagged(cuda, monkeypatch, kind, n, m, d, ard, ls
     its inducing-input gradient; the float64 path stands in for the oracle here
     (test_elbo_and_grad_parity pins it to 1e-9).  Ill-conditioned Kzz (low
     dimensions, long lengthscales, many inducing points) makes the float32
-    statistics too coarse: those cases must come back flagged, and the 1-D and
-    3-D RBF ones break down outright.  The well-conditioned cases (C4's recipe
-    at N = 40,000, M = 1,024; RBF in 32 dimensions) must pass the check, so the
-    test cannot pass by flagging everything."""
+    statistics too coarse: those cases must come back flagged, and the 3-D RBF
+    one breaks down outright.  The well-conditioned cases (C4's recipe at
+    N = 40,000, M = 1,024; RBF in 32 dimensions) must pass the check, so the
+    test cannot pass by flagging everything.  Also tiny shapes (N = 5 and 1)
+    and N = 70,000, whose second 65,536-point chunk is ragged."""
     monkeypatch.setattr(gpx.config, "mixed_check", "off")
     rng = np.random.default_rng(n + m)
     X = rng.uniform(0, 1, (n, d))
