@@ -151,14 +151,14 @@ __global__ __launch_bounds__(256) void cross_kernel(int kind, const double* __re
                                                     TS* __restrict__ out, int64_t ldo,
                                                     int64_t rows_total, int64_t cols_total) {
   constexpr int NPRE = KT * DMAX / 256;  // staged inputs per thread and tile (DMAX >= 4)
-  __shared__ double xs[KT][DMAX];
+  __shared__ double xs[DMAX][KT];  // [dimension][row]: a wave's 16 rows are contiguous
   __shared__ double zs[2][DMAX][KT];
   const int tid = threadIdx.x, jj = tid & 63, rg = tid >> 6;
   const int64_t i0 = (int64_t)blockIdx.y * KT;
   for (int e = tid; e < KT * DMAX; e += 256) {
     const int r = e / DMAX, q = e % DMAX;
     const int64_t gi = i0 + r;
-    xs[r][q] = (q < d && gi < n) ? x[gi * d + q] / theta[nls == 1 ? 0 : q] : 0.0;
+    xs[q][r] = (q < d && gi < n) ? x[gi * d + q] / theta[nls == 1 ? 0 : q] : 0.0;
   }
   // staged element k of a tile: point r = (tid + 256 k) / DMAX, dimension
   // q = (tid + 256 k) % DMAX
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void cross_kernel(int kind, const double* __re
       const double zv = zs[b][q][jj];
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const double df = xs[rg * 16 + r][q] - zv;
+        const double df = xs[q][rg * 16 + r] - zv;
         d2[r] += df * df;
       }
     }
