@@ -754,13 +754,7 @@ int elbo_forward_impl(int kind, const ElboArith& ea, const double* x, const doub
     hipStream_t s2 = st[0];
     GPK_HIP_TRY(hipEventRecord(ev[6], s));
     GPK_HIP_TRY(hipStreamWaitEvent(s2, ev[6], 0));
-    struct Join {
-      hipStream_t s, s2;
-      hipEvent_t e;
-      ~Join() {
-        if (hipEventRecord(e, s2) == hipSuccess) (void)hipStreamWaitEvent(s, e, 0);
-      }
-    } join{s, s2, ev[7]};
+    StreamJoin join{s, s2, ev[7]};
     FK.s = s2;
     GPK_TRY(launch_kernel_matrix<double>(kind, z, m, z, m, d, theta, nls, jitter, 1, Kb, mp, mp,
                                          mp, true, s2));
@@ -881,15 +875,8 @@ int elbo_backward_impl(int kind, const ElboArith& ea, const double* x, const dou
   hipStream_t s2 = st[0];
   GPK_HIP_TRY(hipEventRecord(ev[6], s));
   GPK_HIP_TRY(hipStreamWaitEvent(s2, ev[6], 0));
-  struct Join {
-    hipStream_t s, s2;
-    hipEvent_t e;
-    ~Join() {
-      if (hipEventRecord(e, s2) == hipSuccess) (void)hipStreamWaitEvent(s, e, 0);
-    }
-  };
   {
-    Join join{s, s2, ev[7]};
+    StreamJoin join{s, s2, ev[7]};
     GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, false, Kinv, mp, P, mp, T1, mp,
                                                 mp, mp, mp, 1.0, false, false, KR_FULL, s2));
     GPK_TRY(launch_gemm<double, double, double>(AR_F64, false, false, T1, mp, Kinv, mp, T2, mp,

@@ -121,13 +121,7 @@ struct Factor {
     // Whatever happens below, the calling stream ends up waiting for all the
     // second stream's work (an early error return included), so no work of
     // this call is left unjoined.
-    struct Join {
-      hipStream_t s, s2;
-      hipEvent_t e;
-      ~Join() {
-        if (hipEventRecord(e, s2) == hipSuccess) (void)hipStreamWaitEvent(s, e, 0);
-      }
-    } join{s, s2, ev[2]};
+    StreamJoin join{s, s2, ev[2]};
     Factor<TS> G = *this;
     G.s = s2;
     GPK_TRY(potrf(0, B));

@@ -298,6 +298,17 @@ constexpr int AUX_EVENTS = 8;
 int aux_streams(hipStream_t* streams, hipEvent_t* events);
 int lookahead_resources(hipStream_t* s2, hipEvent_t* ev3);
 
+// Joins a forked library stream back into the caller's: at scope exit (an
+// early error return included) `s` waits for everything issued to `s2` so far,
+// so no work of a call is left unjoined.
+struct StreamJoin {
+  hipStream_t s, s2;
+  hipEvent_t e;
+  ~StreamJoin() {
+    if (hipEventRecord(e, s2) == hipSuccess) (void)hipStreamWaitEvent(s, e, 0);
+  }
+};
+
 // v = S a for a symmetric S held as its packed lower NB-tiles in float64 (the
 // spk layout of launch_kernel_matrix), np x np; nv = 1 or 3 vectors np apart
 // share each read of S.  partial holds 2 * nv * T(T+1)/2 * NB doubles, T = np/NB.
