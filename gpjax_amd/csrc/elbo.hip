@@ -5,7 +5,7 @@
 // and AA^T.  Here the data enter only through the N-summed statistics
 //   P = Kzx Kxz (M x M),  b = Kzx r,  q = Kzx 1,  r = y - c,
 // accumulated chunk by chunk: Kzx is evaluated once per chunk of 65,536
-// points into an M x 65,536 buffer and P += Kc Kc^T runs on the FP64 matrix
+// points into an M x 65,536 buffer and P += Kc Kc^T runs on the matrix
 // cores, so no N x M buffer exists.  With s = sigma^2 and C = Kzz + P / s the
 // determinant and inversion lemmas that the reference's comments state
 // (objectives.py:361-385) give
@@ -56,7 +56,7 @@ __device__ __forceinline__ double d2_of(const double* a, const double* b) {
 // ---- E1: streamed statistics b = Kzx r and q = Kzx 1 over one chunk ----
 // Kc[i * ldk + (g - c0)] = k(z_i, x_g) for the chunk's points; one block per
 // inducing row, fixed-order reduction; the first chunk writes, later ones add.
-// (P = Kzx Kxz is accumulated from the same chunk by one FP64 GEMM.)
+// (P = Kzx Kxz is accumulated from the same chunk by one matrix-core GEMM.)
 template <typename TK>
 __global__ __launch_bounds__(256) void chunk_rowsum_kernel(
     const TK* __restrict__ Kc, int64_t ldk, int64_t c0, int64_t ncol,
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(256) void gpfrob_kernel(const double* __restrict__ 
 
 // ---- E3: W = G_P Kzx + (c / s^2) r^T contracted against dKzx ----
 // One launch per chunk of points [c0, c0 + ncol).  Wt holds the chunk's
-// (G_P Kzx)^T, Wt[(g - c0) * mp + i], formed beforehand by one FP64 GEMM
+// (G_P Kzx)^T, Wt[(g - c0) * mp + i], formed beforehand by one matrix-core GEMM
 // Kc^T G_P (G_P is symmetric) from Kc = Kzx evaluated once for the chunk.
 // Block (mt, cb): inducing rows mt*TILE .. +TILE (thread row li = t % TILE; the
 // two halves of the block take alternate points) against points
