@@ -169,6 +169,7 @@ def batched_fit_line(gpx, w, dev, args) -> dict:
 def other_workload(args) -> dict:
     """The other configurations, one JSON line each (not the headline):
     matern  -- exact MLL + gradient, N = 16,384, D = 8, Matern-5/2 ARD, mixed and FP64;
+    loocv   -- conjugate_loocv + gradient on the same data, mixed and FP64;
     batched -- 4,096 independent exact GPs (N_i in 32..1024, D = 4, RBF ARD), MLL +
                gradient of all of them per step;
     sparse  -- collapsed ELBO + gradient (theta and the 2,048 x 8 inducing inputs),
@@ -301,6 +302,40 @@ def other_workload(args) -> dict:
                     all_finite=bool(torch.isfinite(dist.mean).all()
                                     and torch.isfinite(dist.variance).all()))
         cpu_n = None
+    elif args.workload == "loocv":
+        w = workloads.exact_matern()
+        n = len(w["X"])
+        k = gpx.kernels.Matern52(lengthscale=w["lengthscale"], variance=w["variance"])
+        post = gpx.gps.Prior(mean_function=gpx.mean_functions.Zero(), kernel=k,
+                             jitter=w["jitter"]) * \
+            gpx.likelihoods.Gaussian(num_datapoints=n, obs_stddev=w["obs_stddev"])
+        D = gpx.Dataset(X=torch.tensor(w["X"], device=dev),
+                        y=torch.tensor(w["y"][:, None], device=dev))
+        params = [k.lengthscale, k.variance, post.likelihood.obs_stddev]
+
+        def make(prec):
+            def step():
+                for p in params:
+                    p.value.grad = None
+                obj = gpx.objectives.conjugate_loocv(post, D, precision=prec)
+                obj.backward()
+                return float(obj.detach()), grads(params)
+            return step
+
+        t_m, (v_m, g_m) = timed(make("mixed"), args.steps, args.warmup)
+        t_d, (v_d, g_d) = timed(make("fp64"), args.steps, args.warmup)
+        line.update(metric="LOOCV+grad evals/s at N=16,384 (Matern-5/2 ARD, D=8), mixed",
+                    value=round(1.0 / t_m, 4), unit="evals/s", ms_per_step=round(t_m * 1e3, 2),
+                    dtype="bf16x6-split/f32 with f64 panels",
+                    config={"workload": "conjugate_loocv + grad, N=16384 D=8 Matern-5/2 ARD "
+                                        "(gpjax_amd.workloads.exact_matern)", "n": n, "d": 8},
+                    fp64_gpu={"evals_per_s": round(1.0 / t_d, 4),
+                              "ms_per_step": round(t_d * 1e3, 2)},
+                    speedup_vs_fp64=round(t_d / t_m, 3),
+                    accuracy={"rel_loocv_vs_fp64": abs(v_m - v_d) / abs(v_d),
+                              "rel_grad_vs_fp64":
+                                  float(torch.linalg.norm(g_m - g_d) / torch.linalg.norm(g_d))})
+        cpu_n = 4096
     else:  # matern
         w = workloads.exact_matern()
         n = len(w["X"])
@@ -364,7 +399,8 @@ def main() -> int:
                     help="only run a plain device-to-device copy of BYTES bytes 5 times "
                          "(reference for rocprof FETCH_SIZE / WRITE_SIZE scaling) and exit")
     ap.add_argument("--workload", default="exact_large",
-                    choices=["exact_large", "matern", "batched", "sparse", "predict"],
+                    choices=["exact_large", "matern", "loocv", "batched", "sparse",
+                             "predict"],
                     help="exact_large (default) is the headline; the others print their own line")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -34,9 +34,9 @@ def _median_time(fn, runs):
 
 def other_workload(a) -> dict:
     """Bounded samples of the other configurations, extrapolated to full size:
-    matern (N = 16,384, D = 8) by N^3; batched (4,096 GPs) by the sum of N_i^3
-    over the problems timed against the whole batch; sparse (N = 1e6, M = 2,048)
-    linearly in N (its cost is M^2 N + O(M^3))."""
+    matern and loocv (N = 16,384, D = 8) by N^3; batched (4,096 GPs) by the sum of
+    N_i^3 over the problems timed against the whole batch; sparse (N = 1e6,
+    M = 2,048) linearly in N (its cost is M^2 N + O(M^3))."""
     import numpy as np
 
     from gpjax_amd import workloads
@@ -49,6 +49,14 @@ def other_workload(a) -> dict:
             w["jitter"]), a.runs)
         scale = (a.n / 16384) ** 3
         sample = f"float64 oracle MLL + gradient on exact_matern with N={a.n}; rate scaled to N=16384 by N^3"
+    elif a.workload == "loocv":
+        w = workloads.exact_matern(n=a.n)
+        t = _median_time(lambda: O.conjugate_loocv_grad(
+            w["kind"], w["X"], w["y"], w["lengthscale"], w["variance"], w["obs_stddev"],
+            w["jitter"]), a.runs)
+        scale = (a.n / 16384) ** 3
+        sample = (f"float64 oracle LOOCV + gradient on exact_matern with N={a.n}; rate scaled "
+                  "to N=16384 by N^3")
     elif a.workload == "batched":
         w = workloads.batched()
         k = a.n  # number of problems timed
@@ -84,7 +92,7 @@ def main() -> None:
     ap.add_argument("--runs", type=int, default=3)
     ap.add_argument("--target-n", type=int, default=65536)
     ap.add_argument("--workload", default="exact_large",
-                    choices=["exact_large", "matern", "batched", "sparse"])
+                    choices=["exact_large", "matern", "loocv", "batched", "sparse"])
     a = ap.parse_args()
     if a.workload != "exact_large":
         print(json.dumps(other_workload(a)))
