@@ -174,6 +174,9 @@ def other_workload(args) -> dict:
                gradient of all of them per step;
     sparse  -- collapsed ELBO + gradient (theta and the 2,048 x 8 inducing inputs),
                N = 1e6, D = 8, Matern-3/2 ARD, mixed and FP64;
+    sparse_predict -- CollapsedVariationalGaussian.predict, diagonal variance, on the
+               sparse workload's data and inducing inputs at T = 1,000,000, FP64 (the
+               ELBO's statistics recomputed in every step);
     predict -- ConjugatePosterior.predict with the diagonal variance at large T:
                the matern workload's N = 16,384 training points, T = 1,000,000 test
                points ~ N(0, I_8), mixed and FP64 (factorisation included in every
@@ -302,6 +305,34 @@ def other_workload(args) -> dict:
                     all_finite=bool(torch.isfinite(dist.mean).all()
                                     and torch.isfinite(dist.variance).all()))
         cpu_n = None
+    elif args.workload == "sparse_predict":
+        import numpy as np
+        w = workloads.sparse()
+        t = 1_000_000
+        k = gpx.kernels.Matern32(lengthscale=w["lengthscale"], variance=w["variance"])
+        post = gpx.gps.Prior(mean_function=gpx.mean_functions.Zero(), kernel=k) * \
+            gpx.likelihoods.Gaussian(num_datapoints=len(w["X"]), obs_stddev=w["obs_stddev"])
+        q = gpx.variational_families.CollapsedVariationalGaussian(posterior=post,
+                                                                 inducing_inputs=w["Z"])
+        D = gpx.Dataset(X=torch.tensor(w["X"], device=dev),
+                        y=torch.tensor(w["y"][:, None], device=dev))
+        T = torch.tensor(np.random.default_rng(11).uniform(0.0, 1.0, (t, 8)), device=dev)
+
+        def step():
+            return q.predict(T, D, full_covariance=False)
+
+        tm, dist = timed(step, args.steps, args.warmup)
+        line.update(metric="test points/s, collapsed diagonal predictive, N=1e6, M=2048, "
+                           "T=1e6 (FP64)",
+                    value=round(t / tm, 1), unit="test points/s", ms_per_step=round(tm * 1e3, 2),
+                    dtype="f64",
+                    config={"workload": "sparse_predict: C4 training data and inducing inputs, "
+                                        "T=1e6 test points ~ U[0,1]^8 "
+                                        "(gpjax_amd.workloads.sparse)", "n": len(w["X"]),
+                            "m": 2048, "t": t, "d": 8},
+                    all_finite=bool(torch.isfinite(dist.mean).all()
+                                    and torch.isfinite(dist.variance).all()))
+        cpu_n = None
     elif args.workload == "loocv":
         w = workloads.exact_matern()
         n = len(w["X"])
@@ -400,7 +431,7 @@ def main() -> int:
                          "(reference for rocprof FETCH_SIZE / WRITE_SIZE scaling) and exit")
     ap.add_argument("--workload", default="exact_large",
                     choices=["exact_large", "matern", "loocv", "batched", "sparse",
-                             "predict"],
+                             "predict", "sparse_predict"],
                     help="exact_large (default) is the headline; the others print their own line")
     args = ap.parse_args()
     if args.gpus != 1:
