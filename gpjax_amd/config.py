@@ -3,7 +3,10 @@
 GPJax runs float64 throughout (``jax_enable_x64``, ``tests/conftest.py:4``);
 here ``precision`` picks between the float64 path ("fp64") and the
 mixed-precision path ("mixed": float32 storage, BF16 split products on the
-matrix cores, float64 diagonal blocks, log-determinant and refinement).
+matrix cores, float64 diagonal blocks, log-determinant and refinement) for
+every objective and predictive that takes a ``precision`` argument and is
+called without one (conjugate_mll, conjugate_loocv, collapsed_elbo,
+ConjugatePosterior.predict).
 """
 
 from __future__ import annotations
