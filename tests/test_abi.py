@@ -110,6 +110,29 @@ def test_elbo_predict_workspace_and_validation():
     assert rc == 1 and "xt" in lib.gpk_last_error().decode()
 
 
+def test_elbo_precision_codes():
+    """gpk_elbo_forward takes GPK_PREC_FP64, GPK_PREC_MIXED and
+    GPK_PREC_MIXED_ARITH(p, t, l) with p, t in {0, 3, 4, 6} (checked before any
+    device work: a valid code gets as far as the workspace check), rejects every
+    other code, and needs the same workspace for all of them."""
+    lib = _lib.lib()
+    dummy = ctypes.c_void_p(0x1000)
+    size = lib.gpk_elbo_workspace_bytes(5000, 300, 4, 0)
+    good = [0, 1] + [_lib.mixed_arith(p, t, 6) for p in (0, 3, 4, 6) for t in (0, 3, 4, 6)]
+    for code in good:
+        assert lib.gpk_elbo_workspace_bytes(5000, 300, 4, code) == size
+        rc = lib.gpk_elbo_forward(0, code, dummy, dummy, 100, dummy, 10, 2, dummy, 1, 1e-6,
+                                  dummy, 16, dummy, None)
+        assert rc == 5 and "workspace" in lib.gpk_last_error().decode(), code
+    bad = [2, 0 | _lib.ARITH_FLAG, 1 | (1 << 9),
+           1 | _lib.ARITH_FLAG | (5 << 12) | (6 << 16) | (6 << 20),
+           1 | _lib.ARITH_FLAG | (6 << 12) | (7 << 16) | (6 << 20)]
+    for code in bad:
+        rc = lib.gpk_elbo_forward(0, code, dummy, dummy, 100, dummy, 10, 2, dummy, 1, 1e-6,
+                                  dummy, 1 << 40, dummy, None)
+        assert rc == 1 and "prec" in lib.gpk_last_error().decode(), code
+
+
 def test_mixed_arith_is_a_per_call_precision_code():
     """The stage arithmetic travels in the call's `prec` argument
     (GPK_PREC_MIXED_ARITH); there is no process-wide setting."""
