@@ -50,6 +50,19 @@ def test_predict_exact_small_workload(cuda):
            w["obs_stddev"], 0.0, "fp64", 1e-9, 1e-9)
 
 
+@pytest.mark.parametrize("d", [24, 48])
+def test_predict_wide_inputs(cuda, d):
+    """The cross-covariance Kxt goes through the streaming builder for d <= 32
+    and the general one beyond (both float64, same entries): FP64 parity at
+    D = 24 and 48."""
+    rng = np.random.default_rng(d)
+    X = rng.uniform(0, 1, (400, d))
+    y = np.sin(X[:, :4].sum(1)) + 0.05 * rng.standard_normal(400)
+    T = rng.uniform(0, 1, (90, d))
+    ls = rng.uniform(0.8, 2.0, d)
+    _check(cuda, "matern52", X, y, T, ls, 1.1, 0.2, 0.1, "fp64", 1e-9, 1e-9)
+
+
 @pytest.mark.parametrize("kind", ["matern12", "matern32", "matern52", "rbf"])
 def test_predict_ard(cuda, kind):
     rng = np.random.default_rng(5)
