@@ -4,8 +4,8 @@
 // The reference forms A = Lz^-1 Kzx / sigma (an M x N matrix, materialised)
 // and AA^T.  Here the data enter only through the N-summed statistics
 //   P = Kzx Kxz (M x M),  b = Kzx r,  q = Kzx 1,  r = y - c,
-// accumulated chunk by chunk: Kzx is evaluated once per chunk of 65,536
-// points into an M x 65,536 buffer and P += Kc Kc^T runs on the matrix
+// accumulated chunk by chunk: Kzx is evaluated once per chunk of 131,072
+// points into an M x 131,072 buffer and P += Kc Kc^T runs on the matrix
 // cores, so no N x M buffer exists.  With s = sigma^2 and C = Kzz + P / s the
 // determinant and inversion lemmas that the reference's comments state
 // (objectives.py:361-385) give
@@ -17,7 +17,7 @@
 //   G_K = Kzz^-1 - C^-1 - c c^T / s^2 - Kzz^-1 P Kzz^-1 / s,
 //   g_s = -n/s + tr(C^-1 P)/s^2 + r^T r/s^2 + c^T P c/s^4 - 2 b^T c/s^3
 //         + n var/s^2 - tr(Kzz^-1 P)/s^2.
-// The backward evaluates Kzx once per chunk of 65,536 points, forms W for the
+// The backward evaluates Kzx once per chunk of 131,072 points, forms W for the
 // chunk with one GEMM on the matrix cores and reduces it against
 // dKzx / d(lengthscale, variance, Z) elementwise.
 // GPK_PREC_FP64 runs everything in float64.  GPK_PREC_MIXED keeps the Kzx
@@ -35,7 +35,11 @@ namespace gpk {
 namespace {
 
 constexpr int EDMAX = 32;   // largest input dimension on the ELBO path
-constexpr int64_t BW_CHUNK = 65536;  // points per Kzx chunk (statistics, data pass)
+// Points per Kzx chunk (statistics, data pass).  Against 65,536 it halves the
+// chunk launches and lets the split-K statistics GEMM fill whole rounds of the
+// chip (see S below): 231 -> 228 ms (FP64) and 83.3 -> 82.8 ms (mixed) at
+// N = 1e6, M = 2,048, for 2 GB more workspace.
+constexpr int64_t BW_CHUNK = 131072;
 constexpr int64_t EW = 1024;         // points per block of the gradient data pass
 
 __device__ __forceinline__ double lsq(const double* theta, int nls, int q) {
@@ -492,7 +496,7 @@ __global__ __launch_bounds__(256) void elbo_fwd_finalize(
       // leave P with entry errors R_ij of relative size ~ 2^-24 and random
       // sign, whose first-order effect on the ELBO is tr(G_P R) / 2.  Modelled
       // as 2^-24 |P_ij| with independent signs its spread is
-      // 2^-24 sqrt(2) ||G_P o P||_F / 2 (measured |errors| <= 1.6x it).
+      // 2^-24 sqrt(2) ||G_P o P||_F / 2 (measured |errors| <= 2.9x it).
       const double err = 0.5 * 0x1p-24 * sqrt(2.0 * gp2);
       result[GPK_RES_MLL_ERR] = err;
       const bool bad = info != 0 || !isfinite(elbo) || !(5.0 * err <= 1e-6 * fabs(elbo)) ||
@@ -719,13 +723,17 @@ int elbo_forward_impl(int kind, const ElboArith& ea, const double* x, const doub
   //    entry) bytes): member z reads columns z*kz .. +kz of Kc and accumulates
   //    its float64 partial of P at column z*kp (kp = kz for float64 chunks,
   //    kz / 2 for float32 ones; kp >= mp keeps the partials apart) of a
-  //    [mp][nbw] float64 buffer (the one that holds Wt in the backward).
+  //    [mp][nbw] float64 buffer (the one that holds Wt in the backward).  S is
+  //    the largest power of two up to 64 that keeps kp >= mp: at M = 2,048
+  //    and 131,072-point chunks, S = 32 (float32 chunks, 4,352 workgroups, one
+  //    resident per CU) or S = 64 (float64, 8,704, two per CU), 17 full rounds
+  //    of the chip either way (65,536-point chunks left half a round idle).
   const size_t ke = ea.f32 ? sizeof(float) : sizeof(double);
   double* Pp = at<double>(ws, Lo.off_wt);
   const int64_t nbw = Lo.nbw;
   const int64_t kdiv = ea.f32 ? 2 : 1;
   int64_t S = 1;
-  while (S < 16 && (nbw / (2 * S)) % TILE == 0 && nbw / (2 * S) >= kdiv * mp) S *= 2;
+  while (S < 64 && (nbw / (2 * S)) % TILE == 0 && nbw / (2 * S) >= kdiv * mp) S *= 2;
   const int64_t kz = nbw / S, kp = kz / kdiv;
   Batch sk;
   sk.ws_stride = kz * (int64_t)ke;

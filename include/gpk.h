@@ -191,7 +191,7 @@ size_t gpk_elbo_workspace_bytes(int64_t n, int64_t m, int64_t d, int prec);
  * jitter `jitter` (CollapsedVariationalGaussian, variational_families.py:767-784).
  * The data enter through P = Kzx Kxz, b = Kzx (y - c) and q = Kzx 1, which are
  * accumulated on the matrix cores chunk by chunk (Kzx evaluated once per chunk
- * of 65,536 points into the workspace): no n x m matrix is stored.  result
+ * of 131,072 points into the workspace): no n x m matrix is stored.  result
  * slots: [GPK_RES_MLL] = ELBO, [1] = log|B|, [2] = the quadratic term, [3] = the
  * trace term, [GPK_RES_INFO] = pivot status.  d <= 32.
  * prec: GPK_PREC_FP64, or GPK_PREC_MIXED: the Kzx chunks in float32 (squared
@@ -202,7 +202,7 @@ size_t gpk_elbo_workspace_bytes(int64_t n, int64_t m, int64_t d, int prec);
  * the backward's, each 0, 3, 4 or 6 (l unused).
  * Mixed only: [GPK_RES_MLL_ERR] = 2^-24 ||G_P o P||_F / sqrt(2), a model of
  * the ELBO's error from the split products (G_P = dELBO / dP, measured errors
- * <= 1.6x it), and [GPK_RES_FLAG] = 1 when 5x it exceeds 1e-6 |ELBO|, the
+ * <= 2.9x it), and [GPK_RES_FLAG] = 1 when 5x it exceeds 1e-6 |ELBO|, the
  * factorisation broke down, or the products are ones the model is not
  * calibrated for (p other than 6 or 0, t = 3): recompute in GPK_PREC_FP64.
  * The same workspace size serves both.

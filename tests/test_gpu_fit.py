@@ -156,12 +156,12 @@ def _sparse_model(n, m, d, seed):
                                                                         inducing_inputs=Z)
 
 
-@pytest.mark.parametrize("n,precision", [(20000, "fp64"), (70000, "fp64"), (70000, "mixed")])
+@pytest.mark.parametrize("n,precision", [(20000, "fp64"), (140000, "fp64"), (140000, "mixed")])
 def test_sparse_fit_graph_replay_matches_eager(cuda, monkeypatch, n, precision):
     """fit(objective = -collapsed_elbo) over theta and the inducing inputs, eager and
     graph-captured: the ELBO forks its Kzz-side work onto a library stream and joins
     it back with events, so the captured graph holds that fork and join (and, at
-    N = 70,000, two Kzx chunks per pass).  The mixed path runs with its check off
+    N = 140,000, two Kzx chunks per pass).  The mixed path runs with its check off
     (the captured graph cannot read it on the host), so both runs take it; C4's
     recipe in eight dimensions keeps Kzz well enough conditioned for it."""
     monkeypatch.setattr(gpx.config, "mixed_check", "off")
