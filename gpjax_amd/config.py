@@ -17,9 +17,11 @@ precision = "fp64"
 # early once the residual is below 1e-12); None = library default (1 for fp64,
 # 6 for mixed).
 refine_iters = None
-# What a mixed-precision evaluation does when its a-posteriori check (the
-# Rademacher probes of I - Sigma X^T X, result slot GPK_RES_FLAG) says the
-# result may be outside 1e-6 (MLL) / 1e-4 (gradient) of the float64 answer:
+# What a mixed-precision evaluation does when its a-posteriori check (result
+# slot GPK_RES_FLAG: for the exact objectives the Rademacher probes of
+# I - Sigma X^T X, for the collapsed ELBO the error model of its split-product
+# statistics) says the result may be outside 1e-6 (objective) / 1e-4
+# (gradient) of the float64 answer:
 #   "fallback" recompute in float64 and return that (result slot FLAG = 2),
 #   "raise"    raise objectives.MixedPrecisionError,
 #   "warn"     return the mixed result with a warning,
