@@ -585,8 +585,9 @@ int mll_forward_t(int kind, const Arith3& ar, const double* x, const double* y, 
   }
   {
     PhaseScope ps(GPK_PHASE_POTRF, s);
-    if (!bt.gp && LOOKAHEAD_B > 0 && np >= 4 * LOOKAHEAD_B)
-      GPK_TRY(F.potrf_lookahead(np, LOOKAHEAD_B));
+    const int64_t la = bt.gp ? 0 : lookahead_width(np, std::is_same<TS, float>::value);
+    if (la > 0)
+      GPK_TRY(F.potrf_lookahead(np, la));
     else
       GPK_TRY(F.potrf(0, np));
   }

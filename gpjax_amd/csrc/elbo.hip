@@ -984,7 +984,7 @@ int check_elbo(int kind, int prec, const double* x, const double* y, int64_t n, 
   }
   if (ea && decode_elbo_prec(prec, ea) != GPK_OK) {
     set_error("collapsed ELBO: invalid prec 0x%x (GPK_PREC_FP64, GPK_PREC_MIXED or "
-              "GPK_PREC_MIXED_ARITH(p, t, l) with p, t in {0, 3, 6})", prec);
+              "GPK_PREC_MIXED_ARITH(p, t, l) with p, t, l in {0, 3, 4, 6})", prec);
     return GPK_ERR_BAD_ARG;
   }
   if (!x || !y || !z || !theta) {

@@ -313,8 +313,9 @@ def test_wide_inputs_parity(cuda, n, d, kind, ard, precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_lookahead_path_graph_replay(cuda, monkeypatch, precision):
-    """N = 9,000 (padded 9,088 >= 4 panels of 2,048) runs the look-ahead Cholesky,
-    whose second stream forks from and joins the calling stream through events.
+    """N = 9,000 (padded 9,088 >= 8,192) runs the look-ahead Cholesky, 35 panels
+    of 256 and a last one of 128, whose second stream forks from and joins the
+    calling stream through events.
     One forward + backward captured in a HIP graph and replayed three times, with
     eager evaluations in between, equals the eager result every time, and the
     eager result is within the gates of the oracle."""
