@@ -812,7 +812,12 @@ int elbo_forward_impl(int kind, const ElboArith& ea, const double* x, const doub
   GPK_LAUNCH_CHECK();
   Factor<double> FC{Cb, at<double>(ws, Lo.off_XC), mp, T, at<double>(ws, Lo.off_liC),
                     at<double>(ws, Lo.off_ldC), at<int>(ws, Lo.off_ifC), AR_F64, AR_F64, s};
-  GPK_TRY(FC.potrf(0, mp));
+  // The Kzz branch has joined, so the look-ahead's second stream is free here.
+  const int64_t laC = lookahead_width(mp, false);
+  if (laC > 0)
+    GPK_TRY(FC.potrf_lookahead(mp, laC));
+  else
+    GPK_TRY(FC.potrf(0, mp));
   GPK_TRY(FC.trtri(0, mp));
   // 3. c = C^-1 b = X_C^T X_C b
   GPK_TRY(launch_trmv<double>(FC.X, mp, mp, b, u, false, vpart, s));

@@ -24,25 +24,28 @@ namespace gpk {
 inline int64_t split_point(int64_t n) { return (n / NB / 2) * NB; }
 
 // Panel width of the look-ahead Cholesky (Factor::potrf_lookahead) for a single
-// np x np problem, np >= 8,192: np/32 with float32 storage, np/64 in float64,
-// rounded down to a multiple of NB, at least 2 NB.  Wider panels put more of
-// the update into the large rank-B GEMMs; narrower ones shorten the second
-// stream's chain per panel, which is what the first and last panels wait for.
-// Measured (ms per MLL + gradient, float32 storage / float64):
-//   np =  8,192: B = 2,048 15.8 / 35.1, 512 13.9 / 30.6, 256 12.9 / 28.4
-//   np = 16,384: B = 2,048 50.3 / 123.5, 1,024 45.9 / 117.5, 512 44.4 / 113.1,
-//                256 47.3 / 109.0
-//   np = 32,768: B = 2,048 227.3 / 662.5, 1,024 225.1 / 655.7, 512 239.3 / 652.5
-//   np = 65,536: B = 4,096 1,521 / 4,602, 2,048 1,525 / 4,589, 1,024 1,576 / 4,592
+// np x np problem, np >= 2,048: np/32 with float32 storage, np/64 in float64,
+// rounded down to a multiple of NB, at least NB.  Wider panels put more of the
+// update into large rank-B GEMMs; narrower ones shorten the second stream's
+// chain per panel, which the first and last panels wait for.  Measured (ms per
+// MLL + gradient, float32 storage / float64; * = the rule's choice):
+//   np =  2,048: no look-ahead 2.96 / 6.1, B = 256 2.86 / 5.5, 128 2.76* / 5.1*
+//   np =  4,096: no look-ahead 6.16 / 14.1, B = 256 5.53 / 11.7, 128 5.31* / 10.8*
+//   np =  8,192: B = 2,048 15.8 / 35.1, 512 13.9 / 30.6, 256 12.9* / 28.4,
+//                128 13.1 / 27.2*
+//   np = 16,384: B = 2,048 50.3 / 123.5, 1,024 45.9 / 117.5, 512 44.4* / 113.1,
+//                256 47.3 / 109.0*, 128 57.9 / 114.7
+//   np = 32,768: B = 2,048 227.3 / 662.5, 1,024 225.1* / 655.7, 512 239.3 / 652.5*
+//   np = 65,536: B = 4,096 1,521 / 4,602, 2,048 1,525* / 4,589, 1,024 1,576 / 4,592*
 // GPK_LOOKAHEAD_B set at build time fixes the width (0 turns the look-ahead off).
 #ifndef GPK_LOOKAHEAD_B
 #define GPK_LOOKAHEAD_B -1
 #endif
 inline int64_t lookahead_width(int64_t np, bool f32) {
   if (GPK_LOOKAHEAD_B >= 0) return np >= 4 * GPK_LOOKAHEAD_B ? GPK_LOOKAHEAD_B : 0;
-  if (np < 8192) return 0;
+  if (np < 2048) return 0;
   const int64_t b = (np / (f32 ? 32 : 64)) / NB * NB;
-  return b < 2 * NB ? 2 * NB : b;
+  return b < NB ? NB : b;
 }
 
 template <typename T>
