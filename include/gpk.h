@@ -17,8 +17,9 @@
  *     through `prec`.
  *   - Some calls fork work onto those library streams and join it back into
  *     `stream` with events before they return (the look-ahead Cholesky for
- *     n > 8,064, the batched size buckets, the collapsed ELBO's Kzz-side work
- *     during its streamed chunks): to the caller the call stays
+ *     n > 1,920, and for the collapsed ELBO's M x M factor when m > 1,920, the
+ *     batched size buckets, the collapsed ELBO's Kzz-side work during its
+ *     streamed chunks): to the caller the call stays
  *     ordered on `stream`, and under HIP-graph capture the fork and join become
  *     graph edges.
  *   - Matrices are row-major float64 with an explicit leading dimension.
