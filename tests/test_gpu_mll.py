@@ -109,6 +109,31 @@ def test_mll_grad_fp64_parity(cuda, n, d, kind, ard):
     assert np.linalg.norm(g - gr) <= 1e-8 * np.linalg.norm(gr) + 1e-12, (g, gr)
 
 
+@pytest.mark.parametrize("n,d,kind", [(1921, 3, "matern32"), (4000, 4, "rbf")])
+def test_lookahead_threshold_fp64_parity(cuda, n, d, kind):
+    """Problems padded to 2,048 and 4,096 rows, the smallest that take the
+    look-ahead Cholesky (16 and 32 panels of 128 rows, the first one ragged
+    inside its padding): MLL, log-determinant and gradient against the oracle
+    at the float64 gates."""
+    X, y, ls, var, sd = _problem(n, d, kind, True, seed=3 * n + d)
+    post, D, k = _model(cuda, X, y, kind, ls, var, sd, mean=0.2)
+    mll, res = gpx.objectives.conjugate_mll_with_info(post, D, precision="fp64")
+    mll.backward()
+    ref = O.conjugate_mll_grad(kind, X, y, ls, var, sd, mean=0.2)
+    parts = O.conjugate_mll_parts(kind, X, y, ls, var, sd, mean=0.2)
+    g = np.concatenate([np.atleast_1d(k.lengthscale.value.grad.cpu().numpy()),
+                        [k.variance.value.grad.item(),
+                         post.likelihood.obs_stddev.value.grad.item(),
+                         post.prior.mean_function.constant.value.grad.item()]])
+    gr = np.concatenate([np.atleast_1d(ref["lengthscale"]),
+                         [ref["variance"], ref["obs_stddev"], ref["mean"]]])
+    r = res.detach().cpu().numpy()
+    assert r[_lib.RES_INFO] == 0
+    assert abs(float(mll) - ref["mll"]) <= 1e-10 * abs(ref["mll"])
+    assert abs(r[_lib.RES_LOGDET] - parts["logdet"]) <= 1e-10 * abs(parts["logdet"])
+    assert np.linalg.norm(g - gr) <= 1e-8 * np.linalg.norm(gr), (g, gr)
+
+
 def _mixed_vs_oracle(cuda, X, y, kind, ls, var, sd, precision="mixed"):
     post, D, k = _model(cuda, X, y, kind, ls, var, sd)
     mll, res = gpx.objectives.conjugate_mll_with_info(post, D, precision=precision)
