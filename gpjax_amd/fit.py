@@ -112,6 +112,10 @@ def fit(*, model, objective, train_data, num_iters: int = 100, learning_rate: fl
     """Adam on objective(model, train_data); returns (model, history [num_iters])."""
     if num_iters < 1:
         raise ValueError("num_iters must be >= 1")
+    if capture_graph and warmup < 1:
+        # The first evaluation on a thread creates libgpk's own streams and
+        # events; that must not happen inside the capture.
+        raise ValueError("capture_graph needs warmup >= 1 eager steps before the capture")
     params = trainable if trainable is not None else _collect_parameters(model)
     if not params:
         raise ValueError("model has no trainable Parameter")

@@ -21,7 +21,10 @@
  *     batched size buckets, the collapsed ELBO's Kzz-side work during its
  *     streamed chunks): to the caller the call stays
  *     ordered on `stream`, and under HIP-graph capture the fork and join become
- *     graph edges.
+ *     graph edges.  The streams and events are created by a thread's first
+ *     such call on a device: a thread that captures should make one such call
+ *     uncaptured first (a warm-up step), so that no stream is created inside
+ *     the capture.
  *   - Matrices are row-major float64 with an explicit leading dimension.
  *   - Return value: GPK_OK or an error code; gpk_last_error() gives a
  *     thread-local message for the last failing call.

@@ -53,6 +53,9 @@ def test_validates_arguments():
         fit(model=m, objective=_objective, train_data=None, num_iters=0)
     with pytest.raises(ValueError):
         fit(model=object(), objective=_objective, train_data=None, num_iters=1)
+    with pytest.raises(ValueError, match="warmup"):
+        fit(model=m, objective=_objective, train_data=None, num_iters=10, capture_graph=True,
+            warmup=0)
 
 
 def test_host_parameters_are_refused():
